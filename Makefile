@@ -1,0 +1,34 @@
+# Build of the gfx950 solver library, the host-only problem generators, the
+# C++ facade tests and the CPU oracle.  `make -B` rebuilds everything.
+HIPCC    ?= /opt/rocm/bin/hipcc
+CXX      ?= g++
+ARCH     ?= gfx950
+LIBDIR   := mcls_amd/lib
+SRC      := mcls_amd/csrc
+# -ffp-contract=off: no FMA contraction, so the domain tables and walk paths
+# round exactly like the CPU restatement.  -packed-fp32-ops: no packed FP32.
+HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -ffp-contract=off \
+            -Xclang -target-feature -Xclang -packed-fp32-ops \
+            -Iinclude -I$(SRC) -Wall -Wno-unused-command-line-argument
+CXXFLAGS := -O2 -std=c++17 -fPIC -Wall -Iinclude
+
+GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip
+GPU_HDRS := $(SRC)/kernels.hpp include/mcls_gpu.h
+
+all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so oracle
+
+$(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
+	mkdir -p $(LIBDIR)
+	$(HIPCC) $(HIPFLAGS) -shared -o $@ $(GPU_SRCS)
+
+$(LIBDIR)/libmcls_problems.so: $(SRC)/problems.cpp include/mcls_problems.h
+	mkdir -p $(LIBDIR)
+	$(CXX) $(CXXFLAGS) -shared -o $@ $(SRC)/problems.cpp -lpthread
+
+oracle:
+	$(MAKE) -C oracle
+
+clean:
+	rm -rf $(LIBDIR) oracle/build
+
+.PHONY: all oracle clean

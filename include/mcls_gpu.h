@@ -1,0 +1,172 @@
+/*
+ * mcls_gpu.h -- C ABI of the MI355X (gfx950) adjoint Monte Carlo / MCSA
+ * solver.  Plain pointers and sizes only; every call returns an int status
+ * (MCLS_OK = 0) and no exception crosses the boundary.  The message of the
+ * last failure on the calling thread is returned by mcls_gpu_last_error().
+ *
+ * One context = one GPU stream holding one linear problem (matrix, optional
+ * point-Jacobi left preconditioner, the adjoint domain built from them, and
+ * the work vectors).  A context is not thread-safe; separate contexts are
+ * independent.  Vectors are host or device pointers, chosen by `ptr_kind`.
+ *
+ * Reference interfaces replaced (paths relative to the MCLS source root):
+ *   LinearProblem(A, x, b), setLeftPrec       src/MCLS_LinearProblem.hpp:74-91
+ *   TpetraPointJacobiPreconditioner           tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
+ *   AlmostOptimalDomain(A^T composite, x, pl) src/MCLS_AlmostOptimalDomain_impl.hpp:71-191,258-366
+ *   MonteCarloSolverManager<..,AdjointTag>    src/MCLS_MonteCarloSolverManager.hpp:103-220
+ *   MCSASolverManager<..,AdjointTag>::solve   src/MCLS_MCSASolverManager_impl.hpp:279-432
+ *   Teuchos::ParameterList keys               read at the lines cited per key below
+ *   SolverManager::{achievedTol,getNumIters,getConvergedStatus}
+ *                                             src/MCLS_SolverManager.hpp:60-113
+ */
+#ifndef MCLS_GPU_H
+#define MCLS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define MCLS_OK 0
+#define MCLS_ERR_INVALID_ARG 1  /* null pointer, bad size, unknown key     */
+#define MCLS_ERR_HIP 2          /* HIP runtime failure                     */
+#define MCLS_ERR_MATRIX 3       /* not square, bad/duplicate column, zero
+                                   diagonal under point Jacobi, NaN value  */
+#define MCLS_ERR_STATE 4        /* call order (no matrix / no domain)      */
+#define MCLS_ERR_UNSUPPORTED 5  /* option this build does not provide      */
+#define MCLS_ERR_NUMERIC 6      /* NaN/Inf in a vector or residual         */
+#define MCLS_ERR_NO_DEVICE 7    /* no HIP device / bad device index        */
+
+/* values dtype of set_matrix_csr */
+#define MCLS_F64 0
+#define MCLS_F32 1
+
+/* pointer kinds */
+#define MCLS_HOST 0
+#define MCLS_DEVICE 1
+
+typedef struct mcls_gpu_context mcls_gpu_context;
+
+typedef struct mcls_gpu_stats {
+  int64_t iterations;      /* MCSA iterations of the last solve (getNumIters) */
+  int64_t converged;       /* getConvergedStatus                               */
+  double achieved_tol;     /* ||rp||_2 / ||PL b||_2 (achievedTol)              */
+  double source_norm;      /* ||PL b||_2                                       */
+  int64_t histories;       /* histories simulated in the last solve            */
+  int64_t steps;           /* transitions simulated in the last solve          */
+  int64_t walk_launches;   /* walk-kernel launches in the last solve           */
+  double ms_total;         /* wall time of the last solve call                 */
+  double ms_walk;          /* HIP-event time of the walk kernel                */
+  double ms_source;        /* HIP-event time of the source build               */
+  double ms_residual;      /* HIP-event time of residual / smoother / update   */
+  double ms_domain;        /* HIP-event time of the last domain build          */
+  int64_t domain_nnz;      /* entries of the adjoint domain (rows of H^T)      */
+  int64_t solve_index;     /* MC solves performed by this context so far       */
+  double last_walk_ms;     /* HIP-event time of the most recent walk launch    */
+} mcls_gpu_stats;
+
+/* ------------------------------------------------------------- lifetime */
+/* Context on `device` with its own HIP stream (default priority). */
+int mcls_gpu_create(mcls_gpu_context** ctx, int device, unsigned flags);
+int mcls_gpu_destroy(mcls_gpu_context* ctx);
+/* Message of the last failure on this thread ("" when none). */
+const char* mcls_gpu_last_error(void);
+int mcls_gpu_synchronize(mcls_gpu_context* ctx);
+
+/* ------------------------------------------------------------ problem -- */
+/* LinearProblem operator A (src/MCLS_LinearProblem.hpp:74-91).  n x n CSR,
+ * int64 row pointers, int32 columns (each row's columns unique), values
+ * FP64 or FP32 (`val_dtype`).  The library copies the arrays (host or
+ * device, `ptr_kind`) and invalidates any built domain.  nnz < 2^31. */
+int mcls_gpu_set_matrix_csr(mcls_gpu_context* ctx, int64_t n, int64_t nnz,
+                            const int64_t* row_ptr, const int32_t* col,
+                            const void* val, int val_dtype, int ptr_kind);
+/* LinearProblem::setLeftPrec with PL = diag(dinv) (n doubles). */
+int mcls_gpu_set_left_prec_diag(mcls_gpu_context* ctx, const double* dinv,
+                                int ptr_kind);
+/* PL = diag(1/a_ii) built on the device (point Jacobi). */
+int mcls_gpu_set_left_prec_point_jacobi(mcls_gpu_context* ctx);
+int mcls_gpu_clear_left_prec(mcls_gpu_context* ctx);
+/* Copy PL's diagonal out (n doubles); MCLS_ERR_STATE without PL. */
+int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kind);
+
+/* ---------------------------------------------------------- parameters -- */
+/* Teuchos::ParameterList equivalents, by the reference's key names:
+ *   f64 "Convergence Tolerance"        (1.0)  src/MCLS_MCSASolverManager_impl.hpp:294-297
+ *   i64 "Maximum Iterations"           (1000) :324-328
+ *   i64 "Iteration Print Frequency"    (10)   :329-333 (printing needs Verbosity > 0)
+ *   i64 "Iteration Check Frequency"    (1)    :334-338
+ *   i64 "Smoother Steps"               (1)    :339-343
+ *   str "Fixed Point Type"             ("Richardson" only)  :97-101
+ *   f64 "Richardson Relaxation"        (1.0)  src/MCLS_RichardsonIteration_impl.hpp:61-64
+ *   f64 "Sample Ratio"                 (1.0)  src/MCLS_UniformAdjointSource_impl.hpp:82-86
+ *   str "Source Sampling Type"         ("Random" | "Stratified") :90-101
+ *   i64 "History Length"               (10)   src/MCLS_AlmostOptimalDomain_impl.hpp:88-91
+ *   f64 "Neumann Relaxation"           (1.0)  src/MCLS_AlmostOptimalDomain_impl.hpp:143-148
+ *   f64 "Composite Operator Threshold" (0.0)  src/MCLS_MonteCarloSolverManager_impl.hpp:325-329
+ *   str "Transport Type", i64 "MC Check Frequency", i64 "MC Buffer Size":
+ *        accepted; one GPU holds the whole domain, so there is no handoff.
+ * Build extensions (keys present in MCLS inputs but not read by src/):
+ *   f64 "Weight Cutoff"        (0 = off) relative: stop when |w| < c |w0|
+ *   str "Estimator Type"       ("Collision" | "Expected Value")
+ * Build-only keys:
+ *   i64 "Number of Histories"  (0 = N * Sample Ratio)
+ *   str "Walk Precision"       ("FP64" | "FP32")
+ *   i64 "Random Seed"          (0)
+ *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
+ *   i64 "Verbosity"            (0)
+ *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto)
+ * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
+int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
+int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);
+int mcls_gpu_set_param_str(mcls_gpu_context* ctx, const char* key, const char* v);
+
+/* --------------------------------------------------------------- domain -- */
+/* Build the adjoint domain on the device: rows of H^T with
+ * H = I - omega PL A, |h| < DBL_EPSILON dropped, per-row FP64 and FP32 CDFs,
+ * row weights W_i = sum |h_i.|.  Rebuilt automatically by the solves when
+ * the matrix, preconditioner, "Neumann Relaxation" or "Composite Operator
+ * Threshold" changed (MonteCarloSolverManager::setProblem, :161-178). */
+int mcls_gpu_build_domain(mcls_gpu_context* ctx);
+int mcls_gpu_domain_nnz(mcls_gpu_context* ctx, int64_t* nnz);
+/* Copy the domain out: rp (n+1), col (nnz), h (nnz), cdf (nnz), W (n).
+ * Any pointer may be NULL.  Host pointers. */
+int mcls_gpu_get_domain(mcls_gpu_context* ctx, int64_t* rp, int32_t* col,
+                        double* h, double* cdf, double* W);
+
+/* -------------------------------------------------------------- solves -- */
+/* MonteCarloSolverManager<..,AdjointTag>::solve (not internal): the source
+ * is omega * PL b; x receives the normalised adjoint tally.  Always
+ * "converged" (src/MCLS_MonteCarloSolverManager.hpp:170-171). */
+int mcls_gpu_mc_solve(mcls_gpu_context* ctx, const double* b, double* x,
+                      int ptr_kind);
+/* MCSASolverManager<..,AdjointTag>::solve: x is read as the initial guess
+ * and overwritten with the solution.  Returns MCLS_OK whether or not the
+ * tolerance was met; read converged from mcls_gpu_get_stats. */
+int mcls_gpu_mcsa_solve(mcls_gpu_context* ctx, const double* b, double* x,
+                        int ptr_kind);
+/* LinearProblem::updatePrecResidual: r = PL (b - A x) and ||r||_2. */
+int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
+                           const double* x, double* r, double* norm2,
+                           int ptr_kind);
+int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
+
+/* ------------------------------------------------------------- parity -- */
+/* Run the walk kernel for histories [first, first+count) of MC solve number
+ * `solve_index` with source `rhs` (already preconditioned and scaled) and
+ * record each history's state indices: paths is count x (max_len+1),
+ * padded with -1; weights receives each final weight.  Host pointers. */
+int mcls_gpu_record_paths(mcls_gpu_context* ctx, const double* rhs,
+                          uint64_t solve_index, int64_t first, int64_t count,
+                          int32_t max_len, int32_t* paths, double* weights);
+/* One internal MC solve (no preconditioning, no omega) of number
+ * `solve_index`: x = normalised tally for source rhs.  Host pointers. */
+int mcls_gpu_mc_solve_raw(mcls_gpu_context* ctx, const double* rhs,
+                          uint64_t solve_index, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,333 @@
+"""ctypes bindings to the C ABI of the gfx950 solver (include/mcls_gpu.h) and
+of the workload generators (include/mcls_problems.h).
+
+This is plumbing for the tests and the benchmark: every computation happens
+in libmcls_gpu.so.  Loading fails loudly when the library has not been built;
+there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+GPU_LIB = os.path.join(LIB_DIR, "libmcls_gpu.so")
+PROBLEMS_LIB = os.path.join(LIB_DIR, "libmcls_problems.so")
+
+MCLS_OK = 0
+MCLS_ERR_INVALID_ARG = 1
+MCLS_ERR_HIP = 2
+MCLS_ERR_MATRIX = 3
+MCLS_ERR_STATE = 4
+MCLS_ERR_UNSUPPORTED = 5
+MCLS_ERR_NUMERIC = 6
+MCLS_ERR_NO_DEVICE = 7
+MCLS_F64, MCLS_F32 = 0, 1
+MCLS_HOST, MCLS_DEVICE = 0, 1
+
+# every symbol include/mcls_gpu.h declares
+GPU_SYMBOLS = [
+    "mcls_gpu_create", "mcls_gpu_destroy", "mcls_gpu_last_error", "mcls_gpu_synchronize",
+    "mcls_gpu_set_matrix_csr", "mcls_gpu_set_left_prec_diag",
+    "mcls_gpu_set_left_prec_point_jacobi", "mcls_gpu_clear_left_prec",
+    "mcls_gpu_get_left_prec_diag", "mcls_gpu_set_param_i64", "mcls_gpu_set_param_f64",
+    "mcls_gpu_set_param_str", "mcls_gpu_build_domain", "mcls_gpu_domain_nnz",
+    "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
+    "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
+    "mcls_gpu_mc_solve_raw",
+]
+PROBLEM_SYMBOLS = ["mcls_problem_dims", "mcls_problem_build", "mcls_problem_batch_member",
+                   "mcls_problem_uniform"]
+
+POISSON2D_5PT, DIFFUSION3D_7PT, ANISO2D_5PT, VARCOEF2D_5PT, STENCIL3D_27PT = 1, 2, 3, 4, 5
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("converged", C.c_int64),
+        ("achieved_tol", C.c_double),
+        ("source_norm", C.c_double),
+        ("histories", C.c_int64),
+        ("steps", C.c_int64),
+        ("walk_launches", C.c_int64),
+        ("ms_total", C.c_double),
+        ("ms_walk", C.c_double),
+        ("ms_source", C.c_double),
+        ("ms_residual", C.c_double),
+        ("ms_domain", C.c_double),
+        ("domain_nnz", C.c_int64),
+        ("solve_index", C.c_int64),
+        ("last_walk_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ProblemSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("seed", C.c_uint64),
+        ("p0", C.c_double),
+        ("p1", C.c_double),
+        ("p2", C.c_double),
+        ("p3", C.c_double),
+    ]
+
+
+_gpu = None
+_prob = None
+
+
+def gpu_lib():
+    global _gpu
+    if _gpu is not None:
+        return _gpu
+    if not os.path.exists(GPU_LIB):
+        raise RuntimeError(f"{GPU_LIB} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(GPU_LIB)
+    vp, i64, u64, i32, f64 = C.c_void_p, C.c_int64, C.c_uint64, C.c_int, C.c_double
+    L.mcls_gpu_create.argtypes = [C.POINTER(vp), i32, C.c_uint]
+    L.mcls_gpu_destroy.argtypes = [vp]
+    L.mcls_gpu_last_error.restype = C.c_char_p
+    L.mcls_gpu_synchronize.argtypes = [vp]
+    L.mcls_gpu_set_matrix_csr.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32]
+    L.mcls_gpu_set_left_prec_diag.argtypes = [vp, vp, i32]
+    L.mcls_gpu_set_left_prec_point_jacobi.argtypes = [vp]
+    L.mcls_gpu_clear_left_prec.argtypes = [vp]
+    L.mcls_gpu_get_left_prec_diag.argtypes = [vp, vp, i32]
+    L.mcls_gpu_set_param_i64.argtypes = [vp, C.c_char_p, i64]
+    L.mcls_gpu_set_param_f64.argtypes = [vp, C.c_char_p, f64]
+    L.mcls_gpu_set_param_str.argtypes = [vp, C.c_char_p, C.c_char_p]
+    L.mcls_gpu_build_domain.argtypes = [vp]
+    L.mcls_gpu_domain_nnz.argtypes = [vp, C.POINTER(i64)]
+    L.mcls_gpu_get_domain.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.mcls_gpu_mc_solve.argtypes = [vp, vp, vp, i32]
+    L.mcls_gpu_mcsa_solve.argtypes = [vp, vp, vp, i32]
+    L.mcls_gpu_prec_residual.argtypes = [vp, vp, vp, vp, C.POINTER(f64), i32]
+    L.mcls_gpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.mcls_gpu_record_paths.argtypes = [vp, vp, u64, i64, i64, C.c_int32, vp, vp]
+    L.mcls_gpu_mc_solve_raw.argtypes = [vp, vp, u64, vp]
+    for name in GPU_SYMBOLS:
+        f = getattr(L, name)
+        if name != "mcls_gpu_last_error":
+            f.restype = i32
+    _gpu = L
+    return L
+
+
+def problems_lib():
+    global _prob
+    if _prob is not None:
+        return _prob
+    if not os.path.exists(PROBLEMS_LIB):
+        raise RuntimeError(f"{PROBLEMS_LIB} is missing: build it first")
+    L = C.CDLL(PROBLEMS_LIB)
+    vp = C.c_void_p
+    L.mcls_problem_dims.argtypes = [C.POINTER(ProblemSpec), C.POINTER(C.c_int64)]
+    L.mcls_problem_dims.restype = C.c_int64
+    L.mcls_problem_build.argtypes = [C.POINTER(ProblemSpec), vp, vp, vp, vp, vp, C.c_int]
+    L.mcls_problem_build.restype = C.c_int
+    L.mcls_problem_batch_member.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_double, C.POINTER(ProblemSpec)]
+    L.mcls_problem_batch_member.restype = C.c_int
+    L.mcls_problem_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+    L.mcls_problem_uniform.restype = C.c_double
+    _prob = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Problem:
+    """A generated system: CSR (int64 rp, int32 col, f64 and/or f32 values), b."""
+
+    def __init__(self, spec, rp, col, val64, val32, b):
+        self.spec, self.rp, self.col, self.val64, self.val32, self.b = spec, rp, col, val64, val32, b
+
+    @property
+    def n(self):
+        return len(self.rp) - 1
+
+    @property
+    def nnz(self):
+        return int(self.rp[-1])
+
+    def values(self):
+        return self.val64 if self.val64 is not None else self.val32.astype(np.float64)
+
+
+def make_spec(kind, nx, ny=0, nz=0, seed=0, p0=0.0, p1=0.0, p2=0.0, p3=0.0):
+    return ProblemSpec(kind, nx, ny, nz, seed & (2**64 - 1), p0, p1, p2, p3)
+
+
+def build_problem(spec, want64=True, want32=False, nthreads=None):
+    L = problems_lib()
+    n = C.c_int64(0)
+    nnz = L.mcls_problem_dims(C.byref(spec), C.byref(n))
+    if nnz < 0:
+        raise ValueError("bad problem spec")
+    n = n.value
+    rp = np.empty(n + 1, np.int64)
+    col = np.empty(nnz, np.int32)
+    v64 = np.empty(nnz, np.float64) if want64 else None
+    v32 = np.empty(nnz, np.float32) if want32 else None
+    b = np.empty(n, np.float64)
+    nt = nthreads if nthreads is not None else min(16, os.cpu_count() or 1)
+    rc = L.mcls_problem_build(C.byref(spec), _p(rp), _p(col), _p(v64), _p(v32), _p(b), nt)
+    if rc != 0:
+        raise ValueError("problem build failed")
+    return Problem(spec, rp, col, v64, v32, b)
+
+
+def batch_member(seed, index, lo=32, hi=256, eps_hi=100.0):
+    spec = ProblemSpec()
+    if problems_lib().mcls_problem_batch_member(seed, index, lo, hi, eps_hi, C.byref(spec)) != 0:
+        raise ValueError("bad batch member arguments")
+    return spec
+
+
+class Context:
+    """One mcls_gpu_context (one stream, one linear problem)."""
+
+    def __init__(self, device=0):
+        self.L = gpu_lib()
+        self.h = C.c_void_p()
+        self._check(self.L.mcls_gpu_create(C.byref(self.h), device, 0))
+        self.n = 0
+
+    def _check(self, rc):
+        if rc != MCLS_OK:
+            raise GpuError(rc, self.L.mcls_gpu_last_error().decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.mcls_gpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- problem
+    def set_matrix(self, rp, col, val):
+        rp = np.ascontiguousarray(rp, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val)
+        dt = MCLS_F32 if val.dtype == np.float32 else MCLS_F64
+        if dt == MCLS_F64:
+            val = val.astype(np.float64, copy=False)
+        n = len(rp) - 1
+        self._check(self.L.mcls_gpu_set_matrix_csr(self.h, n, int(rp[-1]), _p(rp), _p(col), _p(val),
+                                                   dt, MCLS_HOST))
+        self.n = n
+
+    def set_problem(self, prob, dtype="f64"):
+        val = prob.val32 if dtype == "f32" else prob.values()
+        self.set_matrix(prob.rp, prob.col, val)
+
+    def set_point_jacobi(self):
+        self._check(self.L.mcls_gpu_set_left_prec_point_jacobi(self.h))
+
+    def set_left_prec_diag(self, dinv):
+        dinv = np.ascontiguousarray(dinv, np.float64)
+        self._check(self.L.mcls_gpu_set_left_prec_diag(self.h, _p(dinv), MCLS_HOST))
+
+    def clear_left_prec(self):
+        self._check(self.L.mcls_gpu_clear_left_prec(self.h))
+
+    def get_left_prec_diag(self):
+        d = np.empty(self.n)
+        self._check(self.L.mcls_gpu_get_left_prec_diag(self.h, _p(d), MCLS_HOST))
+        return d
+
+    def set_params(self, params):
+        for k, v in params.items():
+            kb = k.encode()
+            if isinstance(v, str):
+                self._check(self.L.mcls_gpu_set_param_str(self.h, kb, v.encode()))
+            elif isinstance(v, (bool, int, np.integer)):
+                self._check(self.L.mcls_gpu_set_param_i64(self.h, kb, int(v)))
+            else:
+                self._check(self.L.mcls_gpu_set_param_f64(self.h, kb, float(v)))
+
+    # --- domain
+    def build_domain(self):
+        self._check(self.L.mcls_gpu_build_domain(self.h))
+
+    def domain(self):
+        nnz = C.c_int64(0)
+        self._check(self.L.mcls_gpu_domain_nnz(self.h, C.byref(nnz)))
+        m = nnz.value
+        rp = np.empty(self.n + 1, np.int64)
+        col = np.empty(max(m, 1), np.int32)
+        h = np.empty(max(m, 1))
+        cdf = np.empty(max(m, 1))
+        W = np.empty(self.n)
+        self._check(self.L.mcls_gpu_get_domain(self.h, _p(rp), _p(col), _p(h), _p(cdf), _p(W)))
+        return rp, col[:m], h[:m], cdf[:m], W
+
+    # --- solves
+    def mc_solve(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty(self.n)
+        self._check(self.L.mcls_gpu_mc_solve(self.h, _p(b), _p(x), MCLS_HOST))
+        return x
+
+    def mc_solve_raw(self, rhs, solve_index):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.empty(self.n)
+        self._check(self.L.mcls_gpu_mc_solve_raw(self.h, _p(rhs), int(solve_index), _p(x)))
+        return x
+
+    def mcsa_solve(self, b, x0=None):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
+        self._check(self.L.mcls_gpu_mcsa_solve(self.h, _p(b), _p(x), MCLS_HOST))
+        return x
+
+    def prec_residual(self, b, x):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        r = np.empty(self.n)
+        nrm = C.c_double(0)
+        self._check(self.L.mcls_gpu_prec_residual(self.h, _p(b), _p(x), _p(r), C.byref(nrm), MCLS_HOST))
+        return r, nrm.value
+
+    def record_paths(self, rhs, solve_index, first, count, max_len):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        paths = np.empty((count, max_len + 1), np.int32)
+        w = np.empty(count)
+        self._check(self.L.mcls_gpu_record_paths(self.h, _p(rhs), int(solve_index), int(first),
+                                                 int(count), int(max_len), _p(paths), _p(w)))
+        return paths, w
+
+    def stats(self):
+        s = Stats()
+        self._check(self.L.mcls_gpu_get_stats(self.h, C.byref(s)))
+        return s
+
+    def synchronize(self):
+        self._check(self.L.mcls_gpu_synchronize(self.h))

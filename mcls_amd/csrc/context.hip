@@ -1,0 +1,926 @@
+// C ABI of the gfx950 adjoint MC / MCSA solver (include/mcls_gpu.h).
+//
+// Host orchestration only: every arithmetic step runs in the kernels of
+// kernels.hip on the context's stream.  The MCSA outer loop follows
+// src/MCLS_MCSASolverManager_impl.hpp:279-432 (paths relative to the MCLS
+// source root); the MC solve follows src/MCLS_MCSolver_impl.hpp:83-112 and
+// src/MCLS_MonteCarloSolverManager_impl.hpp:200-235,347-380.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "mcls_gpu.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(MCLS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+#define MCLS_TRY(expr)          \
+  do {                          \
+    int rc_ = (expr);           \
+    if (rc_ != MCLS_OK) return rc_; \
+  } while (0)
+
+// Device allocation owned by the context.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int ensure(size_t count, const char* what) {
+    if (count <= n && p) return MCLS_OK;
+    release();
+    size_t c = count ? count : 1;
+    hipError_t e = hipMalloc((void**)&p, c * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(MCLS_ERR_HIP, std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
+    }
+    n = c;
+    return MCLS_OK;
+  }
+};
+
+struct Params {
+  double tol = 1.0;
+  int64_t max_iters = 1000;
+  int64_t print_freq = 10;
+  int64_t check_freq = 1;
+  int64_t smoother_steps = 1;
+  double richardson_omega = 1.0;
+  double sample_ratio = 1.0;
+  int64_t num_histories = 0;
+  int source = mcls::SRC_RANDOM;
+  int64_t history_length = 10;
+  double neumann_omega = 1.0;
+  double threshold = 0.0;
+  double weight_cutoff = 0.0;
+  int estimator = mcls::EST_COLLISION;
+  int precision = 64;
+  uint64_t seed = 0;
+  int64_t use_graph = 0;
+  int64_t verbosity = 0;
+  int64_t walk_block = 256;
+  int64_t walk_blocks_per_cu = 0;
+};
+
+}  // namespace
+
+struct mcls_gpu_context {
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  Params prm;
+
+  // matrix
+  int n = 0;
+  int64_t nnz = 0;
+  bool f32 = false;
+  bool has_matrix = false;
+  DBuf<int> a_rp, a_col;
+  DBuf<char> a_val;
+  bool has_prec = false;
+  DBuf<double> dinv;
+
+  // domain
+  bool domain_valid = false;
+  double domain_omega = 0.0, domain_threshold = 0.0;
+  int64_t d_nnz = 0;
+  DBuf<int> d_rp;
+  DBuf<uint32_t> d_col;
+  DBuf<double> d_cdf64, d_h64, d_W64;
+  DBuf<float> d_cdf32, d_h32, d_W32;
+
+  // work vectors
+  DBuf<double> x, b, r, rhs, part, scal;
+  DBuf<double> t64;
+  DBuf<float> t32;
+  DBuf<uint64_t> q, qcdf, ncnt, ncdf, scan_tmp;
+  DBuf<unsigned long long> bmax, steps;
+  DBuf<int> flags;
+  double* host_scal = nullptr;  // pinned, 8 doubles
+
+  uint64_t solve_index = 0;
+  mcls_gpu_stats stats{};
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_sig = 0;
+};
+
+namespace {
+
+using mcls::WalkArgs;
+using mcls::WalkConfig;
+
+uint64_t splitmix64_h(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+uint64_t solve_key(uint64_t seed, uint64_t solve_index) {
+  return splitmix64_h(seed ^ splitmix64_h(solve_index));
+}
+
+int check_ctx(mcls_gpu_context* c) {
+  if (!c) return fail(MCLS_ERR_INVALID_ARG, "null context");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return fail(MCLS_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  return MCLS_OK;
+}
+
+int check_matrix(mcls_gpu_context* c) {
+  if (!c->has_matrix) return fail(MCLS_ERR_STATE, "no matrix set (mcls_gpu_set_matrix_csr)");
+  return MCLS_OK;
+}
+
+hipMemcpyKind kind_to_dev(int ptr_kind) {
+  return ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+}
+hipMemcpyKind kind_from_dev(int ptr_kind) {
+  return ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+}
+
+int check_kind(int ptr_kind) {
+  if (ptr_kind != MCLS_HOST && ptr_kind != MCLS_DEVICE)
+    return fail(MCLS_ERR_INVALID_ARG, "ptr_kind must be MCLS_HOST or MCLS_DEVICE");
+  return MCLS_OK;
+}
+
+int read_flags(mcls_gpu_context* c, int out[mcls::FLAG_COUNT]) {
+  HIP_TRY(hipMemcpyAsync(out, c->flags.p, sizeof(int) * mcls::FLAG_COUNT, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCLS_OK;
+}
+
+int reset_flags(mcls_gpu_context* c) {
+  MCLS_TRY(c->flags.ensure(mcls::FLAG_COUNT, "flags"));
+  HIP_TRY(mcls::launch_fill_i32(c->flags.p, mcls::FLAG_COUNT, INT_MAX, c->stream));
+  return MCLS_OK;
+}
+
+int ensure_vectors(mcls_gpu_context* c) {
+  size_t n = (size_t)c->n;
+  MCLS_TRY(c->x.ensure(n, "x"));
+  MCLS_TRY(c->b.ensure(n, "b"));
+  MCLS_TRY(c->r.ensure(n, "r"));
+  MCLS_TRY(c->rhs.ensure(n, "rhs"));
+  MCLS_TRY(c->part.ensure((size_t)mcls::residual_blocks(c->n) + 1, "partials"));
+  MCLS_TRY(c->scal.ensure(8, "scalars"));
+  MCLS_TRY(c->q.ensure(n, "q"));
+  MCLS_TRY(c->qcdf.ensure(n, "qcdf"));
+  MCLS_TRY(c->ncnt.ensure(n, "ncnt"));
+  MCLS_TRY(c->ncdf.ensure(n, "ncdf"));
+  MCLS_TRY(c->scan_tmp.ensure(mcls::scan_tmp_elems((int64_t)n) + 1, "scan tmp"));
+  MCLS_TRY(c->bmax.ensure(1, "bmax"));
+  MCLS_TRY(c->steps.ensure(1, "steps"));
+  if (c->prm.precision == 32) {
+    if (!c->t32.p || c->t32.n < n) {
+      MCLS_TRY(c->t32.ensure(n, "tally32"));
+      HIP_TRY(hipMemsetAsync(c->t32.p, 0, n * sizeof(float), c->stream));
+    }
+  } else {
+    if (!c->t64.p || c->t64.n < n) {
+      MCLS_TRY(c->t64.ensure(n, "tally64"));
+      HIP_TRY(hipMemsetAsync(c->t64.p, 0, n * sizeof(double), c->stream));
+    }
+  }
+  return MCLS_OK;
+}
+
+void invalidate_graph(mcls_gpu_context* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_sig = 0;
+}
+
+// --------------------------------------------------------------- domain
+int build_domain(mcls_gpu_context* c) {
+  MCLS_TRY(check_matrix(c));
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  invalidate_graph(c);
+  HIP_TRY(hipEventRecord(c->ev[0], s));
+  MCLS_TRY(reset_flags(c));
+  DBuf<int> cnt, t_rp, fill, t_src, keep, itmp;
+  DBuf<double> t_val;
+  MCLS_TRY(cnt.ensure((size_t)n, "col counts"));
+  MCLS_TRY(t_rp.ensure((size_t)n + 1, "transpose rp"));
+  MCLS_TRY(fill.ensure((size_t)n, "fill"));
+  MCLS_TRY(t_src.ensure((size_t)c->nnz, "transpose src"));
+  MCLS_TRY(t_val.ensure((size_t)c->nnz, "transpose val"));
+  MCLS_TRY(keep.ensure((size_t)n, "keep"));
+  MCLS_TRY(itmp.ensure(mcls::scan_tmp_elems(n) + 1, "scan tmp"));
+  HIP_TRY(mcls::launch_fill_i32(cnt.p, n, 0, s));
+  HIP_TRY(mcls::launch_fill_i32(fill.p, n, 0, s));
+  const double* dinv = c->has_prec ? c->dinv.p : nullptr;
+  HIP_TRY(mcls::launch_count_cols(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, cnt.p, c->flags.p, s));
+  HIP_TRY(mcls::launch_scan_i32_excl(cnt.p, t_rp.p, n, itmp.p, s));
+  HIP_TRY(mcls::launch_scatter_transpose(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, fill.p,
+                                         t_src.p, t_val.p, s));
+  const double omega = c->prm.neumann_omega, thr = c->prm.threshold;
+  HIP_TRY(mcls::launch_domain_rows(n, t_rp.p, t_src.p, t_val.p, omega, thr, c->has_prec, keep.p, c->flags.p, s));
+  MCLS_TRY(c->d_rp.ensure((size_t)n + 1, "domain rp"));
+  HIP_TRY(mcls::launch_scan_i32_excl(keep.p, c->d_rp.p, n, itmp.p, s));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_BAD_COLUMN] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(flags[mcls::FLAG_BAD_COLUMN] - 1));
+  if (flags[mcls::FLAG_DUPLICATE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "duplicate column in domain row " + std::to_string(flags[mcls::FLAG_DUPLICATE] - 1));
+  if (flags[mcls::FLAG_NONFINITE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "non-finite matrix value in row " + std::to_string(flags[mcls::FLAG_NONFINITE] - 1));
+  int dnnz = 0;
+  HIP_TRY(hipMemcpy(&dnnz, c->d_rp.p + n, sizeof(int), hipMemcpyDeviceToHost));
+  c->d_nnz = dnnz;
+  size_t m = (size_t)(dnnz > 0 ? dnnz : 1);
+  MCLS_TRY(c->d_col.ensure(m, "domain col"));
+  MCLS_TRY(c->d_h64.ensure(m, "domain h64"));
+  MCLS_TRY(c->d_h32.ensure(m, "domain h32"));
+  MCLS_TRY(c->d_cdf64.ensure(m, "domain cdf64"));
+  MCLS_TRY(c->d_cdf32.ensure(m, "domain cdf32"));
+  MCLS_TRY(c->d_W64.ensure((size_t)n, "domain W64"));
+  MCLS_TRY(c->d_W32.ensure((size_t)n, "domain W32"));
+  HIP_TRY(mcls::launch_domain_fill(n, t_rp.p, t_src.p, t_val.p, omega, thr, c->has_prec, c->d_rp.p, c->d_col.p,
+                                   c->d_h64.p, c->d_h32.p, c->d_cdf64.p, c->d_cdf32.p, c->d_W64.p, c->d_W32.p, s));
+  HIP_TRY(hipEventRecord(c->ev[1], s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->stats.ms_domain = ms;
+  c->stats.domain_nnz = dnnz;
+  c->domain_valid = true;
+  c->domain_omega = omega;
+  c->domain_threshold = thr;
+  return MCLS_OK;
+}
+
+int ensure_domain(mcls_gpu_context* c) {
+  if (c->domain_valid && c->domain_omega == c->prm.neumann_omega && c->domain_threshold == c->prm.threshold)
+    return MCLS_OK;
+  return build_domain(c);
+}
+
+// ------------------------------------------------------------- MC solve
+int64_t num_histories(const mcls_gpu_context* c) {
+  if (c->prm.num_histories > 0) return c->prm.num_histories;
+  // src/MCLS_UniformAdjointSource_impl.hpp:82-86: N * Sample Ratio (int)
+  return (int64_t)((double)c->n * c->prm.sample_ratio);
+}
+
+WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
+  WalkConfig wc;
+  wc.precision = c->prm.precision;
+  wc.estimator = c->prm.estimator;
+  wc.source = c->prm.source;
+  wc.record = record;
+  wc.block = (int)c->prm.walk_block;
+  int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
+  int64_t cap = (int64_t)c->num_cus * per_cu;
+  int64_t need = (nh + wc.block - 1) / wc.block;
+  if (need < 1) need = 1;
+  wc.grid = (int)(need < cap ? need : cap);
+  return wc;
+}
+
+WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t first, int64_t last) {
+  WalkArgs a{};
+  a.d.n = c->n;
+  a.d.rp = c->d_rp.p;
+  a.d.col = c->d_col.p;
+  a.d.cdf64 = c->d_cdf64.p;
+  a.d.cdf32 = c->d_cdf32.p;
+  a.d.W64 = c->d_W64.p;
+  a.d.W32 = c->d_W32.p;
+  a.d.h64 = c->d_h64.p;
+  a.d.h32 = c->d_h32.p;
+  a.rhs = rhs;
+  a.q = c->q.p;
+  a.qcdf = c->qcdf.p;
+  a.ncdf = c->ncdf.p;
+  a.key = key;
+  a.first = first;
+  a.last = last;
+  a.history_length = c->prm.history_length;
+  a.cutoff = c->prm.weight_cutoff;
+  a.tally64 = c->prm.precision == 32 ? nullptr : c->t64.p;
+  a.tally32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  a.steps = c->steps.p;
+  return a;
+}
+
+// Source tables for rhs (device): q, qcdf (and stratified counts).
+int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  HIP_TRY(mcls::launch_fill_u64(c->bmax.p, 1, 0ULL, s));
+  HIP_TRY(mcls::launch_absmax(rhs, n, c->bmax.p, s));
+  HIP_TRY(mcls::launch_quantize(rhs, n, c->bmax.p, c->q.p, s));
+  HIP_TRY(mcls::launch_scan_u64(c->q.p, c->qcdf.p, n, c->scan_tmp.p, s));
+  if (c->prm.source == mcls::SRC_STRATIFIED) {
+    HIP_TRY(mcls::launch_strat_counts(c->q.p, n, c->qcdf.p + (n - 1), (uint64_t)nh, c->ncnt.p, s));
+    HIP_TRY(mcls::launch_scan_u64(c->ncnt.p, c->ncdf.p, n, c->scan_tmp.p, s));
+  }
+  return MCLS_OK;
+}
+
+// One MC solve of source rhs (device) into the tally; then x (+)= tally/nh.
+// accumulate: x += ..., else x = ...
+int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx) {
+  hipStream_t s = c->stream;
+  const int64_t nh = num_histories(c);
+  HIP_TRY(hipEventRecord(c->ev[2], s));
+  MCLS_TRY(build_source(c, rhs, nh));
+  HIP_TRY(hipEventRecord(c->ev[3], s));
+  const bool strat = c->prm.source == mcls::SRC_STRATIFIED;
+  const int64_t last = strat ? nh + c->n : nh;
+  WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), 0, last);
+  WalkConfig wc = walk_config(c, false, last);
+  if (nh > 0) HIP_TRY(mcls::launch_walk(a, wc, s));
+  HIP_TRY(hipEventRecord(c->ev[4], s));
+  const bool ev = c->prm.estimator == mcls::EST_EXPECTED_VALUE;
+  double inv = nh > 0 ? 1.0 / (double)nh : 0.0;
+  const uint64_t* mtot = strat ? c->ncdf.p + (c->n - 1) : nullptr;
+  double* t64 = c->prm.precision == 32 ? nullptr : c->t64.p;
+  float* t32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  if (accumulate)
+    HIP_TRY(mcls::launch_apply_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
+  else
+    HIP_TRY(mcls::launch_store_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
+  c->stats.walk_launches += nh > 0 ? 1 : 0;
+  return MCLS_OK;
+}
+
+int accumulate_mc_timing(mcls_gpu_context* c) {
+  float a = 0.f, b = 0.f;
+  HIP_TRY(hipEventElapsedTime(&a, c->ev[2], c->ev[3]));
+  HIP_TRY(hipEventElapsedTime(&b, c->ev[3], c->ev[4]));
+  c->stats.ms_source += a;
+  c->stats.ms_walk += b;
+  c->stats.last_walk_ms = b;
+  return MCLS_OK;
+}
+
+int histories_done(mcls_gpu_context* c, int64_t* out) {
+  if (c->prm.source == mcls::SRC_STRATIFIED) {
+    uint64_t M = 0;
+    HIP_TRY(hipMemcpy(&M, c->ncdf.p + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    *out = (int64_t)M;
+  } else {
+    uint64_t Q = 0;
+    HIP_TRY(hipMemcpy(&Q, c->qcdf.p + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    *out = Q ? num_histories(c) : 0;
+  }
+  return MCLS_OK;
+}
+
+// r = PL (b - A x), norm^2 into scal[slot] and host_scal[slot]
+int residual(mcls_gpu_context* c, const double* b, const double* x, double* r, int slot) {
+  hipStream_t s = c->stream;
+  const int nb = mcls::residual_blocks(c->n);
+  HIP_TRY(mcls::launch_prec_residual(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
+                                     c->has_prec ? c->dinv.p : nullptr, b, x, r, c->part.p, s));
+  HIP_TRY(mcls::launch_sum_partials(c->part.p, nb, c->scal.p + slot, s));
+  HIP_TRY(hipMemcpyAsync(c->host_scal + slot, c->scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
+  return MCLS_OK;
+}
+
+int set_param_common(mcls_gpu_context* c) {
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+bool key_is(const char* a, const char* b) { return std::strcmp(a, b) == 0; }
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* mcls_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int mcls_gpu_create(mcls_gpu_context** out, int device, unsigned flags) {
+  (void)flags;
+  if (!out) return fail(MCLS_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(MCLS_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(MCLS_ERR_NO_DEVICE, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  auto* c = new mcls_gpu_context();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (se != hipSuccess) {
+    delete c;
+    return fail(MCLS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(se));
+  }
+  for (auto& ev : c->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) {
+      mcls_gpu_destroy(c);
+      return fail(MCLS_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  if (hipHostMalloc((void**)&c->host_scal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    c->host_scal = nullptr;
+    mcls_gpu_destroy(c);
+    return fail(MCLS_ERR_HIP, "hipHostMalloc failed");
+  }
+  *out = c;
+  return MCLS_OK;
+}
+
+int mcls_gpu_destroy(mcls_gpu_context* c) {
+  if (!c) return MCLS_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  invalidate_graph(c);
+  for (auto& ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->host_scal) (void)hipHostFree(c->host_scal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return MCLS_OK;
+}
+
+int mcls_gpu_synchronize(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const int64_t* row_ptr,
+                            const int32_t* col, const void* val, int val_dtype, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  if (n <= 0 || n > INT32_MAX - 1) return fail(MCLS_ERR_INVALID_ARG, "n must be in [1, 2^31-2]");
+  if (nnz < 0 || nnz > INT32_MAX) return fail(MCLS_ERR_INVALID_ARG, "nnz must be in [0, 2^31-1]");
+  if (!row_ptr || (nnz > 0 && (!col || !val))) return fail(MCLS_ERR_INVALID_ARG, "null CSR array");
+  if (val_dtype != MCLS_F64 && val_dtype != MCLS_F32) return fail(MCLS_ERR_INVALID_ARG, "val_dtype");
+  // Row pointers are validated on the host (monotone, in range) and narrowed
+  // to int32.
+  std::vector<int64_t> rp64((size_t)n + 1);
+  HIP_TRY(hipMemcpy(rp64.data(), row_ptr, sizeof(int64_t) * (size_t)(n + 1),
+                    ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+  if (rp64[0] != 0 || rp64[(size_t)n] != nnz) return fail(MCLS_ERR_MATRIX, "row_ptr[0] != 0 or row_ptr[n] != nnz");
+  std::vector<int> rp32((size_t)n + 1);
+  for (int64_t i = 0; i <= n; ++i) {
+    if (i > 0 && rp64[(size_t)i] < rp64[(size_t)i - 1]) return fail(MCLS_ERR_MATRIX, "row_ptr not monotone");
+    rp32[(size_t)i] = (int)rp64[(size_t)i];
+  }
+  const size_t vsz = val_dtype == MCLS_F32 ? sizeof(float) : sizeof(double);
+  c->has_matrix = false;
+  c->domain_valid = false;
+  invalidate_graph(c);
+  MCLS_TRY(c->a_rp.ensure((size_t)n + 1, "A rp"));
+  MCLS_TRY(c->a_col.ensure((size_t)nnz, "A col"));
+  MCLS_TRY(c->a_val.ensure((size_t)nnz * vsz, "A val"));
+  HIP_TRY(hipMemcpy(c->a_rp.p, rp32.data(), sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice));
+  if (nnz > 0) {
+    HIP_TRY(hipMemcpy(c->a_col.p, col, sizeof(int) * (size_t)nnz, kind_to_dev(ptr_kind)));
+    HIP_TRY(hipMemcpy(c->a_val.p, val, vsz * (size_t)nnz, kind_to_dev(ptr_kind)));
+  }
+  MCLS_TRY(reset_flags(c));
+  HIP_TRY(mcls::launch_validate_csr((int)n, c->a_rp.p, c->a_col.p, c->a_val.p, val_dtype == MCLS_F32, c->flags.p,
+                                    c->stream));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_BAD_COLUMN] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(flags[mcls::FLAG_BAD_COLUMN] - 1));
+  if (flags[mcls::FLAG_NONFINITE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "non-finite matrix value in row " + std::to_string(flags[mcls::FLAG_NONFINITE] - 1));
+  c->n = (int)n;
+  c->nnz = nnz;
+  c->f32 = val_dtype == MCLS_F32;
+  c->has_matrix = true;
+  if (c->has_prec && (int64_t)c->dinv.n < n) c->has_prec = false;
+  MCLS_TRY(ensure_vectors(c));
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_left_prec_diag(mcls_gpu_context* c, const double* dinv, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!dinv) return fail(MCLS_ERR_INVALID_ARG, "null dinv");
+  MCLS_TRY(c->dinv.ensure((size_t)c->n, "dinv"));
+  HIP_TRY(hipMemcpy(c->dinv.p, dinv, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind)));
+  c->has_prec = true;
+  c->domain_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_left_prec_point_jacobi(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  MCLS_TRY(c->dinv.ensure((size_t)c->n, "dinv"));
+  MCLS_TRY(reset_flags(c));
+  HIP_TRY(mcls::launch_jacobi(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->dinv.p, c->flags.p, c->stream));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_ZERO_DIAG] != INT_MAX) {
+    c->has_prec = false;
+    return fail(MCLS_ERR_MATRIX, "zero diagonal in row " + std::to_string(flags[mcls::FLAG_ZERO_DIAG] - 1));
+  }
+  c->has_prec = true;
+  c->domain_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_clear_left_prec(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  c->has_prec = false;
+  c->domain_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_left_prec_diag(mcls_gpu_context* c, double* dinv, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  if (!c->has_prec) return fail(MCLS_ERR_STATE, "no left preconditioner");
+  if (!dinv) return fail(MCLS_ERR_INVALID_ARG, "null dinv");
+  HIP_TRY(hipMemcpy(dinv, c->dinv.p, sizeof(double) * (size_t)c->n, kind_from_dev(ptr_kind)));
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
+  if (!c || !key) return fail(MCLS_ERR_INVALID_ARG, "null context or key");
+  Params& p = c->prm;
+  if (key_is(key, "Maximum Iterations")) {
+    if (v < 0) return fail(MCLS_ERR_INVALID_ARG, "Maximum Iterations < 0");
+    p.max_iters = v;
+  } else if (key_is(key, "Iteration Print Frequency")) {
+    p.print_freq = v;
+  } else if (key_is(key, "Iteration Check Frequency")) {
+    if (v < 1) return fail(MCLS_ERR_INVALID_ARG, "Iteration Check Frequency < 1");
+    p.check_freq = v;
+  } else if (key_is(key, "Smoother Steps")) {
+    if (v < 0) return fail(MCLS_ERR_INVALID_ARG, "Smoother Steps < 0");
+    p.smoother_steps = v;
+  } else if (key_is(key, "History Length")) {
+    if (v < 1) return fail(MCLS_ERR_INVALID_ARG, "History Length < 1");
+    p.history_length = v;
+  } else if (key_is(key, "Number of Histories")) {
+    if (v < 0) return fail(MCLS_ERR_INVALID_ARG, "Number of Histories < 0");
+    p.num_histories = v;
+  } else if (key_is(key, "Random Seed")) {
+    p.seed = (uint64_t)v;
+  } else if (key_is(key, "Use Graph")) {
+    p.use_graph = v;
+  } else if (key_is(key, "Verbosity")) {
+    p.verbosity = v;
+  } else if (key_is(key, "Walk Block Size")) {
+    if (v != 64 && v != 128 && v != 256) return fail(MCLS_ERR_INVALID_ARG, "Walk Block Size must be 64, 128 or 256");
+    p.walk_block = v;
+  } else if (key_is(key, "Walk Blocks Per CU")) {
+    if (v < 0 || v > 64) return fail(MCLS_ERR_INVALID_ARG, "Walk Blocks Per CU must be in [0, 64]");
+    p.walk_blocks_per_cu = v;
+  } else if (key_is(key, "MC Check Frequency") || key_is(key, "MC Buffer Size") ||
+             key_is(key, "Overlap Size") || key_is(key, "Number of Sets")) {
+    // accepted; single domain, single set
+  } else {
+    return fail(MCLS_ERR_INVALID_ARG, std::string("unknown integer parameter '") + key + "'");
+  }
+  return set_param_common(c);
+}
+
+int mcls_gpu_set_param_f64(mcls_gpu_context* c, const char* key, double v) {
+  if (!c || !key) return fail(MCLS_ERR_INVALID_ARG, "null context or key");
+  Params& p = c->prm;
+  if (!std::isfinite(v)) return fail(MCLS_ERR_INVALID_ARG, std::string("non-finite value for '") + key + "'");
+  if (key_is(key, "Convergence Tolerance")) {
+    p.tol = v;
+  } else if (key_is(key, "Richardson Relaxation")) {
+    p.richardson_omega = v;
+  } else if (key_is(key, "Sample Ratio")) {
+    if (v < 0) return fail(MCLS_ERR_INVALID_ARG, "Sample Ratio < 0");
+    p.sample_ratio = v;
+  } else if (key_is(key, "Neumann Relaxation")) {
+    if (!(v > 0)) return fail(MCLS_ERR_INVALID_ARG, "Neumann Relaxation must be > 0");
+    p.neumann_omega = v;
+  } else if (key_is(key, "Composite Operator Threshold")) {
+    p.threshold = v;
+  } else if (key_is(key, "Weight Cutoff")) {
+    if (v < 0) return fail(MCLS_ERR_INVALID_ARG, "Weight Cutoff < 0");
+    p.weight_cutoff = v;
+  } else if (key_is(key, "History Length")) {
+    // getValidParameters declares it a double (MonteCarloSolverManager_impl:105)
+    if (v < 1) return fail(MCLS_ERR_INVALID_ARG, "History Length < 1");
+    p.history_length = (int64_t)v;
+  } else {
+    return fail(MCLS_ERR_INVALID_ARG, std::string("unknown double parameter '") + key + "'");
+  }
+  return set_param_common(c);
+}
+
+int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) {
+  if (!c || !key || !v) return fail(MCLS_ERR_INVALID_ARG, "null context, key or value");
+  Params& p = c->prm;
+  if (key_is(key, "Source Sampling Type")) {
+    if (key_is(v, "Random")) p.source = mcls::SRC_RANDOM;
+    else if (key_is(v, "Stratified")) p.source = mcls::SRC_STRATIFIED;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Source Sampling Type '") + v + "'");
+  } else if (key_is(key, "Estimator Type")) {
+    if (key_is(v, "Collision")) p.estimator = mcls::EST_COLLISION;
+    else if (key_is(v, "Expected Value")) p.estimator = mcls::EST_EXPECTED_VALUE;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Estimator Type '") + v + "'");
+  } else if (key_is(key, "Walk Precision")) {
+    if (key_is(v, "FP64")) p.precision = 64;
+    else if (key_is(v, "FP32")) p.precision = 32;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Walk Precision '") + v + "'");
+    if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
+  } else if (key_is(key, "Fixed Point Type")) {
+    if (!key_is(v, "Richardson"))
+      return fail(MCLS_ERR_UNSUPPORTED, std::string("Fixed Point Type '") + v + "' (only Richardson)");
+  } else if (key_is(key, "Transport Type")) {
+    if (!key_is(v, "Global") && !key_is(v, "Subdomain"))
+      return fail(MCLS_ERR_INVALID_ARG, std::string("Transport Type '") + v + "'");
+  } else {
+    return fail(MCLS_ERR_INVALID_ARG, std::string("unknown string parameter '") + key + "'");
+  }
+  return set_param_common(c);
+}
+
+int mcls_gpu_build_domain(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  return build_domain(c);
+}
+
+int mcls_gpu_domain_nnz(mcls_gpu_context* c, int64_t* nnz) {
+  MCLS_TRY(check_ctx(c));
+  if (!nnz) return fail(MCLS_ERR_INVALID_ARG, "null nnz");
+  if (!c->domain_valid) return fail(MCLS_ERR_STATE, "domain not built");
+  *nnz = c->d_nnz;
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_domain(mcls_gpu_context* c, int64_t* rp, int32_t* col, double* h, double* cdf, double* W) {
+  MCLS_TRY(check_ctx(c));
+  if (!c->domain_valid) return fail(MCLS_ERR_STATE, "domain not built");
+  const size_t n = (size_t)c->n, m = (size_t)c->d_nnz;
+  if (rp) {
+    std::vector<int> tmp(n + 1);
+    HIP_TRY(hipMemcpy(tmp.data(), c->d_rp.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i <= n; ++i) rp[i] = tmp[i];
+  }
+  if (col && m) {
+    std::vector<uint32_t> tmp(m);
+    HIP_TRY(hipMemcpy(tmp.data(), c->d_col.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < m; ++k) col[k] = (int32_t)(tmp[k] & 0x7FFFFFFFu);
+  }
+  if (h && m) HIP_TRY(hipMemcpy(h, c->d_h64.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+  if (cdf && m) HIP_TRY(hipMemcpy(cdf, c->d_cdf64.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+  if (W) HIP_TRY(hipMemcpy(W, c->d_W64.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return MCLS_OK;
+}
+
+int mcls_gpu_prec_residual(mcls_gpu_context* c, const double* b, const double* x, double* r, double* norm2,
+                           int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  const size_t bytes = sizeof(double) * (size_t)c->n;
+  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), c->stream));
+  HIP_TRY(hipMemcpyAsync(c->x.p, x, bytes, kind_to_dev(ptr_kind), c->stream));
+  MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
+  if (r) HIP_TRY(hipMemcpyAsync(r, c->r.p, bytes, kind_from_dev(ptr_kind), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (norm2) *norm2 = std::sqrt(c->host_scal[0]);
+  return MCLS_OK;
+}
+
+int mcls_gpu_mc_solve_raw(mcls_gpu_context* c, const double* rhs, uint64_t solve_index, double* x) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  if (!rhs || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  const size_t bytes = sizeof(double) * (size_t)c->n;
+  c->stats.walk_launches = 0;
+  c->stats.ms_walk = c->stats.ms_source = 0.0;
+  HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->rhs.p, rhs, bytes, hipMemcpyHostToDevice, c->stream));
+  MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index));
+  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long steps = 0;
+  HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  MCLS_TRY(accumulate_mc_timing(c));
+  c->stats.steps = (int64_t)steps;
+  MCLS_TRY(histories_done(c, &c->stats.histories));
+  return MCLS_OK;
+}
+
+int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve_index, int64_t first,
+                          int64_t count, int32_t max_len, int32_t* paths, double* weights) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  if (!rhs || !paths || !weights || count <= 0 || first < 0 || max_len < 0)
+    return fail(MCLS_ERR_INVALID_ARG, "bad record_paths arguments");
+  if (count > (int64_t)1 << 24 || (int64_t)count * (max_len + 1) > ((int64_t)1 << 28))
+    return fail(MCLS_ERR_INVALID_ARG, "record_paths request too large");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  hipStream_t s = c->stream;
+  const size_t bytes = sizeof(double) * (size_t)c->n;
+  DBuf<int> dpaths;
+  DBuf<double> dw;
+  MCLS_TRY(dpaths.ensure((size_t)count * (size_t)(max_len + 1), "paths"));
+  MCLS_TRY(dw.ensure((size_t)count, "weights"));
+  HIP_TRY(mcls::launch_fill_i32(dpaths.p, count * (int64_t)(max_len + 1), -1, s));
+  HIP_TRY(mcls::launch_fill_f64(dw.p, count, 0.0, s));
+  HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
+  HIP_TRY(hipMemcpyAsync(c->rhs.p, rhs, bytes, hipMemcpyHostToDevice, s));
+  const int64_t nh = num_histories(c);
+  MCLS_TRY(build_source(c, c->rhs.p, nh));
+  WalkArgs a = walk_args(c, c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
+  a.paths = dpaths.p;
+  a.wfinal = dw.p;
+  a.max_len = max_len;
+  WalkConfig wc = walk_config(c, true, count);
+  HIP_TRY(mcls::launch_walk(a, wc, s));
+  // clear the tally the recorded walks wrote
+  double* t64 = c->prm.precision == 32 ? nullptr : c->t64.p;
+  float* t32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  HIP_TRY(mcls::launch_store_tally(c->n, c->x.p, t64, t32, c->rhs.p, false, 0.0, nullptr, s));
+  HIP_TRY(hipMemcpyAsync(paths, dpaths.p, sizeof(int) * (size_t)count * (size_t)(max_len + 1),
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(weights, dw.p, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return MCLS_OK;
+}
+
+int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_kind) {
+  auto t0 = std::chrono::steady_clock::now();
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  hipStream_t s = c->stream;
+  const size_t bytes = sizeof(double) * (size_t)c->n;
+  c->stats.walk_launches = 0;
+  c->stats.ms_walk = c->stats.ms_source = c->stats.ms_residual = 0.0;
+  HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
+  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), s));
+  // buildMonteCarloSource (src/MCLS_MonteCarloSolverManager_impl.hpp:347-380):
+  // rhs = omega * PL b
+  if (c->has_prec) {
+    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, c->b.p, c->rhs.p, s));
+    // rhs_i = dinv_i * b_i via the residual kernel would need x = 0; do it directly
+    HIP_TRY(mcls::launch_fill_f64(c->x.p, c->n, 0.0, s));
+    MCLS_TRY(residual(c, c->b.p, c->x.p, c->rhs.p, 1));  // PL (b - A 0) = PL b
+  } else {
+    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, c->b.p, c->rhs.p, s));
+  }
+  if (c->prm.neumann_omega != 1.0) HIP_TRY(mcls::launch_scale_copy(c->n, c->prm.neumann_omega, c->rhs.p, c->rhs.p, s));
+  const uint64_t sidx = c->solve_index++;
+  MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, sidx));
+  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
+  unsigned long long steps = 0;
+  HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  MCLS_TRY(accumulate_mc_timing(c));
+  c->stats.steps = (int64_t)steps;
+  MCLS_TRY(histories_done(c, &c->stats.histories));
+  c->stats.iterations = 0;
+  c->stats.converged = 1;
+  c->stats.solve_index = (int64_t)c->solve_index;
+  c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_kind) {
+  auto t0 = std::chrono::steady_clock::now();
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  const Params& p = c->prm;
+  mcls_gpu_stats& st = c->stats;
+  st.walk_launches = 0;
+  st.ms_walk = st.ms_source = st.ms_residual = 0.0;
+  st.histories = st.steps = 0;
+  HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
+  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), s));
+  HIP_TRY(hipMemcpyAsync(c->x.p, x, bytes, kind_to_dev(ptr_kind), s));
+
+  // src/MCLS_MCSASolverManager_impl.hpp:300-318: ||PL b||_2, homogeneous case
+  HIP_TRY(mcls::launch_scaled_sq(n, c->has_prec ? c->dinv.p : nullptr, c->b.p, c->part.p, s));
+  HIP_TRY(mcls::launch_sum_partials(c->part.p, mcls::residual_blocks(n), c->scal.p + 2, s));
+  HIP_TRY(hipMemcpyAsync(c->host_scal + 2, c->scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
+  HIP_TRY(hipStreamSynchronize(s));
+  double source_norm = std::sqrt(c->host_scal[2]);
+  if (std::fabs(source_norm) < 10.0 * DBL_EPSILON) source_norm = 1.0;
+  const double crit = p.tol * source_norm;
+  double rnorm = std::sqrt(c->host_scal[0]);
+  if (!std::isfinite(rnorm)) return fail(MCLS_ERR_NUMERIC, "non-finite initial residual");
+  if (p.verbosity > 0) {
+    std::printf("MCLS MCSA (gfx950): N=%d domain nnz=%lld histories/iter=%lld\n", n, (long long)c->d_nnz,
+                (long long)num_histories(c));
+  }
+  int64_t iters = 0;
+  bool go = true;
+  const int64_t nh = num_histories(c);
+  while (go) {
+    ++iters;
+    HIP_TRY(hipEventRecord(c->ev[5], s));
+    for (int64_t l = 0; l < p.smoother_steps; ++l) {
+      // RichardsonIteration::doOneIteration: x += omega rp; rp = PL(b - A x)
+      HIP_TRY(mcls::launch_axpy(n, p.richardson_omega, c->r.p, c->x.p, s));
+      const int nb = mcls::residual_blocks(n);
+      HIP_TRY(mcls::launch_prec_residual(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
+                                         c->has_prec ? c->dinv.p : nullptr, c->b.p, c->x.p, c->r.p, c->part.p, s));
+      (void)nb;
+    }
+    // buildMonteCarloSource with internal solver: rhs = omega * rp
+    HIP_TRY(mcls::launch_scale_copy(n, p.neumann_omega, c->r.p, c->rhs.p, s));
+    HIP_TRY(hipEventRecord(c->ev[6], s));
+    const uint64_t sidx = c->solve_index++;
+    MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, true, sidx));  // x += delta x
+    MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
+    HIP_TRY(hipEventRecord(c->ev[7], s));
+    HIP_TRY(hipStreamSynchronize(s));
+    MCLS_TRY(accumulate_mc_timing(c));
+    float a = 0.f, b2 = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
+    HIP_TRY(hipEventElapsedTime(&b2, c->ev[4], c->ev[7]));
+    st.ms_residual += a + b2;
+    int64_t hd = 0;
+    if (p.source == mcls::SRC_STRATIFIED) MCLS_TRY(histories_done(c, &hd));
+    else hd = nh;
+    st.histories += hd;
+    rnorm = std::sqrt(c->host_scal[0]);
+    if (!std::isfinite(rnorm)) {
+      st.iterations = iters;
+      st.converged = 0;
+      return fail(MCLS_ERR_NUMERIC, "non-finite residual at MCSA iteration " + std::to_string(iters));
+    }
+    if (iters % p.check_freq == 0) go = (rnorm > crit) && (iters < p.max_iters);
+    if (p.verbosity > 0 && ((p.print_freq > 0 && iters % p.print_freq == 0) || !go))
+      std::printf("%18lld%18.4e\n", (long long)iters, rnorm / source_norm);
+  }
+  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
+  unsigned long long steps = 0;
+  HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  st.steps = (int64_t)steps;
+  st.iterations = iters;
+  st.converged = rnorm <= crit;
+  st.achieved_tol = rnorm / source_norm;
+  st.source_norm = source_norm;
+  st.solve_index = (int64_t)c->solve_index;
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_stats(mcls_gpu_context* c, mcls_gpu_stats* out) {
+  if (!c || !out) return fail(MCLS_ERR_INVALID_ARG, "null argument");
+  *out = c->stats;
+  return MCLS_OK;
+}
+
+}  // extern "C"

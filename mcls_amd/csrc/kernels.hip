@@ -1,0 +1,797 @@
+// gfx950 kernels for the adjoint Neumann-Ulam / MCSA path.
+//
+// Built with -ffp-contract=off: every floating-point operation rounds as
+// written, so the domain tables, source weights and walk paths are
+// bit-identical to the CPU restatement that checks them.
+//
+// Reference semantics followed (paths relative to the MCLS source root):
+//   src/MCLS_AlmostOptimalDomain_impl.hpp:258-366  domain rows (h, eps drop, CDF, W)
+//   src/MCLS_AlmostOptimalDomain.hpp:255-284       processTransition
+//   src/MCLS_DomainTransporter_impl.hpp:85-115     tally -> transition -> terminate
+//   src/MCLS_AdjointTally.hpp:115-123              collision tally
+//   src/MCLS_SamplingTools.hpp:67-80               lower_bound CDF inversion
+//   src/MCLS_Xorshift.hpp:103-109                  Xorshift (13,7,17)
+//   src/MCLS_LinearProblem_impl.hpp:411-442        rp = PL (b - A x)
+//   tpetra/src/MCLS_TpetraHelpers.hpp:196-263      composite threshold filter
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace mcls {
+namespace {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += kGolden;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t xorshift64(uint64_t& x) {
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t history_state(uint64_t key, uint64_t hid) {
+  uint64_t s = splitmix64(key ^ splitmix64(hid));
+  return s ? s : kGolden;
+}
+
+__device__ __forceinline__ double u01_f64(uint64_t r) { return (double)(r >> 11) * 0x1.0p-53; }
+__device__ __forceinline__ float u01_f32(uint64_t r) { return (float)(r >> 40) * 0x1.0p-24f; }
+
+// floor((r >> 11) * Q / 2^53)
+__device__ __forceinline__ uint64_t mul_u53(uint64_t r, uint64_t Q) {
+  uint64_t a = r >> 11;
+  uint64_t lo = a * Q;
+  uint64_t hi = __umul64hi(a, Q);
+  return (hi << 11) | (lo >> 53);
+}
+
+// first k in [0, n) with cdf[k] > t
+__device__ __forceinline__ int upper_bound_u64(const uint64_t* __restrict__ cdf, int n, uint64_t t) {
+  int lo = 0, len = n;
+  while (len > 0) {
+    int half = len >> 1;
+    if (cdf[lo + half] <= t) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+
+// first k in [0, n) with cdf[k] >= u  (std::lower_bound)
+template <typename Real>
+__device__ __forceinline__ int lower_bound_cdf(const Real* __restrict__ cdf, int n, Real u) {
+  int lo = 0, len = n;
+  while (len > 0) {
+    int half = len >> 1;
+    if (cdf[lo + half] < u) {
+      lo += half + 1;
+      len -= half + 1;
+    } else {
+      len = half;
+    }
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void tally_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void tally_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+template <typename Real> struct Tables;
+template <> struct Tables<double> {
+  __device__ static const double* cdf(const DomainView& d) { return d.cdf64; }
+  __device__ static const double* W(const DomainView& d) { return d.W64; }
+  __device__ static const double* h(const DomainView& d) { return d.h64; }
+  __device__ static double* tally(const WalkArgs& a) { return a.tally64; }
+  __device__ static double u01(uint64_t r) { return u01_f64(r); }
+};
+template <> struct Tables<float> {
+  __device__ static const float* cdf(const DomainView& d) { return d.cdf32; }
+  __device__ static const float* W(const DomainView& d) { return d.W32; }
+  __device__ static const float* h(const DomainView& d) { return d.h32; }
+  __device__ static float* tally(const WalkArgs& a) { return a.tally32; }
+  __device__ static float u01(uint64_t r) { return u01_f32(r); }
+};
+
+// --------------------------------------------------------------------------
+// Persistent walk kernel.  Each lane owns one live history at a time and
+// refills itself from a static grid-stride schedule the moment its history
+// ends, so lanes never wait on the longest walk of their wave.  Histories
+// follow the reference loop order: tally at the current state, sample the
+// transition from the row CDF, multiply the weight by W_i sign(h_ij), stop
+// when steps >= L or |w| < cutoff |w0|.
+// --------------------------------------------------------------------------
+template <typename Real, int EST, int SRC, bool REC>
+__global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
+  const DomainView& d = a.d;
+  const int n = d.n;
+  const Real* __restrict__ cdf = Tables<Real>::cdf(d);
+  const Real* __restrict__ Wt = Tables<Real>::W(d);
+  const Real* __restrict__ ht = Tables<Real>::h(d);
+  Real* __restrict__ tally = Tables<Real>::tally(a);
+  const int* __restrict__ rp = d.rp;
+  const uint32_t* __restrict__ colp = d.col;
+
+  int64_t last = a.last;
+  uint64_t total = 0;  // Q (random) or M (stratified)
+  if (SRC == SRC_RANDOM) {
+    total = a.qcdf[n - 1];
+    if (total == 0) return;
+  } else {
+    total = a.ncdf[n - 1];
+    if ((int64_t)total < last) last = (int64_t)total;
+  }
+  const double totald = (double)total;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t hid = a.first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x - stride;
+
+  bool alive = false;
+  int state = 0;
+  Real w = 0, wc = 0;
+  int64_t steps = 0;
+  uint64_t rng = 0;
+  uint64_t my_steps = 0;
+  int* path = nullptr;
+
+  for (;;) {
+    if (!alive) {
+      if (REC && path) {
+        for (int k = (int)steps + 1; k <= a.max_len; ++k) path[k] = -1;
+        a.wfinal[hid - a.first] = (double)w;
+      }
+      hid += stride;
+      if (hid >= last) break;
+      rng = history_state(a.key, (uint64_t)hid);
+      double w0;
+      if (SRC == SRC_RANDOM) {
+        uint64_t t = mul_u53(xorshift64(rng), total);
+        state = upper_bound_u64(a.qcdf, n, t);
+        w0 = a.rhs[state] * (totald / (double)a.q[state]);
+      } else {
+        state = upper_bound_u64(a.ncdf, n, (uint64_t)hid);
+        uint64_t nk = a.ncdf[state] - (state ? a.ncdf[state - 1] : 0);
+        w0 = a.rhs[state] * (totald / (double)nk);
+      }
+      w = (Real)w0;
+      wc = (Real)a.cutoff * (w < 0 ? -w : w);
+      steps = 0;
+      alive = true;
+      if (REC) {
+        path = a.paths + (size_t)(hid - a.first) * (size_t)(a.max_len + 1);
+        path[0] = state;
+      }
+    }
+    const int beg = rp[state];
+    const int len = rp[state + 1] - beg;
+    if (EST == EST_EXPECTED_VALUE) {
+      for (int k = 0; k < len; ++k)
+        tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], w * ht[beg + k]);
+    } else {
+      tally_add(&tally[state], w);
+    }
+    if (len == 0) {  // absorbing row (all of h dropped)
+      alive = false;
+      continue;
+    }
+    const Real u = Tables<Real>::u01(xorshift64(rng));
+    const int k = lower_bound_cdf<Real>(cdf + beg, len, u);
+    const uint32_t cp = colp[beg + k];
+    w = w * Wt[state];
+    if (cp & 0x80000000u) w = -w;
+    state = (int)(cp & 0x7FFFFFFFu);
+    ++steps;
+    ++my_steps;
+    if (REC && steps <= a.max_len) path[steps] = state;
+    if (steps >= a.history_length || (w < 0 ? -w : w) < wc) alive = false;
+  }
+  if (my_steps) atomicAdd(a.steps, (unsigned long long)my_steps);
+}
+
+template <typename Real, int EST, int SRC>
+hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if (c.record)
+    k_walk<Real, EST, SRC, true><<<c.grid, c.block, 0, s>>>(a);
+  else
+    k_walk<Real, EST, SRC, false><<<c.grid, c.block, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <typename Real>
+hipError_t launch_walk_p(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if (c.estimator == EST_EXPECTED_VALUE) {
+    return c.source == SRC_STRATIFIED ? launch_walk_t<Real, EST_EXPECTED_VALUE, SRC_STRATIFIED>(a, c, s)
+                                      : launch_walk_t<Real, EST_EXPECTED_VALUE, SRC_RANDOM>(a, c, s);
+  }
+  return c.source == SRC_STRATIFIED ? launch_walk_t<Real, EST_COLLISION, SRC_STRATIFIED>(a, c, s)
+                                    : launch_walk_t<Real, EST_COLLISION, SRC_RANDOM>(a, c, s);
+}
+
+// ------------------------------------------------------------------ source
+__global__ void k_absmax(const double* __restrict__ v, int n, unsigned long long* bmax) {
+  __shared__ unsigned long long sm[kRedThreads];
+  unsigned long long m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double a = fabs(v[i]);
+    unsigned long long bits = (unsigned long long)__double_as_longlong(a);
+    if (bits > m) m = bits;  // non-negative doubles order as integers; NaN is largest
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off && sm[threadIdx.x + off] > sm[threadIdx.x]) sm[threadIdx.x] = sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sm[0]) atomicMax(bmax, sm[0]);
+}
+
+__device__ __forceinline__ int ceil_log2_u64(uint64_t v) {
+  int l = 0;
+  while (((uint64_t)1 << l) < v) ++l;
+  return l;
+}
+
+__global__ void k_quantize(const double* __restrict__ rhs, int n, const unsigned long long* bmax_bits,
+                           uint64_t* __restrict__ q) {
+  const double bmax = __longlong_as_double((long long)*bmax_bits);
+  const int S = 62 - ceil_log2_u64((uint64_t)(n > 0 ? n : 1));
+  int e = 0;
+  bool nonzero = bmax > 0.0 && bmax <= DBL_MAX;
+  if (nonzero) (void)frexp(bmax, &e);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint64_t qi = 0;
+    if (nonzero) qi = (uint64_t)ceil(ldexp(fabs(rhs[i]), S - e));
+    q[i] = qi;
+  }
+}
+
+// floor(num / Q) for num = hi:lo, quotient known to be < 2^53.
+__device__ __forceinline__ uint64_t div128_small(uint64_t hi, uint64_t lo, uint64_t Q) {
+  double numd = (double)hi * 0x1.0p64 + (double)lo;
+  uint64_t e = (uint64_t)(numd / (double)Q);
+  // correct e so that e*Q <= num < (e+1)*Q
+  for (int it = 0; it < 8; ++it) {
+    uint64_t plo = e * Q, phi = __umul64hi(e, Q);
+    bool gt = (phi > hi) || (phi == hi && plo > lo);  // e*Q > num
+    if (gt) {
+      --e;
+      continue;
+    }
+    uint64_t nlo = plo + Q, nhi = phi + (nlo < plo);
+    bool le = (nhi < hi) || (nhi == hi && nlo <= lo);  // (e+1)*Q <= num
+    if (le) {
+      ++e;
+      continue;
+    }
+    break;
+  }
+  return e;
+}
+
+__global__ void k_strat_counts(const uint64_t* __restrict__ q, int n, const uint64_t* qtotal, uint64_t nh,
+                               uint64_t* __restrict__ cnt) {
+  const uint64_t Q = *qtotal;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint64_t c = 0;
+    if (Q) {
+      uint64_t lo = q[i] * nh, hi = __umul64hi(q[i], nh);
+      uint64_t lo2 = lo + (Q - 1);
+      hi += (lo2 < lo);
+      c = div128_small(hi, lo2, Q);
+    }
+    cnt[i] = c;
+  }
+}
+
+// -------------------------------------------------------------------- scan
+template <typename T>
+__device__ T block_excl_scan(T v, T* sm, T* total) {
+  // Hillis-Steele over kScanThreads lanes; exact for integers.
+  const int t = threadIdx.x;
+  sm[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    T add = t >= off ? sm[t - off] : T(0);
+    __syncthreads();
+    sm[t] += add;
+    __syncthreads();
+  }
+  T incl = sm[t];
+  *total = sm[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const T* __restrict__ in, int64_t n,
+                                                                 T* __restrict__ sums) {
+  __shared__ T sm[kScanThreads];
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  T acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + (int64_t)k * kScanThreads + threadIdx.x;
+    if (i < n) acc += in[i];
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kScanThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = sm[0];
+}
+
+// exclusive scan of sums[0..m) in place, single block
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(T* sums, int64_t m) {
+  __shared__ T sm[kScanThreads];
+  __shared__ T carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < m; base += kScanThreads) {
+    int64_t i = base + threadIdx.x;
+    T v = i < m ? sums[i] : T(0);
+    T tot;
+    T ex = block_excl_scan<T>(v, sm, &tot);
+    T carry = carry_s;
+    if (i < m) sums[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + tot;
+    __syncthreads();
+  }
+}
+
+// inclusive scan of each tile plus its tile offset
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T* __restrict__ in, int64_t n,
+                                                             const T* __restrict__ offs, T* out) {
+  __shared__ T sm[kScanThreads];
+  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  T v[kScanItems];
+  T local = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    v[k] = i < n ? in[i] : T(0);
+    local += v[k];
+  }
+  T tot;
+  T ex = block_excl_scan<T>(local, sm, &tot);
+  T run = offs[blockIdx.x] + ex;
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    run += v[k];
+    if (i < n) out[i] = run;
+  }
+}
+
+// --------------------------------------------------------------- fills
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+inline int grid_for(int64_t n, int block, int cap = 65536) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// --------------------------------------------------------------- domain
+template <typename AV>
+__device__ __forceinline__ double aval(const void* v, int k) {
+  return (double)((const AV*)v)[k];
+}
+
+__device__ __forceinline__ void set_flag(int* flags, int which, int row) {
+  atomicMin(&flags[which], row + 1);  // flags initialised to INT_MAX
+}
+
+template <typename AV>
+__global__ void k_jacobi(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                         double* __restrict__ dinv, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double dg = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (col[k] == i) dg = aval<AV>(val, k);
+    if (dg == 0.0) set_flag(flags, FLAG_ZERO_DIAG, i);
+    dinv[i] = 1.0 / dg;
+  }
+}
+
+template <typename AV>
+__global__ void k_validate_csr(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                               int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      int c = col[k];
+      if (c < 0 || c >= n) set_flag(flags, FLAG_BAD_COLUMN, i);
+      if (!(fabs(aval<AV>(val, k)) <= DBL_MAX)) set_flag(flags, FLAG_NONFINITE, i);
+    }
+  }
+}
+
+template <typename AV>
+__global__ void k_count_cols(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                             int* __restrict__ cnt, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      int c = col[k];
+      double v = aval<AV>(val, k);
+      if (!(fabs(v) <= DBL_MAX)) set_flag(flags, FLAG_NONFINITE, i);
+      if (c < 0 || c >= n) {
+        set_flag(flags, FLAG_BAD_COLUMN, i);
+        continue;
+      }
+      atomicAdd(&cnt[c], 1);
+    }
+  }
+}
+
+template <typename AV>
+__global__ void k_scatter_transpose(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const void* val, const double* __restrict__ dinv,
+                                    const int* __restrict__ t_rp, int* __restrict__ fill,
+                                    int* __restrict__ t_src, double* __restrict__ t_val) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      int c = col[k];
+      if (c < 0 || c >= n) continue;
+      int p = t_rp[c] + atomicAdd(&fill[c], 1);
+      t_src[p] = r;
+      double a = aval<AV>(val, k);
+      t_val[p] = dinv ? a * dinv[r] : a;  // composite (PL A)_{rc}
+    }
+  }
+}
+
+// Per domain row: order entries by source row (insertion sort; the scatter
+// order is arbitrary), check duplicates, count the entries that survive the
+// composite threshold and the DBL_EPSILON drop.
+__global__ void k_domain_rows(int n, const int* __restrict__ t_rp, int* __restrict__ t_src,
+                              double* __restrict__ t_val, double omega, double threshold, int has_prec,
+                              int* __restrict__ keep, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = t_rp[i], end = t_rp[i + 1];
+    for (int p = beg + 1; p < end; ++p) {
+      int s = t_src[p];
+      double v = t_val[p];
+      int q = p - 1;
+      while (q >= beg && t_src[q] > s) {
+        t_src[q + 1] = t_src[q];
+        t_val[q + 1] = t_val[q];
+        --q;
+      }
+      t_src[q + 1] = s;
+      t_val[q + 1] = v;
+    }
+    double thr = 0.0;
+    const bool filter = has_prec && end > beg;
+    if (filter) {
+      double mx = t_val[beg];
+      for (int p = beg + 1; p < end; ++p)
+        if (t_val[p] > mx) mx = t_val[p];
+      thr = mx * threshold;
+    }
+    int kept = 0;
+    for (int p = beg; p < end; ++p) {
+      if (p > beg && t_src[p] == t_src[p - 1]) set_flag(flags, FLAG_DUPLICATE, i);
+      double m = t_val[p];
+      if (filter && fabs(m) <= thr) continue;
+      double h = (t_src[p] == i) ? 1.0 - omega * m : -omega * m;
+      if (fabs(h) < DBL_EPSILON) continue;
+      ++kept;
+    }
+    keep[i] = kept;
+  }
+}
+
+__global__ void k_domain_fill(int n, const int* __restrict__ t_rp, const int* __restrict__ t_src,
+                              const double* __restrict__ t_val, double omega, double threshold,
+                              int has_prec, const int* __restrict__ d_rp, uint32_t* __restrict__ col,
+                              double* __restrict__ h64, float* __restrict__ h32,
+                              double* __restrict__ cdf64, float* __restrict__ cdf32,
+                              double* __restrict__ W64, float* __restrict__ W32) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = t_rp[i], end = t_rp[i + 1];
+    double thr = 0.0;
+    const bool filter = has_prec && end > beg;
+    if (filter) {
+      double mx = t_val[beg];
+      for (int p = beg + 1; p < end; ++p)
+        if (t_val[p] > mx) mx = t_val[p];
+      thr = mx * threshold;
+    }
+    int out = d_rp[i];
+    const int obeg = out;
+    double acc = 0.0;
+    for (int p = beg; p < end; ++p) {
+      double m = t_val[p];
+      if (filter && fabs(m) <= thr) continue;
+      int j = t_src[p];
+      double h = (j == i) ? 1.0 - omega * m : -omega * m;
+      if (fabs(h) < DBL_EPSILON) continue;
+      col[out] = (uint32_t)j | (h < 0.0 ? 0x80000000u : 0u);
+      h64[out] = h;
+      h32[out] = (float)h;
+      acc = fabs(h) + acc;
+      cdf64[out] = acc;
+      ++out;
+    }
+    W64[i] = acc;
+    W32[i] = (float)acc;
+    for (int p = obeg; p < out; ++p) {
+      double c = cdf64[p] / acc;
+      cdf64[p] = c;
+      cdf32[p] = (float)c;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- linalg
+template <typename AV>
+__global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int* __restrict__ rp,
+                                                               const int* __restrict__ col, const void* val,
+                                                               const double* __restrict__ dinv,
+                                                               const double* __restrict__ b,
+                                                               const double* __restrict__ x,
+                                                               double* __restrict__ r, double* part) {
+  __shared__ double sm[kRedThreads];
+  const AV* __restrict__ v = (const AV*)val;
+  double sq = 0.0;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + (double)v[k] * x[col[k]];
+    double t = b[i] - s;
+    double ri = dinv ? dinv[i] * t : t;
+    r[i] = ri;
+    sq = ri * ri;
+  }
+  sm[threadIdx.x] = sq;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_scaled_sq(int n, const double* __restrict__ dinv,
+                                                           const double* __restrict__ b, double* part) {
+  __shared__ double sm[kRedThreads];
+  double sq = 0.0;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    double t = dinv ? dinv[i] * b[i] : b[i];
+    sq = t * t;
+  }
+  sm[threadIdx.x] = sq;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// fixed-order reduction of m partials by one block
+__global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __restrict__ part, int m,
+                                                              double* out) {
+  __shared__ double sm[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < m; i += kRedThreads) acc = acc + part[i];
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+__global__ void k_axpy(int n, double a, const double* __restrict__ y, double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + a * y[i];
+}
+
+__global__ void k_scale_copy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i] * a;
+}
+
+template <bool ACCUM>
+__global__ void k_tally_out(int n, double* __restrict__ x, double* __restrict__ t64, float* __restrict__ t32,
+                            const double* __restrict__ rhs, int add_rhs, double inv,
+                            const uint64_t* mtotal) {
+  if (mtotal) {
+    uint64_t M = *mtotal;
+    inv = M ? 1.0 / (double)M : 0.0;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double t;
+    if (t64) {
+      t = t64[i];
+      t64[i] = 0.0;
+    } else {
+      t = (double)t32[i];
+      t32[i] = 0.0f;
+    }
+    double dx = t * inv;
+    if (add_rhs) dx = dx + rhs[i];
+    x[i] = ACCUM ? x[i] + dx : dx;
+  }
+}
+
+}  // namespace
+
+// ============================================================== launchers
+
+hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  return c.precision == 32 ? launch_walk_p<float>(a, c, s) : launch_walk_p<double>(a, c, s);
+}
+
+hipError_t launch_absmax(const double* v, int n, unsigned long long* bmax, hipStream_t s) {
+  k_absmax<<<grid_for(n, kRedThreads, 2048), kRedThreads, 0, s>>>(v, n, bmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantize(const double* rhs, int n, const unsigned long long* bmax, uint64_t* q,
+                           hipStream_t s) {
+  k_quantize<<<grid_for(n, 256), 256, 0, s>>>(rhs, n, bmax, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_strat_counts(const uint64_t* q, int n, const uint64_t* qtotal, uint64_t nh, uint64_t* cnt,
+                               hipStream_t s) {
+  k_strat_counts<<<grid_for(n, 256), 256, 0, s>>>(q, n, qtotal, nh, cnt);
+  return hipGetLastError();
+}
+
+size_t scan_tmp_elems(int64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+template <typename T>
+static hipError_t scan_incl(const T* in, T* out, int64_t n, T* tmp, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  k_scan_tile_sums<T><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, n, tmp);
+  k_scan_sums<T><<<1, kScanThreads, 0, s>>>(tmp, tiles);
+  k_scan_apply<T><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, n, tmp, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* tmp, hipStream_t s) {
+  return scan_incl<unsigned long long>((const unsigned long long*)in, (unsigned long long*)out, n,
+                                       (unsigned long long*)tmp, s);
+}
+
+hipError_t launch_scan_i32_excl(const int* in, int* out, int64_t n, int* tmp, hipStream_t s) {
+  k_fill<int><<<1, 1, 0, s>>>(out, 1, 0);
+  return scan_incl<int>(in, out + 1, n, tmp, s);
+}
+
+hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
+  k_fill<int><<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s) {
+  k_fill<double><<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
+  k_fill<unsigned long long><<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_jacobi(int n, const int* rp, const int* col, const void* val, bool f32, double* dinv,
+                         int* flags, hipStream_t s) {
+  if (f32)
+    k_jacobi<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, flags);
+  else
+    k_jacobi<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_validate_csr(int n, const int* rp, const int* col, const void* val, bool f32, int* flags,
+                               hipStream_t s) {
+  if (f32)
+    k_validate_csr<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, flags);
+  else
+    k_validate_csr<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_cols(int n, const int* rp, const int* col, const void* val, bool f32, int* cnt,
+                             int* flags, hipStream_t s) {
+  if (f32)
+    k_count_cols<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, cnt, flags);
+  else
+    k_count_cols<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, cnt, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_transpose(int n, const int* rp, const int* col, const void* val, bool f32,
+                                    const double* dinv, const int* t_rp, int* fill, int* t_src,
+                                    double* t_val, hipStream_t s) {
+  if (f32)
+    k_scatter_transpose<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, fill, t_src, t_val);
+  else
+    k_scatter_transpose<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, fill, t_src, t_val);
+  return hipGetLastError();
+}
+
+hipError_t launch_domain_rows(int n, const int* t_rp, int* t_src, double* t_val, double omega, double threshold,
+                              bool has_prec, int* keep, int* flags, hipStream_t s) {
+  k_domain_rows<<<grid_for(n, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold, has_prec ? 1 : 0,
+                                                  keep, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const double* t_val, double omega,
+                              double threshold, bool has_prec, const int* d_rp, uint32_t* col, double* h64,
+                              float* h32, double* cdf64, float* cdf32, double* W64, float* W32, hipStream_t s) {
+  k_domain_fill<<<grid_for(n, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold, has_prec ? 1 : 0, d_rp,
+                                                  col, h64, h32, cdf64, cdf32, W64, W32);
+  return hipGetLastError();
+}
+
+int residual_blocks(int n) { return (n + kRedThreads - 1) / kRedThreads; }
+
+hipError_t launch_prec_residual(int n, const int* rp, const int* col, const void* val, bool f32,
+                                const double* dinv, const double* b, const double* x, double* r, double* part,
+                                hipStream_t s) {
+  int g = residual_blocks(n);
+  if (f32)
+    k_prec_residual<float><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part);
+  else
+    k_prec_residual<double><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s) {
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part, hipStream_t s) {
+  k_scaled_sq<<<residual_blocks(n), kRedThreads, 0, s>>>(n, dinv, b, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(int n, double a, const double* y, double* x, hipStream_t s) {
+  k_axpy<<<grid_for(n, 256), 256, 0, s>>>(n, a, y, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s) {
+  k_scale_copy<<<grid_for(n, 256), 256, 0, s>>>(n, a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_tally(int n, double* x, double* t64, float* t32, const double* rhs, bool add_rhs,
+                              double inv, const uint64_t* mtotal, hipStream_t s) {
+  k_tally_out<true><<<grid_for(n, 256), 256, 0, s>>>(n, x, t64, t32, rhs, add_rhs ? 1 : 0, inv, mtotal);
+  return hipGetLastError();
+}
+
+hipError_t launch_store_tally(int n, double* x, double* t64, float* t32, const double* rhs, bool add_rhs,
+                              double inv, const uint64_t* mtotal, hipStream_t s) {
+  k_tally_out<false><<<grid_for(n, 256), 256, 0, s>>>(n, x, t64, t32, rhs, add_rhs ? 1 : 0, inv, mtotal);
+  return hipGetLastError();
+}
+
+}  // namespace mcls

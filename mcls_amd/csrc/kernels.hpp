@@ -1,0 +1,130 @@
+// Launchers for the gfx950 kernels of the adjoint MC / MCSA path.
+// All launches go to the caller's stream and never synchronise.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcls {
+
+// Device error flags written by kernels (first failing row + 1, 0 = ok).
+enum DeviceFlag : int {
+  FLAG_BAD_COLUMN = 0,   // column index out of range
+  FLAG_DUPLICATE = 1,    // duplicate column within a row
+  FLAG_ZERO_DIAG = 2,    // zero / missing diagonal under point Jacobi
+  FLAG_NONFINITE = 3,    // NaN / Inf in a matrix value
+  FLAG_COUNT = 8
+};
+
+// Adjoint domain in HBM: rows of H^T (CSR).  col packs the column index in
+// bits 0..30 and the sign of h in bit 31 (1 = negative).
+struct DomainView {
+  int n;
+  const int* rp;          // n+1
+  const uint32_t* col;    // nnz, packed
+  const double* cdf64;    // nnz
+  const float* cdf32;     // nnz
+  const double* W64;      // n
+  const float* W32;       // n
+  const double* h64;      // nnz
+  const float* h32;       // nnz
+};
+
+enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1 };
+enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
+
+struct WalkArgs {
+  DomainView d;
+  const double* rhs;       // source vector (n)
+  const uint64_t* q;       // quantised |rhs| (n)
+  const uint64_t* qcdf;    // inclusive prefix of q (n); Q = qcdf[n-1]
+  const uint64_t* ncdf;    // stratified: inclusive prefix of counts; M = ncdf[n-1]
+  uint64_t key;            // per-solve stream key
+  int64_t first;           // first history id
+  int64_t last;            // one past the last history id (random source)
+  int64_t history_length;  // max transitions
+  double cutoff;           // relative weight cutoff
+  double* tally64;         // FP64 walks
+  float* tally32;          // FP32 walks
+  unsigned long long* steps;  // total transitions (atomic)
+  int* paths;              // recording: (last-first) x (max_len+1)
+  double* wfinal;          // recording: final weight per history
+  int max_len;
+};
+
+struct WalkConfig {
+  int precision;   // 64 or 32
+  int estimator;   // Estimator
+  int source;      // SourceMode
+  bool record;
+  int block;       // threads per block
+  int grid;        // blocks
+};
+
+hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s);
+
+// ---------------------------------------------------------------- source
+// bits of max |rhs| (as uint64) -> *bmax (must be zeroed first)
+hipError_t launch_absmax(const double* v, int n, unsigned long long* bmax, hipStream_t s);
+// q_i = ceil(ldexp(|rhs_i|, S - e)), bmax < 2^e
+hipError_t launch_quantize(const double* rhs, int n, const unsigned long long* bmax,
+                           uint64_t* q, hipStream_t s);
+// stratified counts n_i = ceil(q_i nh / Q)
+hipError_t launch_strat_counts(const uint64_t* q, int n, const uint64_t* qtotal,
+                               uint64_t nh, uint64_t* cnt, hipStream_t s);
+
+// ------------------------------------------------------------------ scan
+// Inclusive scan of n uint64 values into out (may alias in). tmp holds
+// scan_tmp_elems(n) elements.
+size_t scan_tmp_elems(int64_t n);
+hipError_t launch_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* tmp,
+                           hipStream_t s);
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total).
+hipError_t launch_scan_i32_excl(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
+
+// ---------------------------------------------------------------- domain
+hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_jacobi(int n, const int* rp, const int* col, const void* val, bool f32,
+                         double* dinv, int* flags, hipStream_t s);
+// column range and finiteness of a CSR matrix -> flags
+hipError_t launch_validate_csr(int n, const int* rp, const int* col, const void* val, bool f32, int* flags,
+                               hipStream_t s);
+hipError_t launch_count_cols(int n, const int* rp, const int* col, const void* val, bool f32,
+                             int* cnt, int* flags, hipStream_t s);
+hipError_t launch_scatter_transpose(int n, const int* rp, const int* col, const void* val,
+                                    bool f32, const double* dinv, const int* t_rp, int* fill,
+                                    int* t_src, double* t_val, hipStream_t s);
+hipError_t launch_domain_rows(int n, const int* t_rp, int* t_src, double* t_val, double omega,
+                              double threshold, bool has_prec, int* keep, int* flags,
+                              hipStream_t s);
+hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const double* t_val,
+                              double omega, double threshold, bool has_prec, const int* d_rp,
+                              uint32_t* col, double* h64, float* h32, double* cdf64,
+                              float* cdf32, double* W64, float* W32, hipStream_t s);
+
+// ---------------------------------------------------------------- linalg
+// r = PL (b - A x); partial sums of r^2 per block into part (nblk entries).
+int residual_blocks(int n);
+hipError_t launch_prec_residual(int n, const int* rp, const int* col, const void* val, bool f32,
+                                const double* dinv, const double* b, const double* x,
+                                double* r, double* part, hipStream_t s);
+// out[0] = sum of part[0..m) in a fixed order; out_host (pinned, nullable)
+hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s);
+// ||PL b||_2^2 partials
+hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part,
+                            hipStream_t s);
+// x += a * y
+hipError_t launch_axpy(int n, double a, const double* y, double* x, hipStream_t s);
+// y = a * x
+hipError_t launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
+// x += tally * inv (+ rhs for the expected-value estimator); tally zeroed.
+// inv = 1/nh, or 1/M read from *mtotal when mtotal != NULL.
+hipError_t launch_apply_tally(int n, double* x, double* t64, float* t32, const double* rhs,
+                              bool add_rhs, double inv, const uint64_t* mtotal, hipStream_t s);
+// x = tally * inv (+ rhs); tally zeroed.
+hipError_t launch_store_tally(int n, double* x, double* t64, float* t32, const double* rhs,
+                              bool add_rhs, double inv, const uint64_t* mtotal, hipStream_t s);
+
+}  // namespace mcls

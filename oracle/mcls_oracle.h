@@ -1,0 +1,181 @@
+/*
+ * mcls_oracle.h -- CPU restatement of the MCLS adjoint Neumann-Ulam / MCSA
+ * path.  TEST INFRASTRUCTURE ONLY: this library is the checker that the HIP
+ * path is compared against, and the CPU baseline that bench.py times.  No
+ * product code links it.
+ *
+ * Two families of functions live here:
+ *
+ *  1. "build semantics" (orc_*): the exact arithmetic the GPU library
+ *     implements (per-history Xorshift streams, the integer-quantised source
+ *     CDF, FP64/FP32 domain tables, collision and expected-value estimators,
+ *     relative weight cutoff).  These are bit-exact twins of the HIP kernels
+ *     for everything except floating-point tally accumulation order.
+ *
+ *  2. "reference semantics" (orc_ref_*): a faithful restatement of the
+ *     MCLS algorithm itself (one Xorshift stream per set, FP64 |b| CDF with
+ *     lower_bound, LIFO source stack, weight sign(b)*||b||_1, multi-set sum),
+ *     used as the CPU baseline.
+ *
+ * Every function cites the reference file:line it follows.
+ */
+#ifndef MCLS_ORACLE_H
+#define MCLS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ RNG -- */
+uint64_t orc_splitmix64(uint64_t z);
+uint64_t orc_xorshift64(uint64_t* state);
+uint64_t orc_history_state(uint64_t seed, uint64_t solve_index, uint64_t history_id);
+double orc_u01_f64(uint64_t r);
+float orc_u01_f32(uint64_t r);
+
+/* ------------------------------------------------------------- sampling -- */
+int orc_lower_bound_f64(const double* cdf, int n, double u);
+int orc_lower_bound_f32(const float* cdf, int n, float u);
+int64_t orc_upper_bound_u64(const uint64_t* cdf, int64_t n, uint64_t t);
+
+/* --------------------------------------------------------------- domain -- */
+/* Build the adjoint domain rows (rows of H^T, H = I - omega*PL*A, PL =
+ * diag(dinv) or identity when dinv == NULL).  Output arrays must hold
+ * a_nnz entries (d_rp: n+1).  When a preconditioner is present the
+ * composite row is filtered first: entries with |m| <= max(row)*threshold
+ * are dropped ("Composite Operator Threshold").  Returns the domain nnz, or a
+ * negative error:
+ * -1 column out of range, -2 duplicate column in a row. */
+int64_t orc_build_domain(int n, const int64_t* a_rp, const int* a_col,
+                         const double* a_val, const double* dinv, double omega,
+                         double threshold, int64_t* d_rp, int* d_col,
+                         double* d_h, double* d_cdf, double* d_W);
+
+/* Point-Jacobi inverse diagonal: dinv_i = 1/a_ii.  Returns 0, or -(i+1)
+ * for the first row with a zero / missing diagonal. */
+int orc_jacobi_dinv(int n, const int64_t* a_rp, const int* a_col,
+                    const double* a_val, double* dinv);
+
+/* Integer-quantised |b| source table.  Returns Q (total) and writes
+ * q (n entries) and the inclusive prefix cdf (n entries). */
+uint64_t orc_source_quantize(int n, const double* b, uint64_t* q, uint64_t* cdf);
+
+/* Per-history birth: state and weight of history `hid`.  Returns state or
+ * -1 if the source is empty.  Advances *rng by one draw. */
+int orc_source_birth(int n, const double* b, const uint64_t* q,
+                     const uint64_t* cdf, uint64_t Q, uint64_t* rng,
+                     double* w0);
+
+/* ------------------------------------------------------------- MC solve -- */
+enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1 };
+enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
+
+typedef struct orc_mc_params {
+  int64_t num_histories;   /* histories per MC solve                       */
+  int64_t history_length;  /* max transitions per history ("History Length") */
+  double weight_cutoff;    /* relative cutoff, 0 = off                      */
+  int64_t precision;       /* 64 or 32 (walk weights / CDF / W)            */
+  int64_t estimator;       /* ORC_EST_*                                    */
+  int64_t source;          /* ORC_SRC_*                                    */
+  uint64_t seed;
+  int64_t num_threads;     /* worker threads (tallies reduced in order)    */
+} orc_mc_params;
+
+/* One adjoint MC solve over the domain: x = (1/nh) * sum of tallies (plus
+ * the deterministic birth term b for the expected-value estimator).
+ * Returns 0 on success; *steps_out = total transitions. */
+int orc_mc_solve(int n, const int64_t* d_rp, const int* d_col,
+                 const double* d_h, const double* d_cdf, const double* d_W,
+                 const double* b, const orc_mc_params* p,
+                 uint64_t solve_index, double* x, int64_t* steps_out);
+
+/* Record the state-index path of histories [first, first+count): paths is
+ * count x (max_len+1) ints, padded with -1; weights the final weight. */
+int orc_record_paths(int n, const int64_t* d_rp, const int* d_col,
+                     const double* d_h, const double* d_cdf, const double* d_W,
+                     const double* b, const orc_mc_params* p,
+                     uint64_t solve_index, int64_t first, int64_t count,
+                     int max_len, int* paths, double* weights);
+
+/* --------------------------------------------------------------- linalg -- */
+void orc_spmv(int n, const int64_t* rp, const int* col, const double* val,
+              const double* x, double* y);
+/* rp = PL (b - A x) (PL = diag(dinv) or identity) */
+void orc_prec_residual(int n, const int64_t* rp, const int* col,
+                       const double* val, const double* dinv, const double* b,
+                       const double* x, double* r);
+double orc_norm2(int n, const double* v);
+double orc_norm1(int n, const double* v);
+double orc_norminf(int n, const double* v);
+
+/* --------------------------------------------------------------- solvers -- */
+typedef struct orc_solver_params {
+  double tol;               /* "Convergence Tolerance"       */
+  int64_t max_iters;        /* "Maximum Iterations"          */
+  int64_t check_freq;       /* "Iteration Check Frequency"   */
+  int64_t smoother_steps;   /* "Smoother Steps"              */
+  double richardson_omega;  /* "Richardson Relaxation"       */
+  double neumann_omega;     /* "Neumann Relaxation"          */
+  double threshold;         /* "Composite Operator Threshold" */
+  int64_t num_sets;         /* multi-set replication (>= 1)  */
+  int64_t reference_mode;   /* 1 = MCLS reference MC semantics, 0 = build */
+  orc_mc_params mc;
+} orc_solver_params;
+
+typedef struct orc_solver_result {
+  int64_t iters;
+  int64_t converged;
+  double achieved_tol;
+  int64_t histories;
+  int64_t steps;
+  double seconds_mc;
+  double seconds_total;
+} orc_solver_result;
+
+/* MCSASolverManager<...,AdjointTag>::solve.  dinv may be NULL (no left
+ * preconditioner).  res_hist (nullable) receives max_iters+1 relative
+ * residuals. */
+int orc_mcsa_solve(int n, const int64_t* rp, const int* col, const double* val,
+                   const double* dinv, const double* b, double* x,
+                   const orc_solver_params* sp, uint64_t solve_index0,
+                   orc_solver_result* res, double* res_hist);
+
+/* FixedPointSolverManager (Richardson) with the infinity-norm test. */
+int orc_fixed_point_solve(int n, const int64_t* rp, const int* col,
+                          const double* val, const double* dinv,
+                          const double* b, double* x,
+                          const orc_solver_params* sp, orc_solver_result* res);
+
+/* ------------------------------------------------ reference semantics -- */
+/* UniformAdjointSource::buildRandomSource / buildStratifiedSource with one
+ * stream: fills counts[n] and returns the number of histories to emit (the
+ * normalisation constant).  ref_quirk = 1 reproduces the signed b[0] CDF
+ * seed of the reference; 0 uses |b[0]|. */
+int64_t orc_ref_build_source(int n, const double* b, double sample_ratio_nh,
+                             int stratified, int ref_quirk, uint64_t* rng,
+                             int64_t* counts);
+
+/* One reference-semantics MC solve with a single stream *rng: source
+ * build, LIFO emission (highest state first), collision tally, normalise.
+ * Returns the normalisation constant (histories emitted). */
+int64_t orc_ref_mc_solve(int n, const int64_t* d_rp, const int* d_col,
+                         const double* d_h, const double* d_cdf,
+                         const double* d_W, const double* b,
+                         const orc_mc_params* p, int stratified, int ref_quirk,
+                         uint64_t* rng, double* x, int64_t* steps_out);
+
+/* DomainTransporter::transport of one history from (state, weight):
+ * returns the event (1 = CUTOFF, 2 = weight cutoff, 3 = absorbed on an
+ * empty row) and updates state/weight/steps; tallies into x. */
+int orc_ref_transport(const int64_t* d_rp, const int* d_col, const double* d_h,
+                      const double* d_cdf, const double* d_W,
+                      int64_t history_length, double weight_cutoff,
+                      uint64_t* rng, int* state, double* weight,
+                      int64_t* steps, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

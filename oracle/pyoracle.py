@@ -1,0 +1,319 @@
+"""ctypes bindings to the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  See
+mcls_oracle.h for the semantics and the reference files each function follows.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+EST_COLLISION = 0
+EST_EXPECTED_VALUE = 1
+SRC_RANDOM = 0
+SRC_STRATIFIED = 1
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+class MCParams(C.Structure):
+    _fields_ = [
+        ("num_histories", C.c_int64),
+        ("history_length", C.c_int64),
+        ("weight_cutoff", C.c_double),
+        ("precision", C.c_int64),
+        ("estimator", C.c_int64),
+        ("source", C.c_int64),
+        ("seed", C.c_uint64),
+        ("num_threads", C.c_int64),
+    ]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("max_iters", C.c_int64),
+        ("check_freq", C.c_int64),
+        ("smoother_steps", C.c_int64),
+        ("richardson_omega", C.c_double),
+        ("neumann_omega", C.c_double),
+        ("threshold", C.c_double),
+        ("num_sets", C.c_int64),
+        ("reference_mode", C.c_int64),
+        ("mc", MCParams),
+    ]
+
+
+class SolverResult(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int64),
+        ("converged", C.c_int64),
+        ("achieved_tol", C.c_double),
+        ("histories", C.c_int64),
+        ("steps", C.c_int64),
+        ("seconds_mc", C.c_double),
+        ("seconds_total", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    u64, i64, i32, f64, f32 = C.c_uint64, C.c_int64, C.c_int, C.c_double, C.c_float
+    vp = C.c_void_p
+    L.orc_splitmix64.argtypes = [u64]
+    L.orc_splitmix64.restype = u64
+    L.orc_xorshift64.argtypes = [C.POINTER(u64)]
+    L.orc_xorshift64.restype = u64
+    L.orc_history_state.argtypes = [u64, u64, u64]
+    L.orc_history_state.restype = u64
+    L.orc_u01_f64.argtypes = [u64]
+    L.orc_u01_f64.restype = f64
+    L.orc_u01_f32.argtypes = [u64]
+    L.orc_u01_f32.restype = f32
+    L.orc_lower_bound_f64.argtypes = [_f64p, i32, f64]
+    L.orc_lower_bound_f64.restype = i32
+    L.orc_lower_bound_f32.argtypes = [_f32p, i32, f32]
+    L.orc_lower_bound_f32.restype = i32
+    L.orc_upper_bound_u64.argtypes = [_u64p, i64, u64]
+    L.orc_upper_bound_u64.restype = i64
+    L.orc_build_domain.argtypes = [i32, _i64p, _i32p, _f64p, vp, f64, f64,
+                                   _i64p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_build_domain.restype = i64
+    L.orc_jacobi_dinv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p]
+    L.orc_jacobi_dinv.restype = i32
+    L.orc_source_quantize.argtypes = [i32, _f64p, _u64p, _u64p]
+    L.orc_source_quantize.restype = u64
+    L.orc_source_birth.argtypes = [i32, _f64p, _u64p, _u64p, u64, C.POINTER(u64), C.POINTER(f64)]
+    L.orc_source_birth.restype = i32
+    dom = [i32, _i64p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_mc_solve.argtypes = dom + [_f64p, C.POINTER(MCParams), u64, _f64p, C.POINTER(i64)]
+    L.orc_mc_solve.restype = i32
+    L.orc_record_paths.argtypes = dom + [_f64p, C.POINTER(MCParams), u64, i64, i64, i32,
+                                         _i32p, _f64p]
+    L.orc_record_paths.restype = i32
+    L.orc_spmv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_spmv.restype = None
+    L.orc_prec_residual.argtypes = [i32, _i64p, _i32p, _f64p, vp, _f64p, _f64p, _f64p]
+    L.orc_prec_residual.restype = None
+    for nm in ("orc_norm2", "orc_norm1", "orc_norminf"):
+        getattr(L, nm).argtypes = [i32, _f64p]
+        getattr(L, nm).restype = f64
+    L.orc_mcsa_solve.argtypes = [i32, _i64p, _i32p, _f64p, vp, _f64p, _f64p,
+                                 C.POINTER(SolverParams), u64, C.POINTER(SolverResult), vp]
+    L.orc_mcsa_solve.restype = i32
+    L.orc_fixed_point_solve.argtypes = [i32, _i64p, _i32p, _f64p, vp, _f64p, _f64p,
+                                        C.POINTER(SolverParams), C.POINTER(SolverResult)]
+    L.orc_fixed_point_solve.restype = i32
+    L.orc_ref_build_source.argtypes = [i32, _f64p, f64, i32, i32, C.POINTER(u64), _i64p]
+    L.orc_ref_build_source.restype = i64
+    L.orc_ref_mc_solve.argtypes = dom + [_f64p, C.POINTER(MCParams), i32, i32, C.POINTER(u64),
+                                         _f64p, C.POINTER(i64)]
+    L.orc_ref_mc_solve.restype = i64
+    L.orc_ref_transport.argtypes = [_i64p, _i32p, _f64p, _f64p, _f64p, i64, f64,
+                                    C.POINTER(u64), C.POINTER(i32), C.POINTER(f64),
+                                    C.POINTER(i64), _f64p]
+    L.orc_ref_transport.restype = i32
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Csr:
+    """Plain host CSR (int64 row pointers, int32 columns, float64 values)."""
+
+    def __init__(self, rp, col, val, n=None):
+        self.rp = np.ascontiguousarray(rp, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.n = int(n if n is not None else len(self.rp) - 1)
+
+    @classmethod
+    def from_dense(cls, A, keep_zeros=False):
+        A = np.asarray(A, dtype=np.float64)
+        rp, col, val = [0], [], []
+        for i in range(A.shape[0]):
+            for j in range(A.shape[1]):
+                if A[i, j] != 0.0 or (keep_zeros and keep_zeros(i, j)):
+                    col.append(j)
+                    val.append(A[i, j])
+            rp.append(len(col))
+        return cls(rp, col, val, A.shape[0])
+
+    @classmethod
+    def from_scipy(cls, S):
+        S = S.tocsr()
+        S.sort_indices()
+        return cls(S.indptr, S.indices, S.data, S.shape[0])
+
+    def to_dense(self):
+        A = np.zeros((self.n, self.n))
+        for i in range(self.n):
+            for k in range(self.rp[i], self.rp[i + 1]):
+                A[i, self.col[k]] += self.val[k]
+        return A
+
+
+class Domain:
+    def __init__(self, n, rp, col, h, cdf, W):
+        self.n, self.rp, self.col, self.h, self.cdf, self.W = n, rp, col, h, cdf, W
+
+    @property
+    def nnz(self):
+        return int(self.rp[-1])
+
+    def args(self):
+        return (self.n, self.rp, self.col, self.h, self.cdf, self.W)
+
+
+def jacobi_dinv(A):
+    d = np.zeros(A.n)
+    rc = lib().orc_jacobi_dinv(A.n, A.rp, A.col, A.val, d)
+    if rc != 0:
+        raise ValueError(f"zero diagonal in row {-rc - 1}")
+    return d
+
+
+def build_domain(A, dinv=None, omega=1.0, threshold=0.0):
+    nnz = int(A.rp[-1])
+    cap = max(nnz, 1)
+    rp = np.zeros(A.n + 1, np.int64)
+    col = np.zeros(cap, np.int32)
+    h = np.zeros(cap)
+    cdf = np.zeros(cap)
+    W = np.zeros(A.n)
+    dv = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+    got = lib().orc_build_domain(A.n, A.rp, A.col, A.val, _ptr(dv), omega, threshold,
+                                 rp, col, h, cdf, W)
+    if got < 0:
+        raise ValueError({-1: "column out of range", -2: "duplicate column"}.get(got, got))
+    return Domain(A.n, rp, col[:got].copy(), h[:got].copy(), cdf[:got].copy(), W)
+
+
+def mc_params(num_histories, history_length=10, weight_cutoff=0.0, precision=64,
+              estimator=EST_COLLISION, source=SRC_RANDOM, seed=0, num_threads=1):
+    return MCParams(int(num_histories), int(history_length), float(weight_cutoff),
+                    int(precision), int(estimator), int(source), int(seed) & (2**64 - 1),
+                    int(num_threads))
+
+
+def mc_solve(dom, b, params, solve_index=0):
+    x = np.zeros(dom.n)
+    steps = C.c_int64(0)
+    b = np.ascontiguousarray(b, np.float64)
+    lib().orc_mc_solve(*dom.args(), b, C.byref(params), int(solve_index), x, C.byref(steps))
+    return x, int(steps.value)
+
+
+def record_paths(dom, b, params, first, count, max_len, solve_index=0):
+    paths = np.zeros((count, max_len + 1), np.int32)
+    w = np.zeros(count)
+    b = np.ascontiguousarray(b, np.float64)
+    lib().orc_record_paths(*dom.args(), b, C.byref(params), int(solve_index), int(first),
+                           int(count), int(max_len), paths, w)
+    return paths, w
+
+
+def source_quantize(b):
+    b = np.ascontiguousarray(b, np.float64)
+    q = np.zeros(len(b), np.uint64)
+    cdf = np.zeros(len(b), np.uint64)
+    Q = lib().orc_source_quantize(len(b), b, q, cdf)
+    return int(Q), q, cdf
+
+
+def solver_params(tol=1e-8, max_iters=1000, check_freq=1, smoother_steps=1,
+                  richardson_omega=1.0, neumann_omega=1.0, threshold=0.0, num_sets=1,
+                  reference_mode=0, mc=None):
+    return SolverParams(float(tol), int(max_iters), int(check_freq), int(smoother_steps),
+                        float(richardson_omega), float(neumann_omega), float(threshold),
+                        int(num_sets), int(reference_mode), mc if mc is not None else mc_params(1))
+
+
+def mcsa_solve(A, b, x0=None, dinv=None, params=None, solve_index0=0):
+    params = params if params is not None else solver_params()
+    x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
+    res = SolverResult()
+    hist = np.zeros(params.max_iters + 1)
+    b = np.ascontiguousarray(b, np.float64)
+    dv = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+    rc = lib().orc_mcsa_solve(A.n, A.rp, A.col, A.val, _ptr(dv), b, x, C.byref(params),
+                              int(solve_index0), C.byref(res), _ptr(hist))
+    if rc != 0:
+        raise ValueError(f"oracle mcsa failed: {rc}")
+    return x, res, hist[: min(res.iters, params.max_iters) + 1]
+
+
+def fixed_point_solve(A, b, x0=None, dinv=None, params=None):
+    params = params if params is not None else solver_params()
+    x = np.zeros(A.n) if x0 is None else np.array(x0, np.float64)
+    res = SolverResult()
+    b = np.ascontiguousarray(b, np.float64)
+    dv = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+    lib().orc_fixed_point_solve(A.n, A.rp, A.col, A.val, _ptr(dv), b, x, C.byref(params),
+                                C.byref(res))
+    return x, res
+
+
+def spmv(A, x):
+    y = np.zeros(A.n)
+    lib().orc_spmv(A.n, A.rp, A.col, A.val, np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def prec_residual(A, b, x, dinv=None):
+    r = np.zeros(A.n)
+    dv = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
+    lib().orc_prec_residual(A.n, A.rp, A.col, A.val, _ptr(dv),
+                            np.ascontiguousarray(b, np.float64),
+                            np.ascontiguousarray(x, np.float64), r)
+    return r
+
+
+def ref_build_source(b, nh_total, stratified=False, ref_quirk=False, rng_state=1):
+    b = np.ascontiguousarray(b, np.float64)
+    counts = np.zeros(len(b), np.int64)
+    st = C.c_uint64(rng_state)
+    nh = lib().orc_ref_build_source(len(b), b, float(nh_total), int(stratified),
+                                    int(ref_quirk), C.byref(st), counts)
+    return int(nh), counts
+
+
+def ref_transport(dom, state, weight, history_length, weight_cutoff=0.0, rng_state=1, x=None):
+    x = np.zeros(dom.n) if x is None else x
+    st = C.c_uint64(rng_state)
+    s = C.c_int(state)
+    w = C.c_double(weight)
+    steps = C.c_int64(0)
+    ev = lib().orc_ref_transport(dom.rp, dom.col, dom.h, dom.cdf, dom.W, int(history_length),
+                                 float(weight_cutoff), C.byref(st), C.byref(s), C.byref(w),
+                                 C.byref(steps), x)
+    return ev, s.value, w.value, steps.value, x
+
+
+def ref_mc_solve(dom, b, params, stratified=False, ref_quirk=False, rng_state=1):
+    x = np.zeros(dom.n)
+    st = C.c_uint64(rng_state)
+    steps = C.c_int64(0)
+    nh = lib().orc_ref_mc_solve(*dom.args(), np.ascontiguousarray(b, np.float64),
+                                C.byref(params), int(stratified), int(ref_quirk), C.byref(st),
+                                x, C.byref(steps))
+    return x, int(nh), int(steps.value)

@@ -1,0 +1,186 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Domain tables and sampled state paths are compared bit-exactly; MC tallies
+(same streams, same paths, different summation order) to a relative 1e-10 in
+FP64 and 2e-5 in FP32; MCSA solutions against the oracle's and against a
+direct FP64 solve.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from mcls_amd import capi
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr(prob, f32=False):
+    val = prob.val32.astype(np.float64) if f32 else prob.values()
+    return po.Csr(prob.rp, prob.col, val)
+
+
+def _problem(kind):
+    if kind == "poisson":
+        return capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 40, 40, seed=0)), False
+    if kind == "diff3d":
+        return capi.build_problem(capi.make_spec(capi.DIFFUSION3D_7PT, 12, seed=5, p0=4)), False
+    if kind == "s27":
+        return capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 10, seed=2, p0=0.6, p1=0.9,
+                                                 p2=0.8, p3=1.2), want64=False, want32=True), True
+    if kind == "aniso":
+        return capi.build_problem(capi.make_spec(capi.ANISO2D_5PT, 33, 29, seed=4, p0=37.0)), False
+    raise KeyError(kind)
+
+
+def _setup(ctx, prob, f32, jacobi=True, params=None):
+    ctx.set_problem(prob, "f32" if f32 else "f64")
+    if jacobi:
+        ctx.set_point_jacobi()
+    else:
+        ctx.clear_left_prec()
+    base = {"Neumann Relaxation": 1.0, "Composite Operator Threshold": 0.0,
+            "Walk Precision": "FP64", "Source Sampling Type": "Random",
+            "Estimator Type": "Collision", "Weight Cutoff": 0.0, "History Length": 10,
+            "Number of Histories": 0, "Random Seed": 0}
+    base.update(params or {})
+    ctx.set_params(base)
+
+
+@pytest.mark.parametrize("kind", ["poisson", "diff3d", "s27", "aniso"])
+@pytest.mark.parametrize("omega,thr", [(1.0, 0.0), (0.7, 0.05)])
+def test_domain_bitexact(ctx, kind, omega, thr):
+    prob, f32 = _problem(kind)
+    _setup(ctx, prob, f32, params={"Neumann Relaxation": omega, "Composite Operator Threshold": thr})
+    ctx.build_domain()
+    rp, col, h, cdf, W = ctx.domain()
+    A = _csr(prob, f32)
+    dinv = po.jacobi_dinv(A)
+    assert np.array_equal(ctx.get_left_prec_diag(), dinv)
+    dom = po.build_domain(A, dinv, omega, thr)
+    assert np.array_equal(rp, dom.rp)
+    assert np.array_equal(col, dom.col)
+    assert np.array_equal(h, dom.h)
+    assert np.array_equal(cdf, dom.cdf)
+    assert np.array_equal(W, dom.W)
+
+
+def test_domain_no_prec_and_random_matrix(ctx):
+    rng = np.random.default_rng(11)
+    S = sp.random(300, 300, density=0.02, random_state=rng, format="csr") * 0.2
+    S = (S + sp.eye(300) * 1.5).tocsr()
+    S.sort_indices()
+    prob = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32),
+                        S.data.astype(np.float64), None, rng.random(300))
+    for jac in (False, True):
+        _setup(ctx, prob, False, jacobi=jac)
+        ctx.build_domain()
+        rp, col, h, cdf, W = ctx.domain()
+        A = _csr(prob)
+        dom = po.build_domain(A, po.jacobi_dinv(A) if jac else None)
+        assert np.array_equal(rp, dom.rp) and np.array_equal(col, dom.col)
+        assert np.array_equal(h, dom.h) and np.array_equal(cdf, dom.cdf) and np.array_equal(W, dom.W)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("source", ["Random", "Stratified"])
+@pytest.mark.parametrize("kind", ["poisson", "s27"])
+def test_paths_bitexact(ctx, prec, source, kind):
+    prob, f32 = _problem(kind)
+    L, cutoff, nh = 300, 1e-3, 5000
+    _setup(ctx, prob, f32, params={"Walk Precision": f"FP{prec}", "Source Sampling Type": source,
+                                   "History Length": L, "Weight Cutoff": cutoff,
+                                   "Number of Histories": nh, "Random Seed": 1234})
+    A = _csr(prob, f32)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    rhs[::7] *= -1.0  # signed source
+    first, count = 100, 1500
+    gp, gw = ctx.record_paths(rhs, 17, first, count, L)
+    mp = po.mc_params(nh, L, cutoff, prec, po.EST_COLLISION,
+                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=1234)
+    op, ow = po.record_paths(dom, rhs, mp, first, count, L, solve_index=17)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gw, ow)
+    assert (gp[:, 1] >= 0).mean() > 0.9  # walks actually move
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("est", ["Collision", "Expected Value"])
+@pytest.mark.parametrize("source", ["Random", "Stratified"])
+def test_mc_tally_matches_oracle(ctx, prec, est, source):
+    prob, f32 = _problem("diff3d")
+    nh, L, cutoff = 20000, 50, 1e-2
+    _setup(ctx, prob, f32, params={"Walk Precision": f"FP{prec}", "Estimator Type": est,
+                                   "Source Sampling Type": source, "History Length": L,
+                                   "Weight Cutoff": cutoff, "Number of Histories": nh,
+                                   "Random Seed": 99})
+    A = _csr(prob, f32)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    xg = ctx.mc_solve_raw(rhs, 5)
+    st = ctx.stats()
+    mp = po.mc_params(nh, L, cutoff, prec,
+                      po.EST_EXPECTED_VALUE if est == "Expected Value" else po.EST_COLLISION,
+                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=99)
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=5)
+    assert st.steps == steps
+    tol = 1e-10 if prec == 64 else 2e-5
+    assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
+
+
+def test_mcsa_converges_poisson(ctx):
+    prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 24, 24, seed=0))
+    _setup(ctx, prob, False, params={"History Length": 5000, "Weight Cutoff": 1e-4,
+                                     "Number of Histories": 4000, "Convergence Tolerance": 1e-8,
+                                     "Maximum Iterations": 200, "Random Seed": 3})
+    x = ctx.mcsa_solve(prob.b)
+    st = ctx.stats()
+    assert st.converged == 1 and st.achieved_tol <= 1e-8
+    S = sp.csr_matrix((prob.values(), prob.col, prob.rp), shape=(prob.n, prob.n))
+    xd = sla.spsolve(S.tocsc(), prob.b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    # the oracle's MCSA with the same streams takes the same path to tolerance
+    A = _csr(prob)
+    mp = po.mc_params(4000, 5000, 1e-4, 64, seed=3)
+    xo, res, hist = po.mcsa_solve(A, prob.b, dinv=po.jacobi_dinv(A),
+                                  params=po.solver_params(tol=1e-8, max_iters=200, mc=mp),
+                                  solve_index0=st.solve_index - st.iterations)
+    assert res.converged == 1
+    assert abs(res.iters - st.iterations) <= 3
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_residual_matches_oracle(ctx):
+    prob, f32 = _problem("s27")
+    _setup(ctx, prob, f32)
+    rng = np.random.default_rng(3)
+    x = rng.random(prob.n)
+    r, nrm = ctx.prec_residual(prob.b, x)
+    A = _csr(prob, f32)
+    ro = po.prec_residual(A, prob.b, x, po.jacobi_dinv(A))
+    assert np.array_equal(r, ro)
+    assert abs(nrm - np.linalg.norm(ro)) <= 1e-14 * np.linalg.norm(ro)
+
+
+def test_errors(ctx):
+    rp = np.array([0, 1, 2], np.int64)
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_matrix(rp, np.array([0, 5], np.int32), np.array([1.0, 1.0]))
+    assert e.value.code == capi.MCLS_ERR_MATRIX
+    ctx.set_matrix(rp, np.array([1, 0], np.int32), np.array([1.0, 1.0]))
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_point_jacobi()
+    assert e.value.code == capi.MCLS_ERR_MATRIX
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_params({"No Such Key": 1})
+    assert e.value.code == capi.MCLS_ERR_INVALID_ARG
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_params({"Fixed Point Type": "MINRES"})
+    assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_matrix(rp, np.array([0, 1], np.int32), np.array([np.nan, 1.0]))
+    assert e.value.code == capi.MCLS_ERR_MATRIX
