@@ -15,7 +15,7 @@ CXXFLAGS := -O2 -std=c++17 -fPIC -Wall -Iinclude
 GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip
 GPU_HDRS := $(SRC)/kernels.hpp include/mcls_gpu.h
 
-all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so oracle
+all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade oracle
 
 $(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
 	mkdir -p $(LIBDIR)
@@ -24,6 +24,11 @@ $(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
 $(LIBDIR)/libmcls_problems.so: $(SRC)/problems.cpp include/mcls_problems.h
 	mkdir -p $(LIBDIR)
 	$(CXX) $(CXXFLAGS) -shared -o $@ $(SRC)/problems.cpp -lpthread
+
+tests/cpp/bin/test_facade: tests/cpp/test_facade.cpp mcls_amd/host/*.hpp include/mcls_gpu.h $(LIBDIR)/libmcls_gpu.so
+	mkdir -p tests/cpp/bin
+	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_facade.cpp -L$(LIBDIR) -lmcls_gpu \
+	    -L/opt/rocm/lib -Wl,-rpath,'$$ORIGIN/../../../$(LIBDIR)' -Wl,-rpath,/opt/rocm/lib
 
 oracle:
 	$(MAKE) -C oracle

@@ -1,0 +1,212 @@
+"""Benchmark: MCSA time-to-tolerance and histories/s on one MI355X.
+
+Workload (default): 3-D 27-point stencil on a 256^3 grid (16,777,216 unknowns,
+449,455,096 nonzeros, CSR with FP32 values and int32 columns), point-Jacobi
+preconditioning, 2^28 histories per MCSA iteration, tolerance 1e-8 on
+||rp||_2 / ||PL b||_2.  One step = one full MCSA solve from x = 0 to the
+tolerance, with the matrix, the domain and b resident in HBM.
+
+Forms:
+  chosen: FP32 walks and tallies, FP64 residual/update/convergence test,
+          stratified source, relative weight cutoff 1e-2.
+  plain:  FP64 walks and tallies, random source, weight cutoff 1e-4.
+value = histories/s of the chosen form over the K timed solves.
+
+The walk-kernel roofline uses the per-step algorithmic bytes of the CDF
+walk, B = 8 + p z + 4 + p + 2p (p = 4 or 8 bytes, z = mean domain row
+length), times the transitions of one launch, divided by that launch's
+HIP-event duration measured by the library on its own stream.
+
+cpu_baseline: the oracle's restatement of the MCLS reference algorithm
+(FP64, per-row CDF + lower_bound, one Xorshift stream per set, multi-set
+threads summed / numSets) timed for a bounded number of MCSA iterations on
+the same operator family at a reduced grid (see the printed sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mcls_amd import capi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MCSA wall-time to 1e-8 rel. residual (plain vs chosen form); histories/sec"
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+WORKLOADS = {
+    "cfg4": dict(kind=capi.STENCIL3D_27PT, nx=256, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
+                 histories=1 << 28, label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
+                 "FP32 CSR), point Jacobi, 2^28 histories/iteration, tol 1e-8"),
+    "cfg4-64": dict(kind=capi.STENCIL3D_27PT, nx=64, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
+                    histories=16 * 64 ** 3, label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "
+                    "16 histories/unknown, tol 1e-8"),
+}
+
+FORMS = {
+    "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
+               "Estimator Type": "Collision", "Weight Cutoff": 1e-2, "History Length": 1000},
+    "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random",
+              "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000},
+}
+
+
+def make_problem(w):
+    p0, p1, p2, p3 = w["p"]
+    spec = capi.make_spec(w["kind"], w["nx"], seed=w["seed"], p0=p0, p1=p1, p2=p2, p3=p3)
+    return capi.build_problem(spec, want64=not w["f32"], want32=w["f32"], nthreads=16)
+
+
+def solve_once(ctx, b):
+    t0 = time.perf_counter()
+    ctx.mcsa_solve(b)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, ctx.stats()
+
+
+def bytes_per_step(prec, z):
+    p = 4 if prec == "FP32" else 8
+    return 8 + p * z + 4 + p + 2 * p
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """Oracle restatement of the MCLS algorithm on a 64^3 member of the family."""
+    from oracle import pyoracle as po
+
+    w = WORKLOADS["cfg4-64"]
+    prob = make_problem(w)
+    A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64))
+    dinv = po.jacobi_dinv(A)
+    cores = min(16, os.cpu_count() or 1)
+    nh = w["histories"]
+    mp = po.mc_params(nh // cores, 1000, 1e-4, 64, seed=1)
+    # one MCSA iteration first to size the bounded sample
+    sp = po.solver_params(tol=1e-30, max_iters=1, num_sets=cores, reference_mode=1, mc=mp)
+    x, res, _ = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
+    per_iter = max(res.seconds_mc, 1e-3)
+    iters = int(max(1, min(8, seconds_budget // per_iter)))
+    sp = po.solver_params(tol=1e-30, max_iters=iters, num_sets=cores, reference_mode=1, mc=mp)
+    x, res, _ = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
+    hps = res.histories / res.seconds_mc
+    return {"value": hps, "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": f"{iters} MCSA iterations of the MCLS-semantics CPU restatement "
+                      f"(FP64, CDF lower_bound, {cores} sets x {nh // cores} histories, "
+                      f"weight cutoff 1e-4) on the 64^3 27-point member of the family "
+                      f"(262,144 unknowns); MC transport time only",
+            "seconds_per_iteration": res.seconds_mc / res.iters,
+            "steps_per_history": res.steps / max(res.histories, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg4", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solve")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--form", default="chosen", choices=sorted(FORMS))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(set replication over RCCL is a design note only); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOADS[args.workload]
+    log(f"generating {args.workload}")
+    prob = make_problem(w)
+    log(f"n={prob.n} nnz={prob.nnz}")
+    ctx = capi.Context(0)
+    ctx.set_problem(prob, "f32" if w["f32"] else "f64")
+    ctx.set_point_jacobi()
+    base = {"Convergence Tolerance": 1e-8, "Maximum Iterations": 60, "Number of Histories": w["histories"],
+            "Random Seed": 2024}
+    ctx.set_params(base)
+    ctx.set_params(FORMS[args.form])
+    t0 = time.perf_counter()
+    ctx.build_domain()
+    ctx.synchronize()
+    st = ctx.stats()
+    log(f"domain built: nnz={st.domain_nnz} in {time.perf_counter() - t0:.2f}s (kernel {st.ms_domain:.1f} ms)")
+    z = st.domain_nnz / prob.n
+
+    for i in range(args.warmup):
+        dt, st = solve_once(ctx, prob.b)
+        log(f"warmup {i}: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
+    times, hist, steps, iters, walk_ms, launches, tols = [], 0, 0, [], 0.0, 0, []
+    for i in range(args.steps):
+        dt, st = solve_once(ctx, prob.b)
+        times.append(dt)
+        hist += st.histories
+        steps += st.steps
+        iters.append(st.iterations)
+        walk_ms += st.ms_walk
+        launches += st.walk_launches
+        tols.append(st.achieved_tol)
+        log(f"step {i}: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e} "
+            f"walk={st.ms_walk:.1f}ms src={st.ms_source:.1f}ms res={st.ms_residual:.1f}ms")
+    total = sum(times)
+    hps = hist / total
+    prec = FORMS[args.form]["Walk Precision"]
+    bstep = bytes_per_step(prec, z)
+    avg_launch_s = walk_ms / 1e3 / max(launches, 1)
+    steps_per_launch = steps / max(launches, 1)
+    achieved = bstep * steps_per_launch / avg_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": hps,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec == "FP32" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w['label']}", "form": args.form, "n": prob.n,
+                   "nnz": prob.nnz, "histories_per_iteration": w["histories"],
+                   "parallelism": "1 GPU"},
+        "time_to_tol_s": total / args.steps,
+        "mcsa_iterations": iters,
+        "achieved_tol": max(tols),
+        "steps_per_history": steps / max(hist, 1),
+        "walk_steps_per_s": steps / (walk_ms / 1e3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_walk<{'float' if prec == 'FP32' else 'double'}>",
+                     "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
+                     "avg_launch_ms": avg_launch_s * 1e3},
+    }
+    if not args.no_plain and args.form == "chosen":
+        ctx.set_params(FORMS["plain"])
+        dt, st = solve_once(ctx, prob.b)
+        log(f"plain FP64: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
+        out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
+                        "achieved_tol": st.achieved_tol, "histories_per_s": st.histories / dt,
+                        "walk_steps_per_s": st.steps / (st.ms_walk / 1e3),
+                        "roofline_frac": bytes_per_step("FP64", z) * st.steps / (st.ms_walk / 1e3) / 1e9
+                        / HBM_PEAK_GBS}
+        out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
+    ctx.close()
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline()
+        log(f"cpu baseline {out['cpu_baseline']['value']:.3e} histories/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,366 @@
+// MCLS solver API over the gfx950 C ABI (include/mcls_gpu.h).
+//
+// Same class names, constructor shapes, parameter keys and error convention
+// as MCLS (paths relative to the MCLS source root):
+//   LinearProblem                     src/MCLS_LinearProblem.hpp:59-186
+//   SolverManager                     src/MCLS_SolverManager.hpp:60-113
+//   MonteCarloSolverManager<..,AdjointTag>  src/MCLS_MonteCarloSolverManager.hpp:67-220
+//   MCSASolverManager<..,AdjointTag>  src/MCLS_MCSASolverManager.hpp:70-181
+//   MultiSetLinearProblem (1 set)     src/MCLS_MultiSetLinearProblem.hpp
+//   SolverFactory                     src/MCLS_SolverFactory_impl.hpp:58-146
+//   TpetraPointJacobiPreconditioner   tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
+// Non-zero C ABI status is rethrown as MCLS::Assertion.  The domain is
+// rebuilt only when the operator object changes (pointer identity, as
+// src/MCLS_MonteCarloSolverManager_impl.hpp:161-178).  The left
+// preconditioner must be diagonal (point Jacobi); a right preconditioner,
+// the forward method, Anderson and the stand-alone fixed-point manager are
+// not provided by this build and raise MCLS::Assertion.
+#pragma once
+
+#include <cstring>
+#include <string>
+
+#include "MCLS_Core.hpp"
+#include "MCLS_Traits.hpp"
+#include "mcls_gpu.h"
+
+namespace MCLS {
+
+class ForwardTag {};
+class AdjointTag {};
+
+// RAII owner of one mcls_gpu_context.
+class GpuContext {
+ public:
+  explicit GpuContext(int device = 0) { check(mcls_gpu_create(&d_h, device, 0)); }
+  ~GpuContext() { mcls_gpu_destroy(d_h); }
+  GpuContext(const GpuContext&) = delete;
+  GpuContext& operator=(const GpuContext&) = delete;
+  mcls_gpu_context* get() const { return d_h; }
+  static void check(int rc) {
+    if (rc != MCLS_OK) throw Assertion(std::string("mcls_gpu error ") + std::to_string(rc) + ": " + mcls_gpu_last_error());
+  }
+  // Forward a ParameterList; keys the device solver does not know are
+  // ignored, as Teuchos ignores unused entries.
+  void setParameters(const ParameterList& pl) {
+    for (const auto& kv : pl.values()) {
+      const char* k = kv.first.c_str();
+      int rc = MCLS_OK;
+      if (auto* i = std::get_if<int>(&kv.second)) rc = mcls_gpu_set_param_i64(d_h, k, *i);
+      else if (auto* l = std::get_if<int64_t>(&kv.second)) rc = mcls_gpu_set_param_i64(d_h, k, *l);
+      else if (auto* b = std::get_if<bool>(&kv.second)) rc = mcls_gpu_set_param_i64(d_h, k, *b ? 1 : 0);
+      else if (auto* d = std::get_if<double>(&kv.second)) rc = mcls_gpu_set_param_f64(d_h, k, *d);
+      else if (auto* s = std::get_if<std::string>(&kv.second)) rc = mcls_gpu_set_param_str(d_h, k, s->c_str());
+      if (rc == MCLS_ERR_INVALID_ARG && std::strncmp(mcls_gpu_last_error(), "unknown", 7) == 0) continue;
+      check(rc);
+    }
+  }
+
+ private:
+  mcls_gpu_context* d_h = nullptr;
+};
+
+template <class Vector, class Matrix>
+class LinearProblem {
+ public:
+  using VT = VectorTraits<Vector>;
+  using MT = MatrixTraits<Vector, Matrix>;
+  LinearProblem(const RCP<const Matrix>& A, const RCP<Vector>& x, const RCP<const Vector>& b)
+      : d_A(A), d_x(x), d_b(b) {
+    MCLS_INSIST(A && x && b, "null operator or vector");
+    d_r = VT::clone(*x);
+    d_rp = VT::clone(*x);
+  }
+  void setOperator(const RCP<const Matrix>& A) { d_A = A; }
+  void setLHS(const RCP<Vector>& x) { d_x = x; }
+  void setRHS(const RCP<const Vector>& b) { d_b = b; }
+  void setLeftPrec(const RCP<const Matrix>& PL) { d_PL = PL; }
+  void setRightPrec(const RCP<const Matrix>& PR) { d_PR = PR; }
+  const RCP<const Matrix>& getOperator() const { return d_A; }
+  const RCP<Vector>& getLHS() const { return d_x; }
+  const RCP<const Vector>& getRHS() const { return d_b; }
+  const RCP<const Matrix>& getLeftPrec() const { return d_PL; }
+  const RCP<const Matrix>& getRightPrec() const { return d_PR; }
+  bool isLeftPrec() const { return (bool)d_PL; }
+  bool isRightPrec() const { return (bool)d_PR; }
+  const RCP<Vector>& getResidual() const { return d_r; }
+  const RCP<Vector>& getPrecResidual() const { return d_rp; }
+  // r = b - A x  (src/MCLS_LinearProblem_impl.hpp:390-403)
+  void updateResidual() {
+    MT::apply(*d_A, *d_x, *d_r);
+    VT::update(*d_r, -1.0, *d_b, 1.0);
+  }
+  // rp = PL (b - A x)  (src/MCLS_LinearProblem_impl.hpp:411-442)
+  void updatePrecResidual() {
+    MCLS_INSIST(!d_PR, "right preconditioning is not provided by this build");
+    MT::apply(*d_A, *d_x, *d_rp);
+    VT::update(*d_rp, -1.0, *d_b, 1.0);
+    if (d_PL) {
+      auto tmp = VT::deepCopy(*d_rp);
+      MatrixTraits<Vector, Matrix>::apply(*d_PL, *tmp, *d_rp);
+    }
+  }
+
+ private:
+  RCP<const Matrix> d_A;
+  RCP<Vector> d_x;
+  RCP<const Vector> d_b;
+  RCP<const Matrix> d_PL, d_PR;
+  RCP<Vector> d_r, d_rp;
+};
+
+// One set: the build runs on one GPU, so the set sum is the identity.
+template <class Vector, class Matrix>
+class MultiSetLinearProblem {
+ public:
+  MultiSetLinearProblem(int /*num_sets*/, int /*set_id*/, const RCP<LinearProblem<Vector, Matrix>>& p)
+      : d_problem(p) {}
+  const RCP<LinearProblem<Vector, Matrix>>& getProblem() const { return d_problem; }
+  int numSets() const { return 1; }
+  int globalRank() const { return 0; }
+
+ private:
+  RCP<LinearProblem<Vector, Matrix>> d_problem;
+};
+
+template <class Vector, class Matrix>
+class SolverManager {
+ public:
+  virtual ~SolverManager() = default;
+  virtual const LinearProblem<Vector, Matrix>& getProblem() const = 0;
+  virtual RCP<const ParameterList> getValidParameters() const = 0;
+  virtual RCP<const ParameterList> getCurrentParameters() const = 0;
+  virtual double achievedTol() const = 0;
+  virtual int getNumIters() const = 0;
+  virtual void setProblem(const RCP<LinearProblem<Vector, Matrix>>& problem) = 0;
+  virtual void setParameters(const RCP<ParameterList>& params) = 0;
+  virtual bool solve() = 0;
+  virtual bool getConvergedStatus() const = 0;
+};
+
+namespace detail {
+
+// Shared device plumbing of the two managers.
+template <class Vector, class Matrix>
+class DeviceProblem {
+ public:
+  using Problem = LinearProblem<Vector, Matrix>;
+  DeviceProblem() : d_ctx(std::make_shared<GpuContext>(0)) {}
+
+  // Upload the operator when its object changed; refresh PL each call.
+  void sync(const Problem& p, const ParameterList& pl) {
+    MCLS_INSIST(!p.isRightPrec(), "right preconditioning is not provided by this build");
+    const Matrix& A = *p.getOperator();
+    MCLS_INSIST(A.isFillComplete(), "operator must be fill-complete");
+    if (d_op != p.getOperator().get()) {
+      const void* val = A.values().data();
+      int dt = std::is_same<typename Matrix::scalar_type, float>::value ? MCLS_F32 : MCLS_F64;
+      GpuContext::check(mcls_gpu_set_matrix_csr(d_ctx->get(), A.numRows(), A.nnz(), A.rowPtr().data(),
+                                                A.colInd().data(), val, dt, MCLS_HOST));
+      d_op = p.getOperator().get();
+      d_pl = nullptr;
+    }
+    if (p.isLeftPrec()) {
+      if (d_pl != p.getLeftPrec().get()) {
+        std::vector<double> d;
+        MCLS_INSIST(p.getLeftPrec()->isDiagonal(&d), "only a diagonal (point Jacobi) left preconditioner is provided");
+        GpuContext::check(mcls_gpu_set_left_prec_diag(d_ctx->get(), d.data(), MCLS_HOST));
+        d_pl = p.getLeftPrec().get();
+      }
+    } else if (d_pl) {
+      GpuContext::check(mcls_gpu_clear_left_prec(d_ctx->get()));
+      d_pl = nullptr;
+    }
+    d_ctx->setParameters(pl);
+  }
+  mcls_gpu_context* ctx() const { return d_ctx->get(); }
+  mcls_gpu_stats stats() const {
+    mcls_gpu_stats s;
+    GpuContext::check(mcls_gpu_get_stats(d_ctx->get(), &s));
+    return s;
+  }
+
+ private:
+  RCP<GpuContext> d_ctx;
+  const void* d_op = nullptr;
+  const void* d_pl = nullptr;
+};
+
+inline RCP<ParameterList> mcValidParameters() {
+  // src/MCLS_MonteCarloSolverManager_impl.hpp:99-110
+  auto pl = parameterList();
+  pl->set<double>("Sample Ratio", 1.0);
+  pl->set<double>("History Length", 10);
+  pl->set<int>("MC Check Frequency", 1000);
+  pl->set<int>("MC Buffer Size", 1000);
+  pl->set<double>("Neumann Relaxation", 1.0);
+  return pl;
+}
+
+}  // namespace detail
+
+template <class Vector, class Matrix, class MonteCarloTag = AdjointTag>
+class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
+  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value,
+                "only the adjoint Monte Carlo method is provided by this build");
+
+ public:
+  using Problem = LinearProblem<Vector, Matrix>;
+  using VT = VectorTraits<Vector>;
+  using MT = MatrixTraits<Vector, Matrix>;
+
+  MonteCarloSolverManager(const RCP<ParameterList>& plist, int global_rank = 0, bool internal_solver = false)
+      : d_plist(plist), d_internal(internal_solver) {
+    (void)global_rank;
+    MCLS_INSIST(plist, "null parameter list");
+  }
+  MonteCarloSolverManager(const RCP<Problem>& problem, const RCP<ParameterList>& plist, int global_rank = 0,
+                          bool internal_solver = false)
+      : MonteCarloSolverManager(plist, global_rank, internal_solver) {
+    setProblem(problem);
+  }
+  const Problem& getProblem() const override { return *d_problem; }
+  RCP<const ParameterList> getValidParameters() const override { return detail::mcValidParameters(); }
+  RCP<const ParameterList> getCurrentParameters() const override { return d_plist; }
+  // ||b - A x||_inf / ||b||_inf  (src/MCLS_MonteCarloSolverManager_impl.hpp:120-133)
+  double achievedTol() const override {
+    d_problem->updateResidual();
+    return VT::normInf(*d_problem->getResidual()) / VT::normInf(*d_problem->getRHS());
+  }
+  int getNumIters() const override { return 0; }
+  void setProblem(const RCP<Problem>& problem) override {
+    MCLS_INSIST(problem, "null problem");
+    d_problem = problem;
+  }
+  void setParameters(const RCP<ParameterList>& params) override {
+    MCLS_INSIST(params, "null parameter list");
+    d_plist = params;
+  }
+  // x = MC estimate of the solution for the source omega PL b.
+  bool solve() override {
+    MCLS_INSIST(d_problem, "no problem set");
+    d_dev.sync(*d_problem, *d_plist);
+    Vector& x = *d_problem->getLHS();
+    GpuContext::check(mcls_gpu_mc_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
+    return true;  // a direct solve (src/MCLS_MonteCarloSolverManager.hpp:170-171)
+  }
+  bool getConvergedStatus() const override { return true; }
+  mcls_gpu_stats deviceStats() const { return d_dev.stats(); }
+
+ private:
+  RCP<Problem> d_problem;
+  RCP<ParameterList> d_plist;
+  bool d_internal;
+  detail::DeviceProblem<Vector, Matrix> d_dev;
+};
+
+template <class Vector, class Matrix, class MonteCarloTag = AdjointTag>
+class MCSASolverManager : public SolverManager<Vector, Matrix> {
+  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value,
+                "only the adjoint Monte Carlo method is provided by this build");
+
+ public:
+  using Problem = LinearProblem<Vector, Matrix>;
+  using VT = VectorTraits<Vector>;
+
+  // src/MCLS_MCSASolverManager.hpp:95-97 takes a MultiSetLinearProblem; the
+  // plain-problem and parameter-only forms are also accepted (the
+  // reference's SolverFactory path, src/MCLS_SolverFactory_impl.hpp:105-107).
+  explicit MCSASolverManager(const RCP<ParameterList>& plist) : d_plist(plist) {
+    MCLS_INSIST(plist, "null parameter list");
+  }
+  MCSASolverManager(const RCP<Problem>& problem, const RCP<ParameterList>& plist) : MCSASolverManager(plist) {
+    setProblem(problem);
+  }
+  MCSASolverManager(const RCP<MultiSetLinearProblem<Vector, Matrix>>& ms, const RCP<ParameterList>& plist)
+      : MCSASolverManager(ms->getProblem(), plist) {}
+
+  const Problem& getProblem() const override { return *d_problem; }
+  RCP<const ParameterList> getValidParameters() const override {
+    auto pl = detail::mcValidParameters();
+    pl->set<double>("Convergence Tolerance", 1.0);
+    pl->set<int>("Maximum Iterations", 1000);
+    pl->set<int>("Iteration Print Frequency", 10);
+    pl->set<int>("Iteration Check Frequency", 1);
+    pl->set<int>("Smoother Steps", 1);
+    pl->set<std::string>("Fixed Point Type", "Richardson");
+    pl->set<double>("Richardson Relaxation", 1.0);
+    return pl;
+  }
+  RCP<const ParameterList> getCurrentParameters() const override { return d_plist; }
+  double achievedTol() const override { return d_stats.achieved_tol; }
+  int getNumIters() const override { return (int)d_stats.iterations; }
+  void setProblem(const RCP<Problem>& problem) override {
+    MCLS_INSIST(problem, "null problem");
+    d_problem = problem;
+  }
+  void setParameters(const RCP<ParameterList>& params) override {
+    MCLS_INSIST(params, "null parameter list");
+    d_plist = params;
+  }
+  bool solve() override {
+    MCLS_INSIST(d_problem, "no problem set");
+    d_dev.sync(*d_problem, *d_plist);
+    Vector& x = *d_problem->getLHS();
+    GpuContext::check(mcls_gpu_mcsa_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
+    d_stats = d_dev.stats();
+    return d_stats.converged != 0;
+  }
+  bool getConvergedStatus() const override { return d_stats.converged != 0; }
+  mcls_gpu_stats deviceStats() const { return d_stats; }
+
+ private:
+  RCP<Problem> d_problem;
+  RCP<ParameterList> d_plist;
+  detail::DeviceProblem<Vector, Matrix> d_dev;
+  mcls_gpu_stats d_stats{};
+};
+
+// src/MCLS_SolverFactory_impl.hpp:58-146 with the same solver names.  The
+// factory builds managers from a parameter list alone (setProblem before
+// solve), which the reference's factory cannot do for MCSA (the constructor
+// it calls leaves the multi-set problem null).
+template <class Vector, class Matrix>
+class SolverFactory {
+ public:
+  using Solver = SolverManager<Vector, Matrix>;
+  RCP<Solver> create(const std::string& name, const RCP<ParameterList>& params) const {
+    MCLS_INSIST(params, "null parameter list");
+    if (name == "Adjoint MC") return std::make_shared<MonteCarloSolverManager<Vector, Matrix, AdjointTag>>(params);
+    if (name == "Adjoint MCSA") return std::make_shared<MCSASolverManager<Vector, Matrix, AdjointTag>>(params);
+    if (name == "Forward MC" || name == "Forward MCSA" || name == "Adjoint Anderson" ||
+        name == "Forward Anderson" || name == "Fixed Point")
+      throw Assertion("Solver type '" + name + "' is not provided by this build");
+    throw Assertion("Solver type not supported!");
+  }
+};
+
+// Point Jacobi: PL = diag(1/a_ii) as a diagonal CrsMatrix.
+template <class Scalar>
+class PointJacobiPreconditioner {
+ public:
+  void setOperator(const RCP<const CrsMatrix<Scalar>>& A) { d_A = A; }
+  void buildPreconditioner() {
+    MCLS_INSIST(d_A && d_A->isFillComplete(), "operator must be set and fill-complete");
+    const int64_t n = d_A->numRows();
+    Vector d(n);
+    MatrixTraits<Vector, CrsMatrix<Scalar>>::getLocalDiagCopy(*d_A, d);
+    std::vector<int64_t> rp((size_t)n + 1);
+    std::vector<int32_t> col((size_t)n);
+    std::vector<Scalar> val((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      MCLS_INSIST(d[i] != 0.0, "zero diagonal in row " + std::to_string(i));
+      rp[(size_t)i + 1] = i + 1;
+      col[(size_t)i] = (int32_t)i;
+      val[(size_t)i] = (Scalar)(1.0 / d[i]);
+    }
+    d_P = std::make_shared<CrsMatrix<Scalar>>(n, rp, col, val);
+  }
+  RCP<const CrsMatrix<Scalar>> getLeftPreconditioner() const { return d_P; }
+
+ private:
+  RCP<const CrsMatrix<Scalar>> d_A;
+  RCP<const CrsMatrix<Scalar>> d_P;
+};
+
+}  // namespace MCLS

@@ -1,0 +1,167 @@
+// C++ facade tests on the GPU, transcribing the reference's manager tests
+// (paths relative to the MCLS source root):
+//   tpetra/test/tstTpetraMCSASolverManager.cpp:72-203        (adjoint)
+//   tpetra/test/tstTpetraMCSASolverManager.cpp:206-355       (prec_adjoint)
+//   tpetra/test/tstTpetraMonteCarloSolverManager.cpp:73-197  (adjoint MC)
+//   src/MCLS_SolverFactory_impl.hpp:58-146                   (names)
+// Prints one line per check and exits non-zero on the first failure.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "MCLS_Solvers.hpp"
+
+using namespace MCLS;
+using VectorType = Vector;
+using MatrixType = CrsMatrix<double>;
+using VT = VectorTraits<VectorType>;
+using MT = MatrixTraits<VectorType, MatrixType>;
+
+static int g_checks = 0;
+#define CHECK(c)                                                             \
+  do {                                                                       \
+    ++g_checks;                                                              \
+    if (!(c)) {                                                              \
+      std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);               \
+      std::exit(1);                                                          \
+    }                                                                        \
+  } while (0)
+
+// tridiagonal 1 / -0.14 with explicit zeros in the corner rows
+static RCP<MatrixType> build_operator(int n, double diag, double off) {
+  auto A = std::make_shared<MatrixType>(n);
+  A->insertGlobalValues(0, {0, 1, 2}, {diag, off, 0.0});
+  for (int i = 1; i < n - 1; ++i) A->insertGlobalValues(i, {i - 1, i, i + 1}, {off, diag, off});
+  A->insertGlobalValues(n - 1, {n - 3, n - 2, n - 1}, {0.0, off, diag});
+  A->fillComplete();
+  return A;
+}
+
+static bool all_sign(const VectorType& x, double s) {
+  for (int64_t i = 0; i < x.size(); ++i)
+    if (!(x[i] * s > 0.0)) return false;
+  return true;
+}
+
+static void test_mcsa_adjoint(bool prec) {
+  const int n = 10;
+  auto A = build_operator(n, prec ? 2.0 : 1.0, prec ? -0.28 : -0.14);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*x, 0.0);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, -1.0);
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-8);
+  plist->set<int>("Maximum Iterations", 20);
+  plist->set<int>("MC Check Frequency", 50);
+  plist->set<double>("Sample Ratio", 1.0);
+  plist->set<std::string>("Transport Type", "Global");
+  plist->set<int>("Random Seed", 12345);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  if (prec) {
+    PointJacobiPreconditioner<double> pj;
+    pj.setOperator(A);
+    pj.buildPreconditioner();
+    problem->setLeftPrec(pj.getLeftPreconditioner());
+  }
+  auto ms = std::make_shared<MultiSetLinearProblem<VectorType, MatrixType>>(1, 0, problem);
+  MCSASolverManager<VectorType, MatrixType, AdjointTag> solver(ms, plist);
+
+  bool conv = solver.solve();
+  CHECK(conv && solver.getConvergedStatus());
+  CHECK(solver.getNumIters() < 20);
+  CHECK(solver.achievedTol() > 0.0 && solver.achievedTol() <= 1e-8);
+  CHECK(all_sign(*x, -1.0));
+
+  VT::putScalar(*const_cast<VectorType*>(b.get()), 2.0);
+  VT::putScalar(*x, 0.0);
+  problem->setLHS(x);
+  CHECK(solver.solve() && solver.getNumIters() < 20);
+  CHECK(all_sign(*x, 1.0));
+
+  VT::putScalar(*x, 0.0);
+  solver.setProblem(problem);
+  CHECK(solver.solve() && solver.getNumIters() < 20);
+  CHECK(all_sign(*x, 1.0));
+
+  VT::putScalar(*const_cast<VectorType*>(b.get()), -2.0);
+  VT::putScalar(*x, 0.0);
+  CHECK(solver.solve() && solver.getNumIters() < 20);
+  CHECK(all_sign(*x, -1.0));
+  std::printf("ok MCSASolverManager.%s iters=%d tol=%.2e\n", prec ? "prec_adjoint" : "adjoint",
+              solver.getNumIters(), solver.achievedTol());
+}
+
+static void test_mc_adjoint() {
+  const int n = 10;
+  auto A = build_operator(n, 1.0, -0.14);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, -1.0);
+  auto plist = parameterList();
+  plist->set<double>("Sample Ratio", 10.0);
+  plist->set<int>("History Length", 10);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  MonteCarloSolverManager<VectorType, MatrixType, AdjointTag> solver(problem, plist, 0);
+  CHECK(solver.solve() && solver.getConvergedStatus());
+  CHECK(solver.getNumIters() == 0);
+  CHECK(all_sign(*x, -1.0));
+  CHECK(solver.achievedTol() < 0.5);
+  std::printf("ok MonteCarloSolverManager.adjoint tol=%.3f\n", solver.achievedTol());
+}
+
+static void test_factory_and_errors() {
+  SolverFactory<VectorType, MatrixType> factory;
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-8);
+  plist->set<int>("Maximum Iterations", 20);
+  plist->set<int>("Random Seed", 7);
+  auto mcsa = factory.create("Adjoint MCSA", plist);
+  auto mc = factory.create("Adjoint MC", plist);
+  CHECK(mcsa && mc);
+  bool threw = false;
+  try {
+    factory.create("Forward MCSA", plist);
+  } catch (const Assertion&) {
+    threw = true;
+  }
+  CHECK(threw);
+  threw = false;
+  try {
+    factory.create("Nope", plist);
+  } catch (const Assertion&) {
+    threw = true;
+  }
+  CHECK(threw);
+
+  const int n = 10;
+  auto A = build_operator(n, 1.0, -0.14);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, 1.0);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  mcsa->setProblem(problem);
+  CHECK(mcsa->solve() && all_sign(*x, 1.0));
+
+  // bad parameter value -> MCLS::Assertion through the C ABI status
+  auto bad = parameterList();
+  bad->set<std::string>("Estimator Type", "Track Length");
+  mcsa->setParameters(bad);
+  threw = false;
+  try {
+    mcsa->solve();
+  } catch (const Assertion& e) {
+    threw = std::string(e.what()).find("Estimator Type") != std::string::npos;
+  }
+  CHECK(threw);
+  std::printf("ok SolverFactory names and error convention\n");
+}
+
+int main() {
+  test_mcsa_adjoint(false);
+  test_mcsa_adjoint(true);
+  test_mc_adjoint();
+  test_factory_and_errors();
+  std::printf("ALL %d checks passed\n", g_checks);
+  return 0;
+}

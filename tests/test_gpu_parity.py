@@ -134,8 +134,10 @@ def test_mc_tally_matches_oracle(ctx, prec, est, source):
 
 def test_mcsa_converges_poisson(ctx):
     prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 24, 24, seed=0))
+    # 4000 histories leave the MC noise above the residual on this operator
+    # (MCSA stagnates, on the oracle too); 20000 converge in ~22 iterations.
     _setup(ctx, prob, False, params={"History Length": 5000, "Weight Cutoff": 1e-4,
-                                     "Number of Histories": 4000, "Convergence Tolerance": 1e-8,
+                                     "Number of Histories": 20000, "Convergence Tolerance": 1e-8,
                                      "Maximum Iterations": 200, "Random Seed": 3})
     x = ctx.mcsa_solve(prob.b)
     st = ctx.stats()
@@ -145,7 +147,7 @@ def test_mcsa_converges_poisson(ctx):
     assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
     # the oracle's MCSA with the same streams takes the same path to tolerance
     A = _csr(prob)
-    mp = po.mc_params(4000, 5000, 1e-4, 64, seed=3)
+    mp = po.mc_params(20000, 5000, 1e-4, 64, seed=3)
     xo, res, hist = po.mcsa_solve(A, prob.b, dinv=po.jacobi_dinv(A),
                                   params=po.solver_params(tol=1e-8, max_iters=200, mc=mp),
                                   solve_index0=st.solve_index - st.iterations)
