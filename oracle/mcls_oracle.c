@@ -217,6 +217,52 @@ done:
   return ret;
 }
 
+/* Vose's alias method per domain row (M. D. Vose, "A linear algorithm for
+ * generating random numbers with a given distribution", IEEE TSE 17(9),
+ * 1991), run sequentially per row in a fixed order so that the CPU and the
+ * GPU tables are bit-identical.  Build extension: the reference samples
+ * rows only by CDF inversion. */
+int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
+                        const double* d_W, double* thr, int* alias) {
+  double s[64];
+  int small[64], large[64];
+  for (int i = 0; i < n; ++i) {
+    int64_t beg = d_rp[i];
+    int z = (int)(d_rp[i + 1] - beg);
+    if (z > 64) return -(int64_t)(i + 1);
+    int ns = 0, nl = 0;
+    for (int k = 0; k < z; ++k) {
+      s[k] = (fabs(d_h[beg + k]) / d_W[i]) * (double)z;
+      if (s[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+      int l = small[--ns];
+      int g = large[--nl];
+      thr[beg + l] = s[l];
+      alias[beg + l] = g;
+      s[g] = (s[g] + s[l]) - 1.0;
+      if (s[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+    }
+    while (nl > 0) { int g = large[--nl]; thr[beg + g] = 1.0; alias[beg + g] = g; }
+    while (ns > 0) { int l = small[--ns]; thr[beg + l] = 1.0; alias[beg + l] = l; }
+  }
+  return 0;
+}
+
+/* Alias draw: the high 32 bits pick the slot by multiply-shift, the low 32
+ * bits decide between the slot and its alias. */
+static inline int alias_pick64(uint64_t r, int len, const double* thr, const int* alias) {
+  uint32_t slot = (uint32_t)(((r >> 32) * (uint64_t)len) >> 32);
+  double u2 = (double)(uint32_t)r * 0x1.0p-32;
+  return u2 < thr[slot] ? (int)slot : alias[slot];
+}
+
+static inline int alias_pick32(uint64_t r, int len, const float* thr, const int* alias) {
+  uint32_t slot = (uint32_t)(((r >> 32) * (uint64_t)len) >> 32);
+  float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
+  return u2 < thr[slot] ? (int)slot : alias[slot];
+}
+
 /* =============================================================== source == */
 
 static int ceil_log2_u64(uint64_t v) {
@@ -299,6 +345,9 @@ typedef struct {
   float* h32;
   float* cdf32;
   float* W32;
+  double* thr;   /* alias tables (sampler == ORC_SAMPLER_ALIAS) */
+  float* thr32;
+  int* alias;
 } domain_view;
 
 typedef struct {
@@ -352,8 +401,9 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
         x[state] += (double)w;
       }
       if (len == 0) break;
-      float u = orc_u01_f32(orc_xorshift64(&s));
-      int k = orc_lower_bound_f32(d->cdf32 + beg, len, u);
+      uint64_t r = orc_xorshift64(&s);
+      int k = d->alias ? alias_pick32(r, len, d->thr32 + beg, d->alias + beg)
+                       : orc_lower_bound_f32(d->cdf32 + beg, len, orc_u01_f32(r));
       w = w * d->W32[state];
       if (d->h32[beg + k] < 0.0f) w = -w;
       state = d->col[beg + k];
@@ -375,8 +425,9 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
         x[state] += w;
       }
       if (len == 0) break;
-      double u = orc_u01_f64(orc_xorshift64(&s));
-      int k = orc_lower_bound_f64(d->cdf + beg, len, u);
+      uint64_t r = orc_xorshift64(&s);
+      int k = d->alias ? alias_pick64(r, len, d->thr + beg, d->alias + beg)
+                       : orc_lower_bound_f64(d->cdf + beg, len, orc_u01_f64(r));
       w = w * d->W[state];
       if (d->h[beg + k] < 0.0) w = -w;
       state = d->col[beg + k];
@@ -406,7 +457,8 @@ static void* walk_worker(void* arg) {
 
 static void domain_view_init(domain_view* d, int n, const int64_t* rp,
                              const int* col, const double* h,
-                             const double* cdf, const double* W, int need32) {
+                             const double* cdf, const double* W, int need32,
+                             int need_alias) {
   d->n = n;
   d->rp = rp;
   d->col = col;
@@ -414,6 +466,17 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
   d->cdf = cdf;
   d->W = W;
   d->h32 = d->cdf32 = d->W32 = NULL;
+  d->thr = NULL;
+  d->thr32 = NULL;
+  d->alias = NULL;
+  if (need_alias) {
+    int64_t nnz = rp[n];
+    d->thr = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    d->thr32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
+    d->alias = (int*)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    orc_build_alias(n, rp, h, W, d->thr, d->alias);
+    for (int64_t k = 0; k < nnz; ++k) d->thr32[k] = (float)d->thr[k];
+  }
   if (need32) {
     int64_t nnz = rp[n];
     d->h32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
@@ -429,6 +492,9 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
 }
 
 static void domain_view_free(domain_view* d) {
+  free(d->thr);
+  free(d->thr32);
+  free(d->alias);
   free(d->h32);
   free(d->cdf32);
   free(d->W32);
@@ -442,7 +508,8 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
                      int64_t rec_first, int64_t rec_count, int max_len,
                      int* paths, double* weights) {
   domain_view d;
-  domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32);
+  domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32,
+                   p->sampler == ORC_SAMPLER_ALIAS);
   uint64_t* q = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* qcdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* ncdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));

@@ -68,9 +68,16 @@ int orc_source_birth(int n, const double* b, const uint64_t* q,
                      const uint64_t* cdf, uint64_t Q, uint64_t* rng,
                      double* w0);
 
+/* Per-row alias tables (Vose) over p_k = |h_k| / W_i: thr[nnz] (slot keeps
+ * its own entry when u < thr) and alias[nnz] (row-local index taken
+ * otherwise).  Rows longer than 64 entries return -(row+1). */
+int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
+                        const double* d_W, double* thr, int* alias);
+
 /* ------------------------------------------------------------- MC solve -- */
 enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1 };
 enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
+enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1 };
 
 typedef struct orc_mc_params {
   int64_t num_histories;   /* histories per MC solve                       */
@@ -81,6 +88,7 @@ typedef struct orc_mc_params {
   int64_t source;          /* ORC_SRC_*                                    */
   uint64_t seed;
   int64_t num_threads;     /* worker threads (tallies reduced in order)    */
+  int64_t sampler;         /* ORC_SAMPLER_* (transition sampling)          */
 } orc_mc_params;
 
 /* One adjoint MC solve over the domain: x = (1/nh) * sum of tallies (plus

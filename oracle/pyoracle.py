@@ -16,6 +16,8 @@ EST_COLLISION = 0
 EST_EXPECTED_VALUE = 1
 SRC_RANDOM = 0
 SRC_STRATIFIED = 1
+SAMPLER_CDF = 0
+SAMPLER_ALIAS = 1
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -34,6 +36,7 @@ class MCParams(C.Structure):
         ("source", C.c_int64),
         ("seed", C.c_uint64),
         ("num_threads", C.c_int64),
+        ("sampler", C.c_int64),
     ]
 
 
@@ -95,6 +98,8 @@ def lib():
     L.orc_build_domain.argtypes = [i32, _i64p, _i32p, _f64p, vp, f64, f64,
                                    _i64p, _i32p, _f64p, _f64p, _f64p]
     L.orc_build_domain.restype = i64
+    L.orc_build_alias.argtypes = [i32, _i64p, _f64p, _f64p, _f64p, _i32p]
+    L.orc_build_alias.restype = i64
     L.orc_jacobi_dinv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p]
     L.orc_jacobi_dinv.restype = i32
     L.orc_source_quantize.argtypes = [i32, _f64p, _u64p, _u64p]
@@ -209,10 +214,20 @@ def build_domain(A, dinv=None, omega=1.0, threshold=0.0):
 
 
 def mc_params(num_histories, history_length=10, weight_cutoff=0.0, precision=64,
-              estimator=EST_COLLISION, source=SRC_RANDOM, seed=0, num_threads=1):
+              estimator=EST_COLLISION, source=SRC_RANDOM, seed=0, num_threads=1,
+              sampler=SAMPLER_CDF):
     return MCParams(int(num_histories), int(history_length), float(weight_cutoff),
                     int(precision), int(estimator), int(source), int(seed) & (2**64 - 1),
-                    int(num_threads))
+                    int(num_threads), int(sampler))
+
+
+def build_alias(dom):
+    thr = np.zeros(max(dom.nnz, 1))
+    alias = np.zeros(max(dom.nnz, 1), np.int32)
+    rc = lib().orc_build_alias(dom.n, dom.rp, dom.h, dom.W, thr, alias)
+    if rc != 0:
+        raise ValueError(f"row {-rc - 1} longer than 64 entries")
+    return thr[: dom.nnz], alias[: dom.nnz]
 
 
 def mc_solve(dom, b, params, solve_index=0):
