@@ -8,14 +8,15 @@ tolerance, with the matrix, the domain and b resident in HBM.
 
 Forms:
   chosen: FP32 walks and tallies, FP64 residual/update/convergence test,
-          stratified source, relative weight cutoff 1e-2.
-  plain:  FP64 walks and tallies, random source, weight cutoff 1e-4.
+          stratified source, alias transition sampling, relative weight
+          cutoff 1e-2.
+  plain:  FP64 walks and tallies, random source, CDF inversion (the
+          reference's sampler), weight cutoff 1e-4.
 value = histories/s of the chosen form over the K timed solves.
 
-The walk-kernel roofline uses the per-step algorithmic bytes of the CDF
-walk, B = 8 + p z + 4 + p + 2p (p = 4 or 8 bytes, z = mean domain row
-length), times the transitions of one launch, divided by that launch's
-HIP-event duration measured by the library on its own stream.
+The walk-kernel roofline uses the per-step algorithmic bytes of the walk
+(bytes_per_step below) times the transitions of one launch, divided by that
+launch's HIP-event duration measured by the library on its own stream.
 
 cpu_baseline: the oracle's restatement of the MCLS reference algorithm
 (FP64, per-row CDF + lower_bound, one Xorshift stream per set, multi-set
@@ -54,9 +55,11 @@ WORKLOADS = {
 
 FORMS = {
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
-               "Estimator Type": "Collision", "Weight Cutoff": 1e-2, "History Length": 1000},
+               "Estimator Type": "Collision", "Weight Cutoff": 1e-2, "History Length": 1000,
+               "Transition Sampler": "Alias"},
     "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random",
-              "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000},
+              "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000,
+              "Transition Sampler": "CDF"},
 }
 
 
@@ -74,9 +77,18 @@ def solve_once(ctx, b):
     return dt, ctx.stats()
 
 
-def bytes_per_step(prec, z):
+def bytes_per_step(prec, z, sampler):
+    """Algorithmic bytes of one walk step (one tally + one transition).
+
+    CDF:   16 (packed row header: start, len, W) + p z (CDF row) + 4 (packed
+           column) + 2p (tally read-modify-write)
+    Alias: 16 (row header) + 16 (slot record) + 2p (tally)
+    p = 4 (FP32 walk) or 8 (FP64 walk); z = mean domain row length.
+    """
     p = 4 if prec == "FP32" else 8
-    return 8 + p * z + 4 + p + 2 * p
+    if sampler == "Alias":
+        return 16 + 16 + 2 * p
+    return 16 + p * z + 4 + 2 * p
 
 
 def cpu_baseline(seconds_budget=20.0):
@@ -116,7 +128,19 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--form", default="chosen", choices=sorted(FORMS))
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 1 GiB read + 1 GiB write (elementwise copy kernel)")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(set replication over RCCL is a design note only); run with --gpus 1",
@@ -159,7 +183,7 @@ def main():
     total = sum(times)
     hps = hist / total
     prec = FORMS[args.form]["Walk Precision"]
-    bstep = bytes_per_step(prec, z)
+    bstep = bytes_per_step(prec, z, FORMS[args.form]["Transition Sampler"])
     avg_launch_s = walk_ms / 1e3 / max(launches, 1)
     steps_per_launch = steps / max(launches, 1)
     achieved = bstep * steps_per_launch / avg_launch_s / 1e9
@@ -197,7 +221,8 @@ def main():
         out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
                         "achieved_tol": st.achieved_tol, "histories_per_s": st.histories / dt,
                         "walk_steps_per_s": st.steps / (st.ms_walk / 1e3),
-                        "roofline_frac": bytes_per_step("FP64", z) * st.steps / (st.ms_walk / 1e3) / 1e9
+                        "roofline_frac": bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
+                        * st.steps / (st.ms_walk / 1e3) / 1e9
                         / HBM_PEAK_GBS}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
     ctx.close()

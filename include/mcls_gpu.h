@@ -114,6 +114,8 @@ int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kin
  * Build-only keys:
  *   i64 "Number of Histories"  (0 = N * Sample Ratio)
  *   str "Walk Precision"       ("FP64" | "FP32")
+ *   str "Transition Sampler"   ("CDF" | "Alias"): row CDF inversion (the
+ *        reference's lower_bound) or Vose alias tables (rows <= 64 entries)
  *   i64 "Random Seed"          (0)
  *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
  *   i64 "Verbosity"            (0)

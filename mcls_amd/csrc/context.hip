@@ -87,6 +87,7 @@ struct Params {
   int64_t verbosity = 0;
   int64_t walk_block = 256;
   int64_t walk_blocks_per_cu = 0;
+  int sampler = mcls::SAMPLER_CDF;
 };
 
 }  // namespace
@@ -116,13 +117,15 @@ struct mcls_gpu_context {
   DBuf<uint32_t> d_col;
   DBuf<double> d_cdf64, d_h64, d_W64;
   DBuf<float> d_cdf32, d_h32, d_W32;
+  DBuf<uint4> d_hdr32, d_hdr64, d_slot32, d_slot64;
+  bool alias_valid = false;
 
   // work vectors
   DBuf<double> x, b, r, rhs, part, scal;
   DBuf<double> t64;
   DBuf<float> t32;
   DBuf<uint64_t> q, qcdf, ncnt, ncdf, scan_tmp;
-  DBuf<unsigned long long> bmax, steps;
+  DBuf<unsigned long long> bmax, steps, queue;
   DBuf<int> flags;
   double* host_scal = nullptr;  // pinned, 8 doubles
 
@@ -202,6 +205,7 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->scan_tmp.ensure(mcls::scan_tmp_elems((int64_t)n) + 1, "scan tmp"));
   MCLS_TRY(c->bmax.ensure(1, "bmax"));
   MCLS_TRY(c->steps.ensure(1, "steps"));
+  MCLS_TRY(c->queue.ensure(mcls::kQueueWords, "queue"));
   if (c->prm.precision == 32) {
     if (!c->t32.p || c->t32.n < n) {
       MCLS_TRY(c->t32.ensure(n, "tally32"));
@@ -279,13 +283,43 @@ int build_domain(mcls_gpu_context* c) {
   HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->stats.ms_domain = ms;
   c->stats.domain_nnz = dnnz;
+  MCLS_TRY(c->d_hdr32.ensure((size_t)n, "row headers 32"));
+  MCLS_TRY(c->d_hdr64.ensure((size_t)n, "row headers 64"));
+  HIP_TRY(mcls::launch_pack_rows(n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, c->d_hdr32.p,
+                                 c->d_hdr64.p, nullptr, nullptr, false, c->flags.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c->alias_valid = false;
   c->domain_valid = true;
   c->domain_omega = omega;
   c->domain_threshold = thr;
   return MCLS_OK;
 }
 
+// Alias slot records, built on first use of the alias sampler.
+int ensure_alias(mcls_gpu_context* c) {
+  if (c->prm.sampler != mcls::SAMPLER_ALIAS || c->alias_valid) return MCLS_OK;
+  const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
+  MCLS_TRY(c->d_slot32.ensure(m, "alias slots 32"));
+  MCLS_TRY(c->d_slot64.ensure(m, "alias slots 64"));
+  MCLS_TRY(reset_flags(c));
+  HIP_TRY(mcls::launch_pack_rows(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, c->d_hdr32.p,
+                                 c->d_hdr64.p, c->d_slot32.p, c->d_slot64.p, true, c->flags.p, c->stream));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_LONG_ROW] != INT_MAX)
+    return fail(MCLS_ERR_UNSUPPORTED, "alias sampler: domain row " + std::to_string(flags[mcls::FLAG_LONG_ROW] - 1) +
+                                          " has more than 64 entries (use the CDF sampler)");
+  c->alias_valid = true;
+  return MCLS_OK;
+}
+
+int ensure_domain_impl(mcls_gpu_context* c);
 int ensure_domain(mcls_gpu_context* c) {
+  MCLS_TRY(ensure_domain_impl(c));
+  return ensure_alias(c);
+}
+
+int ensure_domain_impl(mcls_gpu_context* c) {
   if (c->domain_valid && c->domain_omega == c->prm.neumann_omega && c->domain_threshold == c->prm.threshold)
     return MCLS_OK;
   return build_domain(c);
@@ -303,13 +337,16 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   wc.precision = c->prm.precision;
   wc.estimator = c->prm.estimator;
   wc.source = c->prm.source;
+  wc.sampler = c->prm.sampler;
   wc.record = record;
   wc.block = (int)c->prm.walk_block;
   int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
   int64_t cap = (int64_t)c->num_cus * per_cu;
   int64_t need = (nh + wc.block - 1) / wc.block;
   if (need < 1) need = 1;
-  wc.grid = (int)(need < cap ? need : cap);
+  int64_t g = need < cap ? need : cap;
+  g = (g + mcls::kQueueSlots - 1) / mcls::kQueueSlots * mcls::kQueueSlots;
+  wc.grid = (int)g;
   return wc;
 }
 
@@ -324,6 +361,11 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.W32 = c->d_W32.p;
   a.d.h64 = c->d_h64.p;
   a.d.h32 = c->d_h32.p;
+  a.d.hdr32 = c->d_hdr32.p;
+  a.d.hdr64 = c->d_hdr64.p;
+  a.d.slot32 = c->d_slot32.p;
+  a.d.slot64 = c->d_slot64.p;
+  a.queue = c->queue.p;
   a.rhs = rhs;
   a.q = c->q.p;
   a.qcdf = c->qcdf.p;
@@ -659,6 +701,10 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     if (key_is(v, "Collision")) p.estimator = mcls::EST_COLLISION;
     else if (key_is(v, "Expected Value")) p.estimator = mcls::EST_EXPECTED_VALUE;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Estimator Type '") + v + "'");
+  } else if (key_is(key, "Transition Sampler")) {
+    if (key_is(v, "CDF")) p.sampler = mcls::SAMPLER_CDF;
+    else if (key_is(v, "Alias")) p.sampler = mcls::SAMPLER_ALIAS;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Transition Sampler '") + v + "'");
   } else if (key_is(key, "Walk Precision")) {
     if (key_is(v, "FP64")) p.precision = 64;
     else if (key_is(v, "FP32")) p.precision = 32;

@@ -109,39 +109,73 @@ template <> struct Tables<float> {
   __device__ static float u01(uint64_t r) { return u01_f32(r); }
 };
 
+template <typename Real> struct Row;
+template <> struct Row<double> {
+  __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr64[i]; }
+  __device__ static double W(uint4 h) { return __hiloint2double((int)h.w, (int)h.z); }
+  // alias draw: slot from the high 32 bits, keep-or-alias from the low 32
+  __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r) {
+    const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
+    const uint4 s = d.slot64[start + slot];
+    const double u2 = (double)(uint32_t)r * 0x1.0p-32;
+    return u2 < __hiloint2double((int)s.y, (int)s.x) ? s.z : s.w;
+  }
+};
+template <> struct Row<float> {
+  __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr32[i]; }
+  __device__ static float W(uint4 h) { return __uint_as_float(h.z); }
+  __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r) {
+    const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
+    const uint4 s = d.slot32[start + slot];
+    const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
+    return u2 < __uint_as_float(s.x) ? s.y : s.z;
+  }
+};
+
 // --------------------------------------------------------------------------
-// Persistent walk kernel.  Each lane owns one live history at a time and
-// refills itself from a static grid-stride schedule the moment its history
-// ends, so lanes never wait on the longest walk of their wave.  Histories
-// follow the reference loop order: tally at the current state, sample the
-// transition from the row CDF, multiply the weight by W_i sign(h_ij), stop
-// when steps >= L or |w| < cutoff |w0|.
+// Walk kernel.  Each lane owns one live history at a time.  Lanes whose
+// history ended are refilled together from a per-XCD work queue: the wave
+// takes batches of kBatch consecutive history ids with one atomic, so the
+// walks in flight on an XCD were born next to each other (the source emits
+// history ids in state order) and share L2 / MALL lines.  Histories follow
+// the reference loop order: tally at the current state, sample the
+// transition (row CDF inversion or alias slot), multiply the weight by
+// W_i sign(h_ij), stop when steps >= L or |w| < cutoff |w0|.
 // --------------------------------------------------------------------------
-template <typename Real, int EST, int SRC, bool REC>
+constexpr int kBatch = 64;
+
+template <typename Real, int EST, int SRC, int SAMP, bool REC>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const DomainView& d = a.d;
   const int n = d.n;
   const Real* __restrict__ cdf = Tables<Real>::cdf(d);
-  const Real* __restrict__ Wt = Tables<Real>::W(d);
   const Real* __restrict__ ht = Tables<Real>::h(d);
   Real* __restrict__ tally = Tables<Real>::tally(a);
-  const int* __restrict__ rp = d.rp;
   const uint32_t* __restrict__ colp = d.col;
 
-  int64_t last = a.last;
+  int64_t hi_id = a.last;
   uint64_t total = 0;  // Q (random) or M (stratified)
   if (SRC == SRC_RANDOM) {
     total = a.qcdf[n - 1];
     if (total == 0) return;
   } else {
     total = a.ncdf[n - 1];
-    if ((int64_t)total < last) last = (int64_t)total;
+    if ((int64_t)total < hi_id) hi_id = (int64_t)total;
   }
+  const int64_t span = hi_id - a.first;
+  if (span <= 0) return;
   const double totald = (double)total;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t hid = a.first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x - stride;
+  // workgroups are dispatched round-robin over the 8 XCDs: slot = XCD
+  const int qs = blockIdx.x % kQueueSlots;
+  const int64_t sbeg = a.first + span * qs / kQueueSlots;
+  const int64_t send = a.first + span * (qs + 1) / kQueueSlots;
+  unsigned long long* ctr = a.queue + qs * kQueueStride;
+  const int lane = __lane_id();
 
+  int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
+  bool exhausted = false;       // wave-uniform
   bool alive = false;
+  int64_t hid = 0;
   int state = 0;
   Real w = 0, wc = 0;
   int64_t steps = 0;
@@ -150,66 +184,102 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   int* path = nullptr;
 
   for (;;) {
-    if (!alive) {
-      if (REC && path) {
-        for (int k = (int)steps + 1; k <= a.max_len; ++k) path[k] = -1;
-        a.wfinal[hid - a.first] = (double)w;
-      }
-      hid += stride;
-      if (hid >= last) break;
-      rng = history_state(a.key, (uint64_t)hid);
-      double w0;
-      if (SRC == SRC_RANDOM) {
-        uint64_t t = mul_u53(xorshift64(rng), total);
-        state = upper_bound_u64(a.qcdf, n, t);
-        w0 = a.rhs[state] * (totald / (double)a.q[state]);
-      } else {
-        state = upper_bound_u64(a.ncdf, n, (uint64_t)hid);
-        uint64_t nk = a.ncdf[state] - (state ? a.ncdf[state - 1] : 0);
-        w0 = a.rhs[state] * (totald / (double)nk);
-      }
-      w = (Real)w0;
-      wc = (Real)a.cutoff * (w < 0 ? -w : w);
-      steps = 0;
-      alive = true;
-      if (REC) {
-        path = a.paths + (size_t)(hid - a.first) * (size_t)(a.max_len + 1);
-        path[0] = state;
+    if (!exhausted) {
+      uint64_t need = __ballot(!alive);
+      while (need != 0) {
+        if (qnext >= qend) {
+          const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+          unsigned long long b = 0;
+          if (lane == leader) b = atomicAdd(ctr, (unsigned long long)kBatch);
+          b = __shfl(b, leader);
+          qnext = sbeg + (int64_t)b;
+          qend = qnext + kBatch < send ? qnext + kBatch : send;
+          if (qnext >= send) {
+            exhausted = true;
+            break;
+          }
+        }
+        const int64_t avail = qend - qnext;
+        const int cnt = __popcll(need);
+        const int take = (int64_t)cnt < avail ? cnt : (int)avail;
+        const int rank = __popcll(need & __lanemask_lt());
+        if (!alive && rank < take) {
+          hid = qnext + rank;
+          rng = history_state(a.key, (uint64_t)hid);
+          double w0;
+          if (SRC == SRC_RANDOM) {
+            const uint64_t t = mul_u53(xorshift64(rng), total);
+            state = upper_bound_u64(a.qcdf, n, t);
+            w0 = a.rhs[state] * (totald / (double)a.q[state]);
+          } else {
+            state = upper_bound_u64(a.ncdf, n, (uint64_t)hid);
+            const uint64_t nk = a.ncdf[state] - (state ? a.ncdf[state - 1] : 0);
+            w0 = a.rhs[state] * (totald / (double)nk);
+          }
+          w = (Real)w0;
+          wc = (Real)a.cutoff * (w < 0 ? -w : w);
+          steps = 0;
+          alive = true;
+          if (REC) {
+            path = a.paths + (size_t)(hid - a.first) * (size_t)(a.max_len + 1);
+            path[0] = state;
+          }
+        }
+        qnext += take;
+        need = __ballot(!alive);
       }
     }
-    const int beg = rp[state];
-    const int len = rp[state + 1] - beg;
+    if (!alive) break;  // queue exhausted and no history left on this lane
+
+    const uint4 hd = Row<Real>::hdr(d, state);
+    const uint32_t beg = hd.x, len = hd.y;
     if (EST == EST_EXPECTED_VALUE) {
-      for (int k = 0; k < len; ++k)
-        tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], w * ht[beg + k]);
+      for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], w * ht[beg + k]);
     } else {
       tally_add(&tally[state], w);
     }
-    if (len == 0) {  // absorbing row (all of h dropped)
-      alive = false;
-      continue;
+    bool end = len == 0;  // absorbing row (all of h dropped)
+    if (!end) {
+      const uint64_t r = xorshift64(rng);
+      uint32_t cp;
+      if (SAMP == SAMPLER_ALIAS) {
+        cp = Row<Real>::pick(d, beg, len, r);
+      } else {
+        const int k = lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
+        cp = colp[beg + k];
+      }
+      w = w * Row<Real>::W(hd);
+      if (cp & 0x80000000u) w = -w;
+      state = (int)(cp & 0x7FFFFFFFu);
+      ++steps;
+      ++my_steps;
+      if (REC && steps <= a.max_len) path[steps] = state;
+      end = steps >= a.history_length || (w < 0 ? -w : w) < wc;
     }
-    const Real u = Tables<Real>::u01(xorshift64(rng));
-    const int k = lower_bound_cdf<Real>(cdf + beg, len, u);
-    const uint32_t cp = colp[beg + k];
-    w = w * Wt[state];
-    if (cp & 0x80000000u) w = -w;
-    state = (int)(cp & 0x7FFFFFFFu);
-    ++steps;
-    ++my_steps;
-    if (REC && steps <= a.max_len) path[steps] = state;
-    if (steps >= a.history_length || (w < 0 ? -w : w) < wc) alive = false;
+    if (end) {
+      alive = false;
+      if (REC) {
+        for (int k = (int)steps + 1; k <= a.max_len; ++k) path[k] = -1;
+        a.wfinal[hid - a.first] = (double)w;
+      }
+    }
   }
   if (my_steps) atomicAdd(a.steps, (unsigned long long)my_steps);
 }
 
+template <typename Real, int EST, int SRC, int SAMP>
+hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if (c.record)
+    k_walk<Real, EST, SRC, SAMP, true><<<c.grid, c.block, 0, s>>>(a);
+  else
+    k_walk<Real, EST, SRC, SAMP, false><<<c.grid, c.block, 0, s>>>(a);
+  return hipGetLastError();
+}
+
 template <typename Real, int EST, int SRC>
 hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
-  if (c.record)
-    k_walk<Real, EST, SRC, true><<<c.grid, c.block, 0, s>>>(a);
-  else
-    k_walk<Real, EST, SRC, false><<<c.grid, c.block, 0, s>>>(a);
-  return hipGetLastError();
+  return c.sampler == SAMPLER_ALIAS ? launch_walk_s<Real, EST, SRC, SAMPLER_ALIAS>(a, c, s)
+                                    : launch_walk_s<Real, EST, SRC, SAMPLER_CDF>(a, c, s);
 }
 
 template <typename Real>
@@ -544,6 +614,60 @@ __global__ void k_domain_fill(int n, const int* __restrict__ t_rp, const int* __
   }
 }
 
+// Row headers and per-row alias tables (Vose's method, sequential per row in
+// the same operation order as the CPU restatement, so the tables are
+// bit-identical).
+__global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                            const double* __restrict__ h64, const double* __restrict__ W64,
+                            const float* __restrict__ W32, uint4* __restrict__ hdr32, uint4* __restrict__ hdr64,
+                            uint4* __restrict__ slot32, uint4* __restrict__ slot64, int alias, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = d_rp[i];
+    const int z = d_rp[i + 1] - beg;
+    const double W = W64[i];
+    hdr32[i] = make_uint4((uint32_t)beg, (uint32_t)z, __float_as_uint(W32[i]), 0u);
+    hdr64[i] = make_uint4((uint32_t)beg, (uint32_t)z, (uint32_t)__double2loint(W), (uint32_t)__double2hiint(W));
+    if (!alias) continue;
+    if (z > 64) {
+      set_flag(flags, FLAG_LONG_ROW, i);
+      continue;
+    }
+    double s[64];
+    int small[64], large[64], al[64];
+    double thr[64];
+    int ns = 0, nl = 0;
+    for (int k = 0; k < z; ++k) {
+      s[k] = (fabs(h64[beg + k]) / W) * (double)z;
+      if (s[k] < 1.0) small[ns++] = k;
+      else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+      const int l = small[--ns];
+      const int g = large[--nl];
+      thr[l] = s[l];
+      al[l] = g;
+      s[g] = (s[g] + s[l]) - 1.0;
+      if (s[g] < 1.0) small[ns++] = g;
+      else large[nl++] = g;
+    }
+    while (nl > 0) {
+      const int g = large[--nl];
+      thr[g] = 1.0;
+      al[g] = g;
+    }
+    while (ns > 0) {
+      const int l = small[--ns];
+      thr[l] = 1.0;
+      al[l] = l;
+    }
+    for (int k = 0; k < z; ++k) {
+      const uint32_t self = col[beg + k], other = col[beg + al[k]];
+      slot32[beg + k] = make_uint4(__float_as_uint((float)thr[k]), self, other, 0u);
+      slot64[beg + k] = make_uint4((uint32_t)__double2loint(thr[k]), (uint32_t)__double2hiint(thr[k]), self, other);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- linalg
 template <typename AV>
 __global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int* __restrict__ rp,
@@ -642,7 +766,16 @@ __global__ void k_tally_out(int n, double* __restrict__ x, double* __restrict__ 
 // ============================================================== launchers
 
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  k_fill<unsigned long long><<<1, kQueueWords, 0, s>>>(a.queue, kQueueWords, 0ULL);
   return c.precision == 32 ? launch_walk_p<float>(a, c, s) : launch_walk_p<double>(a, c, s);
+}
+
+hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                            const float* W32, uint4* hdr32, uint4* hdr64, uint4* slot32, uint4* slot64,
+                            bool alias, int* flags, hipStream_t s) {
+  k_pack_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, hdr32, hdr64, slot32, slot64,
+                                                alias ? 1 : 0, flags);
+  return hipGetLastError();
 }
 
 hipError_t launch_absmax(const double* v, int n, unsigned long long* bmax, hipStream_t s) {

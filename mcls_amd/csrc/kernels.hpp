@@ -13,11 +13,19 @@ enum DeviceFlag : int {
   FLAG_DUPLICATE = 1,    // duplicate column within a row
   FLAG_ZERO_DIAG = 2,    // zero / missing diagonal under point Jacobi
   FLAG_NONFINITE = 3,    // NaN / Inf in a matrix value
+  FLAG_LONG_ROW = 4,     // domain row longer than the alias builder handles
   FLAG_COUNT = 8
 };
 
 // Adjoint domain in HBM: rows of H^T (CSR).  col packs the column index in
 // bits 0..30 and the sign of h in bit 31 (1 = negative).
+// Packed per-row headers (one 16-byte load per step):
+//   hdr32[i] = {start, len, bits(W32_i), 0}
+//   hdr64[i] = {start, len, lo(W64_i), hi(W64_i)}
+// Alias slot records (one 16-byte load per step), packed columns carry the
+// sign of h in bit 31:
+//   slot32[k] = {bits((float)thr_k), col_k, col_alias(k), 0}
+//   slot64[k] = {lo(thr_k), hi(thr_k), col_k, col_alias(k)}
 struct DomainView {
   int n;
   const int* rp;          // n+1
@@ -28,10 +36,20 @@ struct DomainView {
   const float* W32;       // n
   const double* h64;      // nnz
   const float* h32;       // nnz
+  const uint4* hdr32;     // n
+  const uint4* hdr64;     // n
+  const uint4* slot32;    // nnz (alias sampler)
+  const uint4* slot64;    // nnz (alias sampler)
 };
 
 enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
+enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1 };
+
+// Work queue of the walk kernel: one counter per XCD slot, 128 bytes apart.
+constexpr int kQueueSlots = 8;
+constexpr int kQueueStride = 16;  // uint64 words between counters
+constexpr int kQueueWords = kQueueSlots * kQueueStride;
 
 struct WalkArgs {
   DomainView d;
@@ -47,6 +65,7 @@ struct WalkArgs {
   double* tally64;         // FP64 walks
   float* tally32;          // FP32 walks
   unsigned long long* steps;  // total transitions (atomic)
+  unsigned long long* queue;  // kQueueWords counters, zeroed by launch_walk
   int* paths;              // recording: (last-first) x (max_len+1)
   double* wfinal;          // recording: final weight per history
   int max_len;
@@ -56,12 +75,20 @@ struct WalkConfig {
   int precision;   // 64 or 32
   int estimator;   // Estimator
   int source;      // SourceMode
+  int sampler;     // Sampler
   bool record;
   int block;       // threads per block
-  int grid;        // blocks
+  int grid;        // blocks (rounded up to a multiple of kQueueSlots)
 };
 
+// Zeroes the queue counters, then launches the walk.
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s);
+
+// Row headers, and (alias != 0) Vose alias tables packed into slot records.
+// Rows longer than 64 entries set FLAG_LONG_ROW when alias tables are built.
+hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64,
+                            const double* W64, const float* W32, uint4* hdr32, uint4* hdr64,
+                            uint4* slot32, uint4* slot64, bool alias, int* flags, hipStream_t s);
 
 // ---------------------------------------------------------------- source
 // bits of max |rhs| (as uint64) -> *bmax (must be zeroed first)

@@ -43,7 +43,7 @@ def _setup(ctx, prob, f32, jacobi=True, params=None):
     base = {"Neumann Relaxation": 1.0, "Composite Operator Threshold": 0.0,
             "Walk Precision": "FP64", "Source Sampling Type": "Random",
             "Estimator Type": "Collision", "Weight Cutoff": 0.0, "History Length": 10,
-            "Number of Histories": 0, "Random Seed": 0}
+            "Number of Histories": 0, "Random Seed": 0, "Transition Sampler": "CDF"}
     base.update(params or {})
     ctx.set_params(base)
 
@@ -83,15 +83,17 @@ def test_domain_no_prec_and_random_matrix(ctx):
         assert np.array_equal(h, dom.h) and np.array_equal(cdf, dom.cdf) and np.array_equal(W, dom.W)
 
 
+@pytest.mark.parametrize("sampler", ["CDF", "Alias"])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
 @pytest.mark.parametrize("kind", ["poisson", "s27"])
-def test_paths_bitexact(ctx, prec, source, kind):
+def test_paths_bitexact(ctx, prec, source, kind, sampler):
     prob, f32 = _problem(kind)
     L, cutoff, nh = 300, 1e-3, 5000
     _setup(ctx, prob, f32, params={"Walk Precision": f"FP{prec}", "Source Sampling Type": source,
                                    "History Length": L, "Weight Cutoff": cutoff,
-                                   "Number of Histories": nh, "Random Seed": 1234})
+                                   "Number of Histories": nh, "Random Seed": 1234,
+                                   "Transition Sampler": sampler})
     A = _csr(prob, f32)
     dinv = po.jacobi_dinv(A)
     dom = po.build_domain(A, dinv)
@@ -100,23 +102,25 @@ def test_paths_bitexact(ctx, prec, source, kind):
     first, count = 100, 1500
     gp, gw = ctx.record_paths(rhs, 17, first, count, L)
     mp = po.mc_params(nh, L, cutoff, prec, po.EST_COLLISION,
-                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=1234)
+                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=1234,
+                      sampler=po.SAMPLER_ALIAS if sampler == "Alias" else po.SAMPLER_CDF)
     op, ow = po.record_paths(dom, rhs, mp, first, count, L, solve_index=17)
     assert np.array_equal(gp, op)
     assert np.array_equal(gw, ow)
     assert (gp[:, 1] >= 0).mean() > 0.9  # walks actually move
 
 
+@pytest.mark.parametrize("sampler", ["CDF", "Alias"])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("est", ["Collision", "Expected Value"])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
-def test_mc_tally_matches_oracle(ctx, prec, est, source):
+def test_mc_tally_matches_oracle(ctx, prec, est, source, sampler):
     prob, f32 = _problem("diff3d")
     nh, L, cutoff = 20000, 50, 1e-2
     _setup(ctx, prob, f32, params={"Walk Precision": f"FP{prec}", "Estimator Type": est,
                                    "Source Sampling Type": source, "History Length": L,
                                    "Weight Cutoff": cutoff, "Number of Histories": nh,
-                                   "Random Seed": 99})
+                                   "Random Seed": 99, "Transition Sampler": sampler})
     A = _csr(prob, f32)
     dinv = po.jacobi_dinv(A)
     dom = po.build_domain(A, dinv)
@@ -125,7 +129,8 @@ def test_mc_tally_matches_oracle(ctx, prec, est, source):
     st = ctx.stats()
     mp = po.mc_params(nh, L, cutoff, prec,
                       po.EST_EXPECTED_VALUE if est == "Expected Value" else po.EST_COLLISION,
-                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=99)
+                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=99,
+                      sampler=po.SAMPLER_ALIAS if sampler == "Alias" else po.SAMPLER_CDF)
     xo, steps = po.mc_solve(dom, rhs, mp, solve_index=5)
     assert st.steps == steps
     tol = 1e-10 if prec == 64 else 2e-5
@@ -186,3 +191,19 @@ def test_errors(ctx):
     with pytest.raises(capi.GpuError) as e:
         ctx.set_matrix(rp, np.array([0, 1], np.int32), np.array([np.nan, 1.0]))
     assert e.value.code == capi.MCLS_ERR_MATRIX
+
+
+def test_alias_tables_bitexact(ctx):
+    # the device alias builder follows the oracle's Vose order exactly: the
+    # sampled paths agree only if thresholds and aliases are bit-identical
+    prob, f32 = _problem("aniso")
+    _setup(ctx, prob, f32, params={"Transition Sampler": "Alias", "History Length": 40,
+                                   "Number of Histories": 3000, "Random Seed": 5})
+    A = _csr(prob, f32)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    gp, gw = ctx.record_paths(rhs, 0, 0, 3000, 40)
+    op, ow = po.record_paths(dom, rhs, po.mc_params(3000, 40, 0.0, 64, seed=5, sampler=po.SAMPLER_ALIAS),
+                             0, 3000, 40)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
