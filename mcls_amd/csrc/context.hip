@@ -102,6 +102,7 @@ struct mcls_gpu_context {
   // matrix
   int n = 0;
   int64_t nnz = 0;
+  int res_lanes = 1;  // lanes per row of the residual kernel
   bool f32 = false;
   bool has_matrix = false;
   DBuf<int> a_rp, a_col;
@@ -196,7 +197,7 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->b.ensure(n, "b"));
   MCLS_TRY(c->r.ensure(n, "r"));
   MCLS_TRY(c->rhs.ensure(n, "rhs"));
-  MCLS_TRY(c->part.ensure((size_t)mcls::residual_blocks(c->n) + 1, "partials"));
+  MCLS_TRY(c->part.ensure((size_t)mcls::kResidualPartials + 1, "partials"));
   MCLS_TRY(c->scal.ensure(8, "scalars"));
   MCLS_TRY(c->q.ensure(n, "q"));
   MCLS_TRY(c->qcdf.ensure(n, "qcdf"));
@@ -449,8 +450,8 @@ int histories_done(mcls_gpu_context* c, int64_t* out) {
 // r = PL (b - A x), norm^2 into scal[slot] and host_scal[slot]
 int residual(mcls_gpu_context* c, const double* b, const double* x, double* r, int slot) {
   hipStream_t s = c->stream;
-  const int nb = mcls::residual_blocks(c->n);
-  HIP_TRY(mcls::launch_prec_residual(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
+  const int nb = mcls::residual_blocks(c->n, c->res_lanes);
+  HIP_TRY(mcls::launch_prec_residual(c->n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
                                      c->has_prec ? c->dinv.p : nullptr, b, x, r, c->part.p, s));
   HIP_TRY(mcls::launch_sum_partials(c->part.p, nb, c->scal.p + slot, s));
   HIP_TRY(hipMemcpyAsync(c->host_scal + slot, c->scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -567,6 +568,7 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   c->nnz = nnz;
   c->f32 = val_dtype == MCLS_F32;
   c->has_matrix = true;
+  c->res_lanes = mcls::residual_lanes(n, nnz);
   if (c->has_prec && (int64_t)c->dinv.n < n) c->has_prec = false;
   MCLS_TRY(ensure_vectors(c));
   return MCLS_OK;
@@ -895,7 +897,7 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
 
   // src/MCLS_MCSASolverManager_impl.hpp:300-318: ||PL b||_2, homogeneous case
   HIP_TRY(mcls::launch_scaled_sq(n, c->has_prec ? c->dinv.p : nullptr, c->b.p, c->part.p, s));
-  HIP_TRY(mcls::launch_sum_partials(c->part.p, mcls::residual_blocks(n), c->scal.p + 2, s));
+  HIP_TRY(mcls::launch_sum_partials(c->part.p, mcls::residual_blocks(n, 1), c->scal.p + 2, s));
   HIP_TRY(hipMemcpyAsync(c->host_scal + 2, c->scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
   MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
   HIP_TRY(hipStreamSynchronize(s));
@@ -917,10 +919,8 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
     for (int64_t l = 0; l < p.smoother_steps; ++l) {
       // RichardsonIteration::doOneIteration: x += omega rp; rp = PL(b - A x)
       HIP_TRY(mcls::launch_axpy(n, p.richardson_omega, c->r.p, c->x.p, s));
-      const int nb = mcls::residual_blocks(n);
-      HIP_TRY(mcls::launch_prec_residual(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
+      HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
                                          c->has_prec ? c->dinv.p : nullptr, c->b.p, c->x.p, c->r.p, c->part.p, s));
-      (void)nb;
     }
     // buildMonteCarloSource with internal solver: rhs = omega * rp
     HIP_TRY(mcls::launch_scale_copy(n, p.neumann_omega, c->r.p, c->rhs.p, s));

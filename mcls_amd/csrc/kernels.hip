@@ -669,7 +669,13 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
 }
 
 // ---------------------------------------------------------------- linalg
-template <typename AV>
+// r = PL (b - A x) with V lanes per row ("vector CSR"): the lanes of a row
+// read its entries with unit stride, so a wave's loads are coalesced; the
+// row sum is reduced by a fixed xor butterfly (deterministic), accumulated
+// in FP64 for FP32 or FP64 values.  V = 1 is one row per lane with the
+// entries summed in stored order.  Grid-stride over rows with a fixed grid,
+// so the per-block partial sums of r^2 are reproducible.
+template <typename AV, int V>
 __global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int* __restrict__ rp,
                                                                const int* __restrict__ col, const void* val,
                                                                const double* __restrict__ dinv,
@@ -678,15 +684,21 @@ __global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int*
                                                                double* __restrict__ r, double* part) {
   __shared__ double sm[kRedThreads];
   const AV* __restrict__ v = (const AV*)val;
+  constexpr int RPB = kRedThreads / V;
+  const int g = threadIdx.x / V, j = threadIdx.x % V;
   double sq = 0.0;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  for (int i = blockIdx.x * RPB + g; i < n; i += gridDim.x * RPB) {
     double s = 0.0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) s = s + (double)v[k] * x[col[k]];
-    double t = b[i] - s;
-    double ri = dinv ? dinv[i] * t : t;
-    r[i] = ri;
-    sq = ri * ri;
+    const int end = rp[i + 1];
+    for (int k = rp[i] + j; k < end; k += V) s = s + (double)v[k] * x[col[k]];
+    if (V > 1)
+      for (int off = V / 2; off > 0; off >>= 1) s = s + __shfl_xor(s, off, V);
+    if (j == 0) {
+      const double t = b[i] - s;
+      const double ri = dinv ? dinv[i] * t : t;
+      r[i] = ri;
+      sq = sq + ri * ri;
+    }
   }
   sm[threadIdx.x] = sq;
   __syncthreads();
@@ -701,10 +713,9 @@ __global__ __launch_bounds__(kRedThreads) void k_scaled_sq(int n, const double* 
                                                            const double* __restrict__ b, double* part) {
   __shared__ double sm[kRedThreads];
   double sq = 0.0;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    double t = dinv ? dinv[i] * b[i] : b[i];
-    sq = t * t;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double t = dinv ? dinv[i] * b[i] : b[i];
+    sq = sq + t * t;
   }
   sm[threadIdx.x] = sq;
   __syncthreads();
@@ -882,17 +893,41 @@ hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const do
   return hipGetLastError();
 }
 
-int residual_blocks(int n) { return (n + kRedThreads - 1) / kRedThreads; }
+int residual_lanes(int64_t n, int64_t nnz) {
+  const double z = n > 0 ? (double)nnz / (double)n : 0.0;
+  if (z < 3.0) return 1;
+  if (z <= 6.0) return 4;
+  if (z <= 12.0) return 8;
+  if (z <= 24.0) return 16;
+  return 32;
+}
 
-hipError_t launch_prec_residual(int n, const int* rp, const int* col, const void* val, bool f32,
+int residual_blocks(int n, int lanes) {
+  int64_t g = ((int64_t)n * lanes + kRedThreads - 1) / kRedThreads;
+  if (g < 1) g = 1;
+  return (int)(g < kResidualPartials ? g : kResidualPartials);
+}
+
+template <typename AV>
+static hipError_t prec_residual_t(int n, int lanes, const int* rp, const int* col, const void* val,
+                                  const double* dinv, const double* b, const double* x, double* r, double* part,
+                                  hipStream_t s) {
+  const int g = residual_blocks(n, lanes);
+  switch (lanes) {
+    case 1: k_prec_residual<AV, 1><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part); break;
+    case 4: k_prec_residual<AV, 4><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part); break;
+    case 8: k_prec_residual<AV, 8><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part); break;
+    case 16: k_prec_residual<AV, 16><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part); break;
+    default: k_prec_residual<AV, 32><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col, const void* val, bool f32,
                                 const double* dinv, const double* b, const double* x, double* r, double* part,
                                 hipStream_t s) {
-  int g = residual_blocks(n);
-  if (f32)
-    k_prec_residual<float><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part);
-  else
-    k_prec_residual<double><<<g, kRedThreads, 0, s>>>(n, rp, col, val, dinv, b, x, r, part);
-  return hipGetLastError();
+  return f32 ? prec_residual_t<float>(n, lanes, rp, col, val, dinv, b, x, r, part, s)
+             : prec_residual_t<double>(n, lanes, rp, col, val, dinv, b, x, r, part, s);
 }
 
 hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s) {
@@ -901,7 +936,7 @@ hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream
 }
 
 hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part, hipStream_t s) {
-  k_scaled_sq<<<residual_blocks(n), kRedThreads, 0, s>>>(n, dinv, b, part);
+  k_scaled_sq<<<residual_blocks(n, 1), kRedThreads, 0, s>>>(n, dinv, b, part);
   return hipGetLastError();
 }
 

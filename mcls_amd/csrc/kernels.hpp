@@ -132,10 +132,14 @@ hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const do
                               float* cdf32, double* W64, float* W32, hipStream_t s);
 
 // ---------------------------------------------------------------- linalg
-// r = PL (b - A x); partial sums of r^2 per block into part (nblk entries).
-int residual_blocks(int n);
-hipError_t launch_prec_residual(int n, const int* rp, const int* col, const void* val, bool f32,
-                                const double* dinv, const double* b, const double* x,
+// r = PL (b - A x); partial sums of r^2 per block into part
+// (residual_blocks(n, lanes) <= kResidualPartials entries).  lanes per row
+// from residual_lanes(n, nnz).
+constexpr int kResidualPartials = 4096;
+int residual_lanes(int64_t n, int64_t nnz);
+int residual_blocks(int n, int lanes);
+hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col, const void* val,
+                                bool f32, const double* dinv, const double* b, const double* x,
                                 double* r, double* part, hipStream_t s);
 // out[0] = sum of part[0..m) in a fixed order; out_host (pinned, nullable)
 hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s);

@@ -169,7 +169,9 @@ def test_residual_matches_oracle(ctx):
     r, nrm = ctx.prec_residual(prob.b, x)
     A = _csr(prob, f32)
     ro = po.prec_residual(A, prob.b, x, po.jacobi_dinv(A))
-    assert np.array_equal(r, ro)
+    # the device sums each row with a lane butterfly: same terms, other order
+    scale = po.spmv(po.Csr(A.rp, A.col, np.abs(A.val)), np.abs(x)) * np.abs(po.jacobi_dinv(A))
+    assert np.all(np.abs(r - ro) <= 64 * np.finfo(float).eps * (scale + np.abs(ro)))
     assert abs(nrm - np.linalg.norm(ro)) <= 1e-14 * np.linalg.norm(ro)
 
 

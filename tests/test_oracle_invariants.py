@@ -1,0 +1,130 @@
+"""The oracle's estimators pinned by what the mathematics fixes (no GPU).
+
+On a small nonsymmetric system the adjoint estimators must reproduce
+  collision:       E[x] = sum_{k=0}^{L-1} H^k b
+  expected value:  E[x] = sum_{k=0}^{L}   H^k b
+with H = I - omega D^-1 A (the walk samples rows of H^T), for every source
+mode, sampler and precision.  A transposed operand, a tally after (instead
+of before) the transition, or a dropped sign each move the mean by many
+standard errors.  Also: alias tables reproduce the row probabilities, the
+reference-semantics solve is unbiased, MCSA converges to the direct solution.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+def _system(n=12, seed=3):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = 4.0 + rng.random()
+        for j in rng.choice([k for k in range(n) if k != i], 3, replace=False):
+            A[i, j] = -rng.random() * (1.0 if rng.random() < 0.7 else -1.0)
+    b = rng.random(n) - 0.3
+    return A, b
+
+
+def _replicas(dom, b, mp, reps):
+    xs = []
+    for r in range(reps):
+        x, _ = po.mc_solve(dom, b, mp, solve_index=1000 + r)
+        xs.append(x)
+    xs = np.array(xs)
+    return xs.mean(0), xs.std(0, ddof=1) / np.sqrt(reps)
+
+
+def _check_z(mean, se, exact):
+    z = (mean - exact) / np.maximum(se, 1e-300)
+    assert np.all(np.abs(z) < 5.0), z
+    assert 0.2 < np.mean(z * z) < 3.0, np.mean(z * z)
+
+
+@pytest.mark.parametrize("est", [po.EST_COLLISION, po.EST_EXPECTED_VALUE])
+@pytest.mark.parametrize("source", [po.SRC_RANDOM, po.SRC_STRATIFIED])
+@pytest.mark.parametrize("sampler", [po.SAMPLER_CDF, po.SAMPLER_ALIAS])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_estimator_expectation(est, source, sampler, prec):
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    omega = 0.9
+    dom = po.build_domain(Ac, dinv, omega)
+    H = np.eye(n) - omega * dinv[:, None] * A
+    L = 6
+    terms = L + 1 if est == po.EST_EXPECTED_VALUE else L
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(terms))
+    mp = po.mc_params(3000, L, 0.0, prec, est, source, seed=prec + 7 * est + 3 * source + sampler,
+                      sampler=sampler)
+    mean, se = _replicas(dom, b, mp, 60)
+    _check_z(mean, se, exact)
+    # a plausible mistake: walking H instead of H^T moves the mean away
+    wrong = sum(np.linalg.matrix_power(H.T, k) @ b for k in range(terms))
+    assert np.max(np.abs(mean - wrong) / np.maximum(se, 1e-300)) > 8
+
+
+def test_reference_semantics_unbiased():
+    A, b = _system(10, seed=8)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv)
+    H = np.eye(10) - dinv[:, None] * A
+    L = 5
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(L))
+    xs = []
+    for r in range(80):
+        x, nh, _ = po.ref_mc_solve(dom, b, po.mc_params(2000, L), rng_state=r + 1)
+        xs.append(x)
+    xs = np.array(xs)
+    _check_z(xs.mean(0), xs.std(0, ddof=1) / np.sqrt(len(xs)), exact)
+
+
+def test_alias_tables_reproduce_row_probabilities():
+    A, b = _system(30, seed=1)
+    Ac = po.Csr.from_dense(A)
+    dom = po.build_domain(Ac, po.jacobi_dinv(Ac))
+    thr, alias = po.build_alias(dom)
+    for i in range(dom.n):
+        s, e = dom.rp[i], dom.rp[i + 1]
+        z = e - s
+        if z == 0:  # absorbing row (column of A with only its diagonal)
+            continue
+        p = np.abs(dom.h[s:e]) / dom.W[i]
+        q = np.zeros(z)
+        for k in range(z):
+            q[k] += thr[s + k] / z
+            q[alias[s + k]] += (1.0 - thr[s + k]) / z
+        assert np.max(np.abs(q - p)) < 1e-14
+
+
+def test_source_quantisation_is_exact_and_proportional():
+    rng = np.random.default_rng(2)
+    b = rng.normal(size=1000) * np.exp(rng.normal(size=1000) * 3)
+    b[::50] = 0.0
+    Q, q, cdf = po.source_quantize(b)
+    assert Q == int(cdf[-1]) and np.all(np.diff(cdf.astype(object)) >= 0)
+    nz = b != 0
+    assert np.all(q[~nz] == 0) and np.all(q[nz] >= 1)
+    # q_i = ceil(|b_i| 2^(S-e)): relative error below 2^(S-e)/|b_i|-scale
+    ratio = q[nz].astype(float) / np.abs(b[nz])
+    big = np.abs(b[nz]) > np.abs(b).max() * 1e-6
+    assert np.allclose(ratio[big], ratio[big][0], rtol=1e-9)
+
+
+def test_mcsa_oracle_converges_to_direct_solution():
+    A, b = _system(40, seed=5)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    mp = po.mc_params(4000, 50, 1e-3, 64, seed=4)
+    x, res, hist = po.mcsa_solve(Ac, b, dinv=dinv, params=po.solver_params(tol=1e-10, max_iters=60, mc=mp))
+    assert res.converged == 1 and res.achieved_tol <= 1e-10
+    xd = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    # reference semantics, 4 sets in threads
+    sp = po.solver_params(tol=1e-10, max_iters=60, num_sets=4, reference_mode=1,
+                          mc=po.mc_params(1000, 50, 1e-3, 64, seed=9))
+    x2, res2, _ = po.mcsa_solve(Ac, b, dinv=dinv, params=sp)
+    assert res2.converged == 1
+    assert np.linalg.norm(x2 - xd) <= 1e-8 * np.linalg.norm(xd)
