@@ -155,6 +155,22 @@ int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            int ptr_kind);
 int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
 
+/* ------------------------------------------------------- batched systems -- */
+/* Many independent systems in one launch (no counterpart in the reference,
+ * which solves one system per manager; its multi-RHS Thyra adapter loops
+ * over columns, thyra/src/MCLS_SolverManagerAdapter_impl.hpp:83-154).  The
+ * matrix set with mcls_gpu_set_matrix_csr is block diagonal with system s
+ * on rows [offsets[s], offsets[s+1]) (nsys+1 host int64 offsets, 0 and n at
+ * the ends); nsys = 0 returns to one system.  mcls_gpu_mcsa_solve then runs
+ * every system's MCSA iteration in the same kernels: histories are
+ * allocated per system in proportion to its unknown count, each system is
+ * tested against its own ||PL b_s||_2, a converged system gets no further
+ * MC correction, and the solve ends when all have converged (stats:
+ * achieved_tol = worst system). */
+int mcls_gpu_set_systems(mcls_gpu_context* ctx, int64_t nsys, const int64_t* offsets);
+/* Relative residual of each system after the last batched solve (nsys). */
+int mcls_gpu_get_system_residuals(mcls_gpu_context* ctx, double* rel);
+
 /* ------------------------------------------------------------- parity -- */
 /* Run the walk kernel for histories [first, first+count) of MC solve number
  * `solve_index` with source `rhs` (already preconditioned and scaled) and

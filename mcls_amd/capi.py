@@ -35,7 +35,7 @@ GPU_SYMBOLS = [
     "mcls_gpu_set_param_str", "mcls_gpu_build_domain", "mcls_gpu_domain_nnz",
     "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
     "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
-    "mcls_gpu_mc_solve_raw",
+    "mcls_gpu_mc_solve_raw", "mcls_gpu_set_systems", "mcls_gpu_get_system_residuals",
 ]
 PROBLEM_SYMBOLS = ["mcls_problem_dims", "mcls_problem_build", "mcls_problem_batch_member",
                    "mcls_problem_uniform"]
@@ -119,6 +119,8 @@ def gpu_lib():
     L.mcls_gpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.mcls_gpu_record_paths.argtypes = [vp, vp, u64, i64, i64, C.c_int32, vp, vp]
     L.mcls_gpu_mc_solve_raw.argtypes = [vp, vp, u64, vp]
+    L.mcls_gpu_set_systems.argtypes = [vp, i64, vp]
+    L.mcls_gpu_get_system_residuals.argtypes = [vp, vp]
     for name in GPU_SYMBOLS:
         f = getattr(L, name)
         if name != "mcls_gpu_last_error":
@@ -208,6 +210,7 @@ class Context:
         self.h = C.c_void_p()
         self._check(self.L.mcls_gpu_create(C.byref(self.h), device, 0))
         self.n = 0
+        self.nsys = 0
 
     def _check(self, rc):
         if rc != MCLS_OK:
@@ -323,6 +326,21 @@ class Context:
         self._check(self.L.mcls_gpu_record_paths(self.h, _p(rhs), int(solve_index), int(first),
                                                  int(count), int(max_len), _p(paths), _p(w)))
         return paths, w
+
+    def set_systems(self, offsets):
+        """Block-diagonal batch: system s on rows [offsets[s], offsets[s+1])."""
+        if offsets is None or len(offsets) == 0:
+            self._check(self.L.mcls_gpu_set_systems(self.h, 0, None))
+            self.nsys = 0
+            return
+        off = np.ascontiguousarray(offsets, np.int64)
+        self._check(self.L.mcls_gpu_set_systems(self.h, len(off) - 1, _p(off)))
+        self.nsys = len(off) - 1
+
+    def system_residuals(self):
+        out = np.empty(self.nsys)
+        self._check(self.L.mcls_gpu_get_system_residuals(self.h, _p(out)))
+        return out
 
     def stats(self):
         s = Stats()

@@ -128,7 +128,7 @@ struct mcls_gpu_context {
   DBuf<uint64_t> q, qcdf, ncnt, ncdf, scan_tmp;
   DBuf<unsigned long long> bmax, steps, queue;
   DBuf<int> flags;
-  double* host_scal = nullptr;  // pinned, 8 doubles
+  double* host_scal = nullptr;  // pinned, 16 doubles (see kHostM / kHostUnconv)
 
   uint64_t solve_index = 0;
   mcls_gpu_stats stats{};
@@ -136,6 +136,16 @@ struct mcls_gpu_context {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint64_t graph_sig = 0;
+  DBuf<uint64_t> key_dev;               // stream key of the current MC solve
+  DBuf<unsigned long long> sidx_dev;    // device copy of solve_index (graph mode)
+
+  // batched systems: block-diagonal operator with rows [sys_off[s], sys_off[s+1])
+  int nsys = 0;
+  DBuf<int64_t> sys_off;
+  DBuf<int> sys_of, sys_conv, sys_unconv;
+  DBuf<double> sys_b2, sys_r2, sys_l1, mag;
+  std::vector<int64_t> sys_off_h;
+  std::vector<double> sys_rel;  // per-system relative residuals of the last solve
 };
 
 namespace {
@@ -207,6 +217,9 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->bmax.ensure(1, "bmax"));
   MCLS_TRY(c->steps.ensure(1, "steps"));
   MCLS_TRY(c->queue.ensure(mcls::kQueueWords, "queue"));
+  MCLS_TRY(c->key_dev.ensure(1, "key"));
+  MCLS_TRY(c->sidx_dev.ensure(1, "solve index"));
+  MCLS_TRY(c->mag.ensure(n, "source magnitudes"));
   if (c->prm.precision == 32) {
     if (!c->t32.p || c->t32.n < n) {
       MCLS_TRY(c->t32.ensure(n, "tally32"));
@@ -382,13 +395,15 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   return a;
 }
 
-// Source tables for rhs (device): q, qcdf (and stratified counts).
-int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh) {
+// Source tables for rhs (device): q, qcdf (and stratified counts).  Births
+// are allocated by |mag| (mag = rhs unless a batch reallocates them).
+int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const double* mag = nullptr) {
   hipStream_t s = c->stream;
   const int n = c->n;
+  if (!mag) mag = rhs;
   HIP_TRY(mcls::launch_fill_u64(c->bmax.p, 1, 0ULL, s));
-  HIP_TRY(mcls::launch_absmax(rhs, n, c->bmax.p, s));
-  HIP_TRY(mcls::launch_quantize(rhs, n, c->bmax.p, c->q.p, s));
+  HIP_TRY(mcls::launch_absmax(mag, n, c->bmax.p, s));
+  HIP_TRY(mcls::launch_quantize(mag, n, c->bmax.p, c->q.p, s));
   HIP_TRY(mcls::launch_scan_u64(c->q.p, c->qcdf.p, n, c->scan_tmp.p, s));
   if (c->prm.source == mcls::SRC_STRATIFIED) {
     HIP_TRY(mcls::launch_strat_counts(c->q.p, n, c->qcdf.p + (n - 1), (uint64_t)nh, c->ncnt.p, s));
@@ -398,19 +413,23 @@ int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh) {
 }
 
 // One MC solve of source rhs (device) into the tally; then x (+)= tally/nh.
-// accumulate: x += ..., else x = ...
-int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx) {
+// accumulate: x += ..., else x = ...  `timed` brackets the source build and
+// the walk with events (not inside a graph capture); `key_dev` (nullable)
+// makes the walk read its stream key from device memory.
+int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx,
+                    bool timed = true, const uint64_t* key_dev = nullptr) {
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
-  HIP_TRY(hipEventRecord(c->ev[2], s));
-  MCLS_TRY(build_source(c, rhs, nh));
-  HIP_TRY(hipEventRecord(c->ev[3], s));
+  if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
+  MCLS_TRY(build_source(c, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
+  if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
   const bool strat = c->prm.source == mcls::SRC_STRATIFIED;
   const int64_t last = strat ? nh + c->n : nh;
   WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), 0, last);
+  a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, last);
   if (nh > 0) HIP_TRY(mcls::launch_walk(a, wc, s));
-  HIP_TRY(hipEventRecord(c->ev[4], s));
+  if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
   const bool ev = c->prm.estimator == mcls::EST_EXPECTED_VALUE;
   double inv = nh > 0 ? 1.0 / (double)nh : 0.0;
   const uint64_t* mtot = strat ? c->ncdf.p + (c->n - 1) : nullptr;
@@ -465,6 +484,51 @@ int set_param_common(mcls_gpu_context* c) {
 
 bool key_is(const char* a, const char* b) { return std::strcmp(a, b) == 0; }
 
+// Host-visible scalars in the pinned block host_scal (16 doubles):
+//   [0] ||rp||^2   [2] ||PL b||^2   [8] M (stratified histories, bits)
+//   [9] unconverged systems (int in the low word)
+constexpr int kHostM = 8, kHostUnconv = 9;
+
+// One MCSA iteration (src/MCLS_MCSASolverManager_impl.hpp:356-406): smoother
+// steps, MC correction of the preconditioned residual, update, residual and
+// its norm.  Everything stays on the stream; with timed = false it contains
+// no event records and reads its stream key from key_dev, so it can be
+// captured once into a hipGraph and replayed.
+int mcsa_iteration(mcls_gpu_context* c, bool timed, const uint64_t* key_dev, uint64_t sidx) {
+  hipStream_t s = c->stream;
+  const Params& p = c->prm;
+  const int n = c->n;
+  const double* dinv = c->has_prec ? c->dinv.p : nullptr;
+  if (timed) HIP_TRY(hipEventRecord(c->ev[5], s));
+  for (int64_t l = 0; l < p.smoother_steps; ++l) {
+    // RichardsonIteration::doOneIteration: x += omega rp; rp = PL(b - A x)
+    HIP_TRY(mcls::launch_axpy(n, p.richardson_omega, c->r.p, c->x.p, s));
+    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, c->b.p,
+                                       c->x.p, c->r.p, c->part.p, s));
+  }
+  // buildMonteCarloSource with the internal solver: rhs = omega rp
+  if (c->nsys > 0) {
+    HIP_TRY(mcls::launch_segment_norm(c->nsys, c->sys_off.p, c->r.p, nullptr, false, c->sys_l1.p, s));
+    HIP_TRY(mcls::launch_batch_source(n, c->sys_of.p, c->sys_off.p, c->r.p, p.neumann_omega, c->sys_conv.p,
+                                      c->sys_l1.p, c->rhs.p, c->mag.p, s));
+  } else {
+    HIP_TRY(mcls::launch_scale_copy(n, p.neumann_omega, c->r.p, c->rhs.p, s));
+  }
+  if (timed) HIP_TRY(hipEventRecord(c->ev[6], s));
+  if (key_dev) HIP_TRY(mcls::launch_next_key(p.seed, c->sidx_dev.p, c->key_dev.p, s));
+  MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, true, sidx, timed, key_dev));  // x += delta x
+  MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
+  if (c->nsys > 0) {
+    HIP_TRY(mcls::launch_segment_norm(c->nsys, c->sys_off.p, c->r.p, nullptr, true, c->sys_r2.p, s));
+    HIP_TRY(mcls::launch_batch_check(c->nsys, c->sys_r2.p, c->sys_b2.p, p.tol, c->sys_conv.p, c->sys_unconv.p, s));
+    HIP_TRY(hipMemcpyAsync(c->host_scal + kHostUnconv, c->sys_unconv.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  }
+  if (p.source == mcls::SRC_STRATIFIED)
+    HIP_TRY(hipMemcpyAsync(c->host_scal + kHostM, c->ncdf.p + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  if (timed) HIP_TRY(hipEventRecord(c->ev[7], s));
+  return MCLS_OK;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -496,7 +560,7 @@ int mcls_gpu_create(mcls_gpu_context** out, int device, unsigned flags) {
       return fail(MCLS_ERR_HIP, "hipEventCreate failed");
     }
   }
-  if (hipHostMalloc((void**)&c->host_scal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&c->host_scal, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     c->host_scal = nullptr;
     mcls_gpu_destroy(c);
     return fail(MCLS_ERR_HIP, "hipHostMalloc failed");
@@ -569,6 +633,7 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   c->f32 = val_dtype == MCLS_F32;
   c->has_matrix = true;
   c->res_lanes = mcls::residual_lanes(n, nnz);
+  c->nsys = 0;  // system offsets describe the previous operator
   if (c->has_prec && (int64_t)c->dinv.n < n) c->has_prec = false;
   MCLS_TRY(ensure_vectors(c));
   return MCLS_OK;
@@ -913,31 +978,62 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   int64_t iters = 0;
   bool go = true;
   const int64_t nh = num_histories(c);
+  const bool batch = c->nsys > 0;
+  if (batch) {
+    // per-system ||PL b_s||^2 and convergence flags
+    HIP_TRY(mcls::launch_segment_norm(c->nsys, c->sys_off.p, c->b.p, c->has_prec ? c->dinv.p : nullptr, true,
+                                      c->sys_b2.p, s));
+    HIP_TRY(mcls::launch_fill_i32(c->sys_conv.p, c->nsys, 0, s));
+  }
+  // hipGraph of one iteration (build option "Use Graph"): captured once per
+  // problem and parameter set, replayed every iteration.
+  const bool graph = p.use_graph != 0;
+  if (graph) {
+    unsigned long long si = c->solve_index;
+    HIP_TRY(hipMemcpyAsync(c->sidx_dev.p, &si, sizeof(si), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (!c->graph_exec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = mcsa_iteration(c, false, c->key_dev.p, 0);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc != MCLS_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIP_TRY(e);
+      c->graph = g;
+      HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+      st.walk_launches = 0;  // the capture itself launched nothing
+    }
+  }
   while (go) {
     ++iters;
-    HIP_TRY(hipEventRecord(c->ev[5], s));
-    for (int64_t l = 0; l < p.smoother_steps; ++l) {
-      // RichardsonIteration::doOneIteration: x += omega rp; rp = PL(b - A x)
-      HIP_TRY(mcls::launch_axpy(n, p.richardson_omega, c->r.p, c->x.p, s));
-      HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
-                                         c->has_prec ? c->dinv.p : nullptr, c->b.p, c->x.p, c->r.p, c->part.p, s));
-    }
-    // buildMonteCarloSource with internal solver: rhs = omega * rp
-    HIP_TRY(mcls::launch_scale_copy(n, p.neumann_omega, c->r.p, c->rhs.p, s));
-    HIP_TRY(hipEventRecord(c->ev[6], s));
     const uint64_t sidx = c->solve_index++;
-    MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, true, sidx));  // x += delta x
-    MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
-    HIP_TRY(hipEventRecord(c->ev[7], s));
-    HIP_TRY(hipStreamSynchronize(s));
-    MCLS_TRY(accumulate_mc_timing(c));
-    float a = 0.f, b2 = 0.f;
-    HIP_TRY(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
-    HIP_TRY(hipEventElapsedTime(&b2, c->ev[4], c->ev[7]));
-    st.ms_residual += a + b2;
-    int64_t hd = 0;
-    if (p.source == mcls::SRC_STRATIFIED) MCLS_TRY(histories_done(c, &hd));
-    else hd = nh;
+    if (graph) {
+      HIP_TRY(hipEventRecord(c->ev[5], s));
+      HIP_TRY(hipGraphLaunch(c->graph_exec, s));
+      HIP_TRY(hipEventRecord(c->ev[7], s));
+      HIP_TRY(hipStreamSynchronize(s));
+      float a = 0.f;
+      HIP_TRY(hipEventElapsedTime(&a, c->ev[5], c->ev[7]));
+      st.ms_walk += a;  // whole iteration (phases are not separated in a graph)
+      st.walk_launches += nh > 0 ? 1 : 0;
+    } else {
+      MCLS_TRY(mcsa_iteration(c, true, nullptr, sidx));
+      HIP_TRY(hipStreamSynchronize(s));
+      MCLS_TRY(accumulate_mc_timing(c));
+      float a = 0.f, b2 = 0.f;
+      HIP_TRY(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
+      HIP_TRY(hipEventElapsedTime(&b2, c->ev[4], c->ev[7]));
+      st.ms_residual += a + b2;
+    }
+    int64_t hd = nh;
+    if (p.source == mcls::SRC_STRATIFIED) {
+      uint64_t M = 0;
+      std::memcpy(&M, c->host_scal + kHostM, sizeof(M));
+      hd = (int64_t)M;
+    }
     st.histories += hd;
     rnorm = std::sqrt(c->host_scal[0]);
     if (!std::isfinite(rnorm)) {
@@ -945,7 +1041,15 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
       st.converged = 0;
       return fail(MCLS_ERR_NUMERIC, "non-finite residual at MCSA iteration " + std::to_string(iters));
     }
-    if (iters % p.check_freq == 0) go = (rnorm > crit) && (iters < p.max_iters);
+    if (iters % p.check_freq == 0) {
+      if (batch) {
+        int unconv = 0;
+        std::memcpy(&unconv, c->host_scal + kHostUnconv, sizeof(int));
+        go = unconv > 0 && iters < p.max_iters;
+      } else {
+        go = (rnorm > crit) && (iters < p.max_iters);
+      }
+    }
     if (p.verbosity > 0 && ((p.print_freq > 0 && iters % p.print_freq == 0) || !go))
       std::printf("%18lld%18.4e\n", (long long)iters, rnorm / source_norm);
   }
@@ -958,8 +1062,62 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   st.converged = rnorm <= crit;
   st.achieved_tol = rnorm / source_norm;
   st.source_norm = source_norm;
+  if (batch) {
+    // per-system relative residuals; the batch converges when all do
+    std::vector<double> r2((size_t)c->nsys), b2((size_t)c->nsys);
+    HIP_TRY(hipMemcpy(r2.data(), c->sys_r2.p, sizeof(double) * r2.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b2.data(), c->sys_b2.p, sizeof(double) * b2.size(), hipMemcpyDeviceToHost));
+    c->sys_rel.assign((size_t)c->nsys, 0.0);
+    double worst = 0.0;
+    for (int k = 0; k < c->nsys; ++k) {
+      double bn = std::sqrt(b2[(size_t)k]);
+      if (std::fabs(bn) < 10.0 * DBL_EPSILON) bn = 1.0;
+      c->sys_rel[(size_t)k] = std::sqrt(r2[(size_t)k]) / bn;
+      worst = std::max(worst, c->sys_rel[(size_t)k]);
+    }
+    st.achieved_tol = worst;
+    st.converged = worst <= p.tol;
+  }
   st.solve_index = (int64_t)c->solve_index;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_systems(mcls_gpu_context* c, int64_t nsys, const int64_t* offsets) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  invalidate_graph(c);
+  if (nsys == 0) {
+    c->nsys = 0;
+    return MCLS_OK;
+  }
+  if (nsys < 0 || nsys > c->n || !offsets) return fail(MCLS_ERR_INVALID_ARG, "bad system count or offsets");
+  if (offsets[0] != 0 || offsets[nsys] != c->n) return fail(MCLS_ERR_INVALID_ARG, "offsets must span [0, n]");
+  std::vector<int> sys_of((size_t)c->n);
+  for (int64_t s = 0; s < nsys; ++s) {
+    if (offsets[s + 1] <= offsets[s]) return fail(MCLS_ERR_INVALID_ARG, "system offsets must increase");
+    for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i) sys_of[(size_t)i] = (int)s;
+  }
+  MCLS_TRY(c->sys_off.ensure((size_t)nsys + 1, "system offsets"));
+  MCLS_TRY(c->sys_of.ensure((size_t)c->n, "system of row"));
+  MCLS_TRY(c->sys_conv.ensure((size_t)nsys, "system flags"));
+  MCLS_TRY(c->sys_unconv.ensure(1, "unconverged count"));
+  MCLS_TRY(c->sys_b2.ensure((size_t)nsys, "system source norms"));
+  MCLS_TRY(c->sys_r2.ensure((size_t)nsys, "system residual norms"));
+  MCLS_TRY(c->sys_l1.ensure((size_t)nsys, "system 1-norms"));
+  HIP_TRY(hipMemcpy(c->sys_off.p, offsets, sizeof(int64_t) * (size_t)(nsys + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->sys_of.p, sys_of.data(), sizeof(int) * sys_of.size(), hipMemcpyHostToDevice));
+  c->sys_off_h.assign(offsets, offsets + nsys + 1);
+  c->nsys = (int)nsys;
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_system_residuals(mcls_gpu_context* c, double* rel) {
+  MCLS_TRY(check_ctx(c));
+  if (!rel) return fail(MCLS_ERR_INVALID_ARG, "null output");
+  if (c->nsys == 0 || c->sys_rel.size() != (size_t)c->nsys)
+    return fail(MCLS_ERR_STATE, "no batched solve has run");
+  std::memcpy(rel, c->sys_rel.data(), sizeof(double) * c->sys_rel.size());
   return MCLS_OK;
 }
 

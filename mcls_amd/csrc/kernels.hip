@@ -165,6 +165,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const int64_t span = hi_id - a.first;
   if (span <= 0) return;
   const double totald = (double)total;
+  const uint64_t key = a.key_dev ? *a.key_dev : a.key;
   // workgroups are dispatched round-robin over the 8 XCDs: slot = XCD
   const int qs = blockIdx.x % kQueueSlots;
   const int64_t sbeg = a.first + span * qs / kQueueSlots;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         const int rank = __popcll(need & __lanemask_lt());
         if (!alive && rank < take) {
           hid = qnext + rank;
-          rng = history_state(a.key, (uint64_t)hid);
+          rng = history_state(key, (uint64_t)hid);
           double w0;
           if (SRC == SRC_RANDOM) {
             const uint64_t t = mul_u53(xorshift64(rng), total);
@@ -290,6 +291,15 @@ hipError_t launch_walk_p(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
   }
   return c.source == SRC_STRATIFIED ? launch_walk_t<Real, EST_COLLISION, SRC_STRATIFIED>(a, c, s)
                                     : launch_walk_t<Real, EST_COLLISION, SRC_RANDOM>(a, c, s);
+}
+
+// Stream key of the next MC solve, advanced on the device so that a captured
+// MCSA iteration replays with a fresh key each time:
+// key = splitmix64(seed ^ splitmix64(index)), index = (*counter)++.
+__global__ void k_next_key(uint64_t seed, unsigned long long* counter, uint64_t* key) {
+  const unsigned long long idx = *counter;
+  *counter = idx + 1;
+  *key = splitmix64(seed ^ splitmix64((uint64_t)idx));
 }
 
 // ------------------------------------------------------------------ source
@@ -741,6 +751,72 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __re
   if (threadIdx.x == 0) out[0] = sm[0];
 }
 
+// ------------------------------------------------------- batched systems
+// One block per system s (rows [off[s], off[s+1])): out[s] = sum_i (d_i v_i)^2
+// (d = dinv or 1) when square, else sum_i |v_i|; fixed-order block tree.
+template <bool SQUARE>
+__global__ __launch_bounds__(kRedThreads) void k_segment_norm(int nsys, const int64_t* __restrict__ off,
+                                                              const double* __restrict__ v,
+                                                              const double* __restrict__ dinv,
+                                                              double* __restrict__ out) {
+  __shared__ double sm[kRedThreads];
+  for (int sys = blockIdx.x; sys < nsys; sys += gridDim.x) {
+    double acc = 0.0;
+    for (int64_t i = off[sys] + threadIdx.x; i < off[sys + 1]; i += kRedThreads) {
+      const double t = dinv ? dinv[i] * v[i] : v[i];
+      acc = SQUARE ? acc + t * t : acc + fabs(t);
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[sys] = sm[0];
+    __syncthreads();
+  }
+}
+
+// Per-system convergence test ||rp_s||_2 <= tol ||PL b_s||_2 (homogeneous
+// systems use 1, src/MCLS_MCSASolverManager_impl.hpp:312-318); conv[s] set
+// once converged; *unconverged = number of systems still iterating.
+__global__ __launch_bounds__(kRedThreads) void k_batch_check(int nsys, const double* __restrict__ r2,
+                                                             const double* __restrict__ b2, double tol,
+                                                             int* __restrict__ conv, int* unconverged) {
+  __shared__ int sm[kRedThreads];
+  int cnt = 0;
+  for (int s = threadIdx.x; s < nsys; s += kRedThreads) {
+    double bn = sqrt(b2[s]);
+    if (fabs(bn) < 10.0 * DBL_EPSILON) bn = 1.0;
+    const bool ok = sqrt(r2[s]) <= tol * bn;
+    if (ok) conv[s] = 1;
+    cnt += conv[s] ? 0 : 1;
+  }
+  sm[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *unconverged = sm[0];
+}
+
+// MC source of a batch: rhs_i = omega rp_i, zero for converged systems;
+// mag_i = |rhs_i| n_s / ||rhs_s||_1 sets the birth allocation (histories
+// per system proportional to its unknown count; any allocation is unbiased
+// because the birth weight divides by the sampled probability).
+__global__ void k_batch_source(int n, const int* __restrict__ sys_of, const int64_t* __restrict__ off,
+                               const double* __restrict__ r, double omega, const int* __restrict__ conv,
+                               const double* __restrict__ l1, double* __restrict__ rhs, double* __restrict__ mag) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int s = sys_of[i];
+    double v = conv[s] ? 0.0 : r[i] * omega;
+    rhs[i] = v;
+    const double norm1 = l1[s] * fabs(omega);
+    mag[i] = (v != 0.0 && norm1 > 0.0) ? fabs(v) * ((double)(off[s + 1] - off[s]) / norm1) : 0.0;
+  }
+}
+
 __global__ void k_axpy(int n, double a, const double* __restrict__ y, double* __restrict__ x) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = x[i] + a * y[i];
 }
@@ -786,6 +862,11 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
                             bool alias, int* flags, hipStream_t s) {
   k_pack_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, hdr32, hdr64, slot32, slot64,
                                                 alias ? 1 : 0, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_next_key(uint64_t seed, unsigned long long* counter, uint64_t* key, hipStream_t s) {
+  k_next_key<<<1, 1, 0, s>>>(seed, counter, key);
   return hipGetLastError();
 }
 
@@ -937,6 +1018,28 @@ hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream
 
 hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part, hipStream_t s) {
   k_scaled_sq<<<residual_blocks(n, 1), kRedThreads, 0, s>>>(n, dinv, b, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_norm(int nsys, const int64_t* off, const double* v, const double* dinv, bool square,
+                               double* out, hipStream_t s) {
+  const int g = nsys < 65535 ? nsys : 65535;
+  if (square)
+    k_segment_norm<true><<<g, kRedThreads, 0, s>>>(nsys, off, v, dinv, out);
+  else
+    k_segment_norm<false><<<g, kRedThreads, 0, s>>>(nsys, off, v, dinv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_check(int nsys, const double* r2, const double* b2, double tol, int* conv,
+                              int* unconverged, hipStream_t s) {
+  k_batch_check<<<1, kRedThreads, 0, s>>>(nsys, r2, b2, tol, conv, unconverged);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_source(int n, const int* sys_of, const int64_t* off, const double* r, double omega,
+                               const int* conv, const double* l1, double* rhs, double* mag, hipStream_t s) {
+  k_batch_source<<<grid_for(n, 256), 256, 0, s>>>(n, sys_of, off, r, omega, conv, l1, rhs, mag);
   return hipGetLastError();
 }
 

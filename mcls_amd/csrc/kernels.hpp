@@ -58,6 +58,7 @@ struct WalkArgs {
   const uint64_t* qcdf;    // inclusive prefix of q (n); Q = qcdf[n-1]
   const uint64_t* ncdf;    // stratified: inclusive prefix of counts; M = ncdf[n-1]
   uint64_t key;            // per-solve stream key
+  const uint64_t* key_dev; // when set, the key is read from device memory
   int64_t first;           // first history id
   int64_t last;            // one past the last history id (random source)
   int64_t history_length;  // max transitions
@@ -91,6 +92,8 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
                             uint4* slot32, uint4* slot64, bool alias, int* flags, hipStream_t s);
 
 // ---------------------------------------------------------------- source
+// *key = splitmix64(seed ^ splitmix64(*counter)); ++*counter  (one thread)
+hipError_t launch_next_key(uint64_t seed, unsigned long long* counter, uint64_t* key, hipStream_t s);
 // bits of max |rhs| (as uint64) -> *bmax (must be zeroed first)
 hipError_t launch_absmax(const double* v, int n, unsigned long long* bmax, hipStream_t s);
 // q_i = ceil(ldexp(|rhs_i|, S - e)), bmax < 2^e
@@ -146,6 +149,14 @@ hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream
 // ||PL b||_2^2 partials
 hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part,
                             hipStream_t s);
+// batched systems (block-diagonal operator, rows [off[s], off[s+1]))
+// out[s] = sum (dinv v)^2 (square) or sum |v| over system s
+hipError_t launch_segment_norm(int nsys, const int64_t* off, const double* v, const double* dinv, bool square,
+                               double* out, hipStream_t s);
+hipError_t launch_batch_check(int nsys, const double* r2, const double* b2, double tol, int* conv,
+                              int* unconverged, hipStream_t s);
+hipError_t launch_batch_source(int n, const int* sys_of, const int64_t* off, const double* r, double omega,
+                               const int* conv, const double* l1, double* rhs, double* mag, hipStream_t s);
 // x += a * y
 hipError_t launch_axpy(int n, double a, const double* y, double* x, hipStream_t s);
 // y = a * x
