@@ -12,8 +12,8 @@ HIPFLAGS := --offload-arch=$(ARCH) -O3 -std=c++17 -fPIC -ffp-contract=off \
             -Iinclude -I$(SRC) -Wall -Wno-unused-command-line-argument
 CXXFLAGS := -O2 -std=c++17 -fPIC -Wall -Iinclude
 
-GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip
-GPU_HDRS := $(SRC)/kernels.hpp include/mcls_gpu.h
+GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip $(SRC)/mlp.hip
+GPU_HDRS := $(SRC)/kernels.hpp $(SRC)/mlp.hpp include/mcls_gpu.h include/mcls_mlp.h
 
 all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade oracle
 

@@ -155,6 +155,42 @@ int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            int ptr_kind);
 int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
 
+/* ------------------------------------------------------ learned proposal -- */
+/* Importance g (n FP32 values > 0) for "Transition Sampler" = "Learned"
+ * (no counterpart in the reference, whose transitions are always
+ * P_ij = |h_ij| / W_i, src/MCLS_AlmostOptimalDomain_impl.hpp:342-363):
+ * the walk then samples P_ij = |h_ij| g_j / S_i, S_i = sum_k |h_ik| g_k, and
+ * multiplies the weight by sign(h_ij) S_i / g_j, which keeps every
+ * estimator unbiased for any g.  g = 1 reproduces the plain transition
+ * probabilities.  The tables are rebuilt on the device. */
+int mcls_gpu_set_importance(mcls_gpu_context* ctx, const float* g, int ptr_kind);
+/* Current importance g (n floats): the one set, or the one the attached
+ * network computed for the last MC source. */
+int mcls_gpu_get_importance(mcls_gpu_context* ctx, float* g, int ptr_kind);
+/* Attach an importance network (include/mcls_mlp.h; NULL detaches).  With
+ * the "Learned" sampler every MC solve then computes g from the network on
+ * its own source: features of (A, source) -> network -> g = exp(clip(o,
+ * -4, 4)) -> proposal tables, all on the device (and inside the captured
+ * MCSA iteration).  The context does not own the network. */
+struct mcls_mlp;
+int mcls_gpu_set_importance_model(mcls_gpu_context* ctx, struct mcls_mlp* m);
+/* Per-state switch (n int8, 0 = use g = 1 there, the plain proposal); NULL
+ * switches every state back on.  The fallback where g raises variance. */
+int mcls_gpu_set_importance_mask(mcls_gpu_context* ctx, const signed char* mask, int ptr_kind);
+/* The network's inputs for source v (n x 8 floats):
+ *   f0..f3  sums of -a_jk / a_jj over off-diagonal k < j-1, k = j-1,
+ *           k = j+1, k > j+1 (the S, W, E, N couplings of a 5-point row)
+ *   f4      sum_{k != j} |a_jk| / |a_jj|
+ *   f5      log(|a_jj| / mean |a_ii|)
+ *   f6      log(|v_j| / mean |v| + 1e-3)
+ *   f7      0 */
+int mcls_gpu_importance_features(mcls_gpu_context* ctx, const double* v, float* X, int ptr_kind);
+/* Training targets from the solver's own adjoint tally: one MC solve of
+ * source v (number `solve_index`, plain transitions), then
+ * t_j = log(clip(|x_j| / mean |x|, e^-4, e^4)) (n floats). */
+int mcls_gpu_importance_targets(mcls_gpu_context* ctx, const double* v, uint64_t solve_index, float* T,
+                                int ptr_kind);
+
 /* ------------------------------------------------------- batched systems -- */
 /* Many independent systems in one launch (no counterpart in the reference,
  * which solves one system per manager; its multi-RHS Thyra adapter loops

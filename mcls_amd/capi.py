@@ -36,7 +36,15 @@ GPU_SYMBOLS = [
     "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
     "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
     "mcls_gpu_mc_solve_raw", "mcls_gpu_set_systems", "mcls_gpu_get_system_residuals",
+    "mcls_gpu_set_importance", "mcls_gpu_get_importance", "mcls_gpu_set_importance_model",
+    "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
 ]
+MLP_SYMBOLS = [
+    "mcls_mlp_create", "mcls_mlp_destroy", "mcls_mlp_get_params", "mcls_mlp_set_params",
+    "mcls_mlp_forward", "mcls_mlp_loss_grad", "mcls_mlp_dataset_clear", "mcls_mlp_dataset_append",
+    "mcls_mlp_dataset_rows", "mcls_mlp_train",
+]
+MLP_PARAMS = 8961
 PROBLEM_SYMBOLS = ["mcls_problem_dims", "mcls_problem_build", "mcls_problem_batch_member",
                    "mcls_problem_uniform"]
 
@@ -121,7 +129,23 @@ def gpu_lib():
     L.mcls_gpu_mc_solve_raw.argtypes = [vp, vp, u64, vp]
     L.mcls_gpu_set_systems.argtypes = [vp, i64, vp]
     L.mcls_gpu_get_system_residuals.argtypes = [vp, vp]
-    for name in GPU_SYMBOLS:
+    L.mcls_gpu_set_importance.argtypes = [vp, vp, i32]
+    L.mcls_gpu_get_importance.argtypes = [vp, vp, i32]
+    L.mcls_gpu_set_importance_model.argtypes = [vp, vp]
+    L.mcls_gpu_set_importance_mask.argtypes = [vp, vp, i32]
+    L.mcls_gpu_importance_features.argtypes = [vp, vp, vp, i32]
+    L.mcls_gpu_importance_targets.argtypes = [vp, vp, u64, vp, i32]
+    L.mcls_mlp_create.argtypes = [C.POINTER(vp), i32, u64]
+    L.mcls_mlp_destroy.argtypes = [vp]
+    L.mcls_mlp_get_params.argtypes = [vp, vp]
+    L.mcls_mlp_set_params.argtypes = [vp, vp]
+    L.mcls_mlp_forward.argtypes = [vp, vp, i64, vp, i32]
+    L.mcls_mlp_loss_grad.argtypes = [vp, vp, vp, i64, C.POINTER(f64), vp, i32]
+    L.mcls_mlp_dataset_clear.argtypes = [vp]
+    L.mcls_mlp_dataset_append.argtypes = [vp, vp, vp, i64, i32]
+    L.mcls_mlp_dataset_rows.argtypes = [vp, C.POINTER(i64)]
+    L.mcls_mlp_train.argtypes = [vp, i32, i64, C.c_float, u64, C.POINTER(f64)]
+    for name in GPU_SYMBOLS + MLP_SYMBOLS:
         f = getattr(L, name)
         if name != "mcls_gpu_last_error":
             f.restype = i32
@@ -342,6 +366,39 @@ class Context:
         self._check(self.L.mcls_gpu_get_system_residuals(self.h, _p(out)))
         return out
 
+    # --- learned proposal
+    def set_importance(self, g):
+        g = np.ascontiguousarray(g, np.float32)
+        self._check(self.L.mcls_gpu_set_importance(self.h, _p(g), MCLS_HOST))
+
+    def importance(self):
+        g = np.empty(self.n, np.float32)
+        self._check(self.L.mcls_gpu_get_importance(self.h, _p(g), MCLS_HOST))
+        return g
+
+    def set_importance_model(self, mlp):
+        self._model = mlp
+        self._check(self.L.mcls_gpu_set_importance_model(self.h, mlp.h if mlp is not None else None))
+
+    def set_importance_mask(self, mask):
+        if mask is None:
+            self._check(self.L.mcls_gpu_set_importance_mask(self.h, None, MCLS_HOST))
+            return
+        m = np.ascontiguousarray(mask, np.int8)
+        self._check(self.L.mcls_gpu_set_importance_mask(self.h, _p(m), MCLS_HOST))
+
+    def importance_features(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        X = np.empty((self.n, 8), np.float32)
+        self._check(self.L.mcls_gpu_importance_features(self.h, _p(v), _p(X), MCLS_HOST))
+        return X
+
+    def importance_targets(self, v, solve_index):
+        v = np.ascontiguousarray(v, np.float64)
+        T = np.empty(self.n, np.float32)
+        self._check(self.L.mcls_gpu_importance_targets(self.h, _p(v), int(solve_index), _p(T), MCLS_HOST))
+        return T
+
     def stats(self):
         s = Stats()
         self._check(self.L.mcls_gpu_get_stats(self.h, C.byref(s)))
@@ -349,3 +406,70 @@ class Context:
 
     def synchronize(self):
         self._check(self.L.mcls_gpu_synchronize(self.h))
+
+
+class Mlp:
+    """The importance network (include/mcls_mlp.h)."""
+
+    def __init__(self, device=0, seed=0):
+        self.L = gpu_lib()
+        self.h = C.c_void_p()
+        self._check(self.L.mcls_mlp_create(C.byref(self.h), device, seed & (2**64 - 1)))
+
+    def _check(self, rc):
+        if rc != MCLS_OK:
+            raise GpuError(rc, self.L.mcls_gpu_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.mcls_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.L.mcls_mlp_get_params(self.h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self.L.mcls_mlp_set_params(self.h, _p(p)))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty(len(X), np.float32)
+        self._check(self.L.mcls_mlp_forward(self.h, _p(X), len(X), _p(out), MCLS_HOST))
+        return out
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_double(0)
+        self._check(self.L.mcls_mlp_loss_grad(self.h, _p(X), _p(T), len(X), C.byref(loss), _p(g), MCLS_HOST))
+        return loss.value, g
+
+    def dataset_clear(self):
+        self._check(self.L.mcls_mlp_dataset_clear(self.h))
+
+    def dataset_append(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        self._check(self.L.mcls_mlp_dataset_append(self.h, _p(X), _p(T), len(X), MCLS_HOST))
+
+    def dataset_rows(self):
+        r = C.c_int64(0)
+        self._check(self.L.mcls_mlp_dataset_rows(self.h, C.byref(r)))
+        return r.value
+
+    def train(self, epochs, batch_rows, lr, seed=0):
+        loss = C.c_double(0)
+        self._check(self.L.mcls_mlp_train(self.h, int(epochs), int(batch_rows), C.c_float(lr),
+                                          seed & (2**64 - 1), C.byref(loss)))
+        return loss.value

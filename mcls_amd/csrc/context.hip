@@ -18,6 +18,8 @@
 
 #include "kernels.hpp"
 #include "mcls_gpu.h"
+#include "mcls_mlp.h"
+#include "mlp.hpp"
 
 namespace {
 
@@ -92,6 +94,10 @@ struct Params {
 
 }  // namespace
 
+namespace mcls {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace mcls
+
 struct mcls_gpu_context {
   int device = 0;
   int num_cus = 256;
@@ -120,6 +126,15 @@ struct mcls_gpu_context {
   DBuf<float> d_cdf32, d_h32, d_W32;
   DBuf<uint4> d_hdr32, d_hdr64, d_slot32, d_slot64;
   bool alias_valid = false;
+  // learned proposal: importance g and its transition tables
+  DBuf<float> g_imp;
+  bool has_importance = false, proposal_valid = false;
+  DBuf<double> d_lcdf64, d_lfac64;
+  DBuf<float> d_lcdf32, d_lfac32;
+  mcls_mlp* model = nullptr;       // recomputes g on each MC source when set
+  DBuf<float> feat, mlp_out;       // n x 8 features, n outputs
+  DBuf<signed char> imp_mask;      // 0 = importance switched off at that state
+  bool has_mask = false;
 
   // work vectors
   DBuf<double> x, b, r, rhs, part, scal;
@@ -207,7 +222,7 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->b.ensure(n, "b"));
   MCLS_TRY(c->r.ensure(n, "r"));
   MCLS_TRY(c->rhs.ensure(n, "rhs"));
-  MCLS_TRY(c->part.ensure((size_t)mcls::kResidualPartials + 1, "partials"));
+  MCLS_TRY(c->part.ensure(2 * (size_t)mcls::kResidualPartials + 1, "partials"));
   MCLS_TRY(c->scal.ensure(8, "scalars"));
   MCLS_TRY(c->q.ensure(n, "q"));
   MCLS_TRY(c->qcdf.ensure(n, "qcdf"));
@@ -303,6 +318,7 @@ int build_domain(mcls_gpu_context* c) {
                                  c->d_hdr64.p, nullptr, nullptr, false, c->flags.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->alias_valid = false;
+  c->proposal_valid = false;
   c->domain_valid = true;
   c->domain_omega = omega;
   c->domain_threshold = thr;
@@ -327,10 +343,66 @@ int ensure_alias(mcls_gpu_context* c) {
   return MCLS_OK;
 }
 
+int ensure_proposal_buffers(mcls_gpu_context* c) {
+  const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
+  MCLS_TRY(c->d_lcdf64.ensure(m, "proposal cdf64"));
+  MCLS_TRY(c->d_lcdf32.ensure(m, "proposal cdf32"));
+  MCLS_TRY(c->d_lfac64.ensure(m, "proposal factors64"));
+  MCLS_TRY(c->d_lfac32.ensure(m, "proposal factors32"));
+  MCLS_TRY(c->g_imp.ensure((size_t)c->n, "importance"));
+  return MCLS_OK;
+}
+
+// Importance features of source v into c->feat (device).
+int compute_features(mcls_gpu_context* c, const double* v) {
+  MCLS_TRY(c->feat.ensure((size_t)c->n * 8, "features"));
+  HIP_TRY(mcls::launch_feature_sums(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, v, c->part.p, c->scal.p + 4,
+                                    c->stream));
+  HIP_TRY(mcls::launch_features(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, v, c->scal.p + 4, c->feat.p,
+                                c->stream));
+  return MCLS_OK;
+}
+
+// g from the attached network on source v, then the proposal tables; all on
+// the stream (capturable).
+int model_importance(mcls_gpu_context* c, const double* v) {
+  MCLS_TRY(ensure_proposal_buffers(c));
+  MCLS_TRY(c->mlp_out.ensure((size_t)c->n, "network output"));
+  MCLS_TRY(compute_features(c, v));
+  HIP_TRY(mcls::launch_mlp_forward(mcls::mlp_params_device(c->model), c->feat.p, c->n, c->mlp_out.p, c->stream));
+  HIP_TRY(mcls::launch_importance(c->n, c->mlp_out.p, c->has_mask ? c->imp_mask.p : nullptr, c->g_imp.p,
+                                  c->stream));
+  HIP_TRY(mcls::launch_build_proposal(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lcdf64.p,
+                                      c->d_lcdf32.p, c->d_lfac64.p, c->d_lfac32.p, c->flags.p, c->stream));
+  return MCLS_OK;
+}
+
+// Learned-proposal tables from the stored importance (rebuilt after a
+// domain rebuild or a new g).  With a network attached the tables are
+// rebuilt for every MC source instead.
+int ensure_proposal(mcls_gpu_context* c) {
+  if (c->prm.sampler != mcls::SAMPLER_LEARNED || c->proposal_valid) return MCLS_OK;
+  if (c->model) return ensure_proposal_buffers(c);
+  if (!c->has_importance)
+    return fail(MCLS_ERR_STATE, "Transition Sampler 'Learned' needs an importance (mcls_gpu_set_importance)");
+  MCLS_TRY(ensure_proposal_buffers(c));
+  MCLS_TRY(reset_flags(c));
+  HIP_TRY(mcls::launch_build_proposal(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lcdf64.p,
+                                      c->d_lcdf32.p, c->d_lfac64.p, c->d_lfac32.p, c->flags.p, c->stream));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_BAD_IMPORTANCE] != INT_MAX)
+    return fail(MCLS_ERR_NUMERIC, "importance g must be finite and > 0 (domain row " +
+                                      std::to_string(flags[mcls::FLAG_BAD_IMPORTANCE] - 1) + ")");
+  c->proposal_valid = true;
+  return MCLS_OK;
+}
+
 int ensure_domain_impl(mcls_gpu_context* c);
 int ensure_domain(mcls_gpu_context* c) {
   MCLS_TRY(ensure_domain_impl(c));
-  return ensure_alias(c);
+  MCLS_TRY(ensure_alias(c));
+  return ensure_proposal(c);
 }
 
 int ensure_domain_impl(mcls_gpu_context* c) {
@@ -379,6 +451,10 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.hdr64 = c->d_hdr64.p;
   a.d.slot32 = c->d_slot32.p;
   a.d.slot64 = c->d_slot64.p;
+  a.d.lcdf64 = c->d_lcdf64.p;
+  a.d.lcdf32 = c->d_lcdf32.p;
+  a.d.lfac64 = c->d_lfac64.p;
+  a.d.lfac32 = c->d_lfac32.p;
   a.queue = c->queue.p;
   a.rhs = rhs;
   a.q = c->q.p;
@@ -421,6 +497,7 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
+  if (c->prm.sampler == mcls::SAMPLER_LEARNED && c->model) MCLS_TRY(model_importance(c, rhs));
   MCLS_TRY(build_source(c, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
   if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
   const bool strat = c->prm.source == mcls::SRC_STRATIFIED;
@@ -634,6 +711,8 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   c->has_matrix = true;
   c->res_lanes = mcls::residual_lanes(n, nnz);
   c->nsys = 0;  // system offsets describe the previous operator
+  c->has_importance = false;
+  c->proposal_valid = false;
   if (c->has_prec && (int64_t)c->dinv.n < n) c->has_prec = false;
   MCLS_TRY(ensure_vectors(c));
   return MCLS_OK;
@@ -771,6 +850,7 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
   } else if (key_is(key, "Transition Sampler")) {
     if (key_is(v, "CDF")) p.sampler = mcls::SAMPLER_CDF;
     else if (key_is(v, "Alias")) p.sampler = mcls::SAMPLER_ALIAS;
+    else if (key_is(v, "Learned")) p.sampler = mcls::SAMPLER_LEARNED;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Transition Sampler '") + v + "'");
   } else if (key_is(key, "Walk Precision")) {
     if (key_is(v, "FP64")) p.precision = 64;
@@ -1080,6 +1160,93 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   }
   st.solve_index = (int64_t)c->solve_index;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!g) return fail(MCLS_ERR_INVALID_ARG, "null importance");
+  MCLS_TRY(c->g_imp.ensure((size_t)c->n, "importance"));
+  HIP_TRY(hipMemcpy(c->g_imp.p, g, sizeof(float) * (size_t)c->n, kind_to_dev(ptr_kind)));
+  c->has_importance = true;
+  c->proposal_valid = false;
+  invalidate_graph(c);
+  if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_importance(mcls_gpu_context* c, float* g, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  if (!g) return fail(MCLS_ERR_INVALID_ARG, "null output");
+  if (!c->g_imp.p) return fail(MCLS_ERR_STATE, "no importance has been set or computed");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(g, c->g_imp.p, sizeof(float) * (size_t)c->n, kind_from_dev(ptr_kind)));
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_importance_model(mcls_gpu_context* c, mcls_mlp* m) {
+  MCLS_TRY(check_ctx(c));
+  if (m && mcls::mlp_device(m) != c->device)
+    return fail(MCLS_ERR_INVALID_ARG, "network and context are on different devices");
+  c->model = m;
+  c->proposal_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_importance_mask(mcls_gpu_context* c, const signed char* mask, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  invalidate_graph(c);
+  if (!mask) {
+    c->has_mask = false;
+    return MCLS_OK;
+  }
+  MCLS_TRY(c->imp_mask.ensure((size_t)c->n, "importance mask"));
+  HIP_TRY(hipMemcpy(c->imp_mask.p, mask, (size_t)c->n, kind_to_dev(ptr_kind)));
+  c->has_mask = true;
+  return MCLS_OK;
+}
+
+int mcls_gpu_importance_features(mcls_gpu_context* c, const double* v, float* X, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!v || !X) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  MCLS_TRY(ensure_vectors(c));
+  HIP_TRY(hipMemcpyAsync(c->rhs.p, v, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), c->stream));
+  MCLS_TRY(compute_features(c, c->rhs.p));
+  HIP_TRY(hipMemcpyAsync(X, c->feat.p, sizeof(float) * 8 * (size_t)c->n, kind_from_dev(ptr_kind), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MCLS_OK;
+}
+
+int mcls_gpu_importance_targets(mcls_gpu_context* c, const double* v, uint64_t solve_index, float* T, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!v || !T) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  MCLS_TRY(ensure_vectors(c));
+  // one plain adjoint MC solve of v (almost-optimal transitions)
+  const int sampler = c->prm.sampler;
+  if (sampler == mcls::SAMPLER_LEARNED) c->prm.sampler = mcls::SAMPLER_CDF;
+  int rc = ensure_domain(c);
+  if (rc == MCLS_OK) {
+    HIP_TRY(hipMemcpyAsync(c->rhs.p, v, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), c->stream));
+    rc = mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index);
+  }
+  c->prm.sampler = sampler;
+  MCLS_TRY(rc);
+  MCLS_TRY(c->mlp_out.ensure((size_t)c->n, "targets"));
+  HIP_TRY(mcls::launch_feature_sums(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->x.p, c->part.p,
+                                    c->scal.p + 4, c->stream));
+  HIP_TRY(mcls::launch_targets(c->n, c->x.p, c->scal.p + 4, c->mlp_out.p, c->stream));
+  HIP_TRY(hipMemcpyAsync(T, c->mlp_out.p, sizeof(float) * (size_t)c->n, kind_from_dev(ptr_kind), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;
 }
 

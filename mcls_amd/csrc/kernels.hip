@@ -95,6 +95,8 @@ __device__ __forceinline__ void tally_add(float* p, float v) { unsafeAtomicAdd(p
 
 template <typename Real> struct Tables;
 template <> struct Tables<double> {
+  __device__ static const double* lcdf(const DomainView& d) { return d.lcdf64; }
+  __device__ static const double* lfac(const DomainView& d) { return d.lfac64; }
   __device__ static const double* cdf(const DomainView& d) { return d.cdf64; }
   __device__ static const double* W(const DomainView& d) { return d.W64; }
   __device__ static const double* h(const DomainView& d) { return d.h64; }
@@ -102,6 +104,8 @@ template <> struct Tables<double> {
   __device__ static double u01(uint64_t r) { return u01_f64(r); }
 };
 template <> struct Tables<float> {
+  __device__ static const float* lcdf(const DomainView& d) { return d.lcdf32; }
+  __device__ static const float* lfac(const DomainView& d) { return d.lfac32; }
   __device__ static const float* cdf(const DomainView& d) { return d.cdf32; }
   __device__ static const float* W(const DomainView& d) { return d.W32; }
   __device__ static const float* h(const DomainView& d) { return d.h32; }
@@ -243,13 +247,20 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if (!end) {
       const uint64_t r = xorshift64(rng);
       uint32_t cp;
-      if (SAMP == SAMPLER_ALIAS) {
-        cp = Row<Real>::pick(d, beg, len, r);
-      } else {
-        const int k = lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
+      if (SAMP == SAMPLER_LEARNED) {
+        // importance-reshaped row: P = |h| g / S, weight factor S / g_j
+        const int k = lower_bound_cdf<Real>(Tables<Real>::lcdf(d) + beg, (int)len, Tables<Real>::u01(r));
         cp = colp[beg + k];
+        w = w * Tables<Real>::lfac(d)[beg + k];
+      } else {
+        if (SAMP == SAMPLER_ALIAS) {
+          cp = Row<Real>::pick(d, beg, len, r);
+        } else {
+          const int k = lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
+          cp = colp[beg + k];
+        }
+        w = w * Row<Real>::W(hd);
       }
-      w = w * Row<Real>::W(hd);
       if (cp & 0x80000000u) w = -w;
       state = (int)(cp & 0x7FFFFFFFu);
       ++steps;
@@ -279,6 +290,7 @@ hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
 
 template <typename Real, int EST, int SRC>
 hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if (c.sampler == SAMPLER_LEARNED) return launch_walk_s<Real, EST, SRC, SAMPLER_LEARNED>(a, c, s);
   return c.sampler == SAMPLER_ALIAS ? launch_walk_s<Real, EST, SRC, SAMPLER_ALIAS>(a, c, s)
                                     : launch_walk_s<Real, EST, SRC, SAMPLER_CDF>(a, c, s);
 }
@@ -678,6 +690,126 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
   }
 }
 
+// --------------------------------------------- importance features / targets
+// Per-block partial sums of |a_jj| and |v_j| (grid-stride, fixed grid).
+template <typename AV>
+__global__ __launch_bounds__(kRedThreads) void k_feature_sums(int n, const int* __restrict__ rp,
+                                                              const int* __restrict__ col, const void* val,
+                                                              const double* __restrict__ v, double* part_d,
+                                                              double* part_v) {
+  __shared__ double sd[kRedThreads], sv[kRedThreads];
+  const AV* __restrict__ a = (const AV*)val;
+  double accd = 0.0, accv = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      if (col[k] == i) accd = accd + fabs((double)a[k]);
+    accv = accv + fabs(v[i]);
+  }
+  sd[threadIdx.x] = accd;
+  sv[threadIdx.x] = accv;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sd[threadIdx.x] = sd[threadIdx.x] + sd[threadIdx.x + o];
+      sv[threadIdx.x] = sv[threadIdx.x] + sv[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_d[blockIdx.x] = sd[0];
+    part_v[blockIdx.x] = sv[0];
+  }
+}
+
+// Eight features per state j from row j of A and the MC source v:
+//   f0..f3: sums of -a_jk / a_jj over off-diagonal columns k < j-1, k = j-1,
+//           k = j+1, k > j+1 (the S, W, E, N couplings of a 2-D 5-point row)
+//   f4: sum_k |a_jk| / |a_jj| over k != j (the row sum of |H|)
+//   f5: log(|a_jj| / mean |a_ii|)
+//   f6: log(|v_j| / mean |v| + 1e-3)
+//   f7: 0
+template <typename AV>
+__global__ void k_features(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                           const double* __restrict__ v, const double* __restrict__ sums, float* __restrict__ X) {
+  const AV* __restrict__ a = (const AV*)val;
+  const double mean_d = sums[0] / (double)n, mean_v = sums[1] / (double)n;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int k = rp[j]; k < rp[j + 1]; ++k)
+      if (col[k] == j) d = (double)a[k];
+    double f[4] = {0.0, 0.0, 0.0, 0.0}, rs = 0.0;
+    const double inv = d != 0.0 ? 1.0 / d : 0.0;
+    for (int k = rp[j]; k < rp[j + 1]; ++k) {
+      const int c = col[k];
+      if (c == j) continue;
+      const double t = -(double)a[k] * inv;
+      const int cls = c < j - 1 ? 0 : (c == j - 1 ? 1 : (c == j + 1 ? 2 : 3));
+      f[cls] += t;
+      rs += fabs(t);
+    }
+    float* x = X + (size_t)j * 8;
+    x[0] = (float)f[0];
+    x[1] = (float)f[1];
+    x[2] = (float)f[2];
+    x[3] = (float)f[3];
+    x[4] = (float)rs;
+    x[5] = (float)(mean_d > 0.0 && d != 0.0 ? log(fabs(d) / mean_d) : 0.0);
+    x[6] = (float)log((mean_v > 0.0 ? fabs(v[j]) / mean_v : 0.0) + 1e-3);
+    x[7] = 0.f;
+  }
+}
+
+// g_j = exp(clip(o_j, -4, 4)); states switched off by the mask use g = 1.
+__global__ void k_importance(int n, const float* __restrict__ o, const signed char* __restrict__ mask,
+                             float* __restrict__ g) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const float c = fminf(fmaxf(o[j], -4.f), 4.f);
+    g[j] = (mask && !mask[j]) ? 1.f : expf(c);
+  }
+}
+
+// Training target from an adjoint tally x: t_j = log(clip(|x_j| / mean|x|, e^-4, e^4)).
+__global__ void k_targets(int n, const double* __restrict__ x, const double* __restrict__ sums,
+                          float* __restrict__ T) {
+  const double mean = sums[1] / (double)n;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    double r = mean > 0.0 ? fabs(x[j]) / mean : 1.0;
+    r = fmin(fmax(r, exp(-4.0)), exp(4.0));
+    T[j] = (float)log(r);
+  }
+}
+
+// Learned-proposal tables, per row sequentially in the same operation order
+// as the CPU restatement: acc += |h| g_j; lcdf = acc_k / S; lfac = S / g_j.
+__global__ void k_build_proposal(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                                 const double* __restrict__ h64, const float* __restrict__ g,
+                                 double* __restrict__ lcdf64, float* __restrict__ lcdf32,
+                                 double* __restrict__ lfac64, float* __restrict__ lfac32, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = d_rp[i], end = d_rp[i + 1];
+    double acc = 0.0;
+    bool bad = false;
+    for (int p = beg; p < end; ++p) {
+      const double gj = (double)g[col[p] & 0x7FFFFFFFu];
+      if (!(gj > 0.0) || !(gj <= DBL_MAX)) bad = true;
+      acc = fabs(h64[p]) * gj + acc;
+      lcdf64[p] = acc;
+    }
+    if (bad) {
+      set_flag(flags, FLAG_BAD_IMPORTANCE, i);
+      continue;
+    }
+    for (int p = beg; p < end; ++p) {
+      const double c = lcdf64[p] / acc;
+      const double f = acc / (double)g[col[p] & 0x7FFFFFFFu];
+      lcdf64[p] = c;
+      lcdf32[p] = (float)c;
+      lfac64[p] = f;
+      lfac32[p] = (float)f;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- linalg
 // r = PL (b - A x) with V lanes per row ("vector CSR"): the lanes of a row
 // read its entries with unit stride, so a wave's loads are coalesced; the
@@ -855,6 +987,44 @@ __global__ void k_tally_out(int n, double* __restrict__ x, double* __restrict__ 
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
   k_fill<unsigned long long><<<1, kQueueWords, 0, s>>>(a.queue, kQueueWords, 0ULL);
   return c.precision == 32 ? launch_walk_p<float>(a, c, s) : launch_walk_p<double>(a, c, s);
+}
+
+hipError_t launch_feature_sums(int n, const int* rp, const int* col, const void* val, bool f32, const double* v,
+                               double* part, double* sums, hipStream_t s) {
+  const int g = residual_blocks(n, 1);
+  if (f32)
+    k_feature_sums<float><<<g, kRedThreads, 0, s>>>(n, rp, col, val, v, part, part + kResidualPartials);
+  else
+    k_feature_sums<double><<<g, kRedThreads, 0, s>>>(n, rp, col, val, v, part, part + kResidualPartials);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part, g, sums);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part + kResidualPartials, g, sums + 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_features(int n, const int* rp, const int* col, const void* val, bool f32, const double* v,
+                           const double* sums, float* X, hipStream_t s) {
+  if (f32)
+    k_features<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, v, sums, X);
+  else
+    k_features<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, v, sums, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_importance(int n, const float* o, const signed char* mask, float* g, hipStream_t s) {
+  k_importance<<<grid_for(n, 256), 256, 0, s>>>(n, o, mask, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_targets(int n, const double* x, const double* sums, float* T, hipStream_t s) {
+  k_targets<<<grid_for(n, 256), 256, 0, s>>>(n, x, sums, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, const double* h64, const float* g,
+                                 double* lcdf64, float* lcdf32, double* lfac64, float* lfac32, int* flags,
+                                 hipStream_t s) {
+  k_build_proposal<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, g, lcdf64, lcdf32, lfac64, lfac32, flags);
+  return hipGetLastError();
 }
 
 hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,

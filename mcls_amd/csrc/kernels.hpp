@@ -14,6 +14,7 @@ enum DeviceFlag : int {
   FLAG_ZERO_DIAG = 2,    // zero / missing diagonal under point Jacobi
   FLAG_NONFINITE = 3,    // NaN / Inf in a matrix value
   FLAG_LONG_ROW = 4,     // domain row longer than the alias builder handles
+  FLAG_BAD_IMPORTANCE = 5,  // importance g <= 0 or non-finite
   FLAG_COUNT = 8
 };
 
@@ -40,11 +41,16 @@ struct DomainView {
   const uint4* hdr64;     // n
   const uint4* slot32;    // nnz (alias sampler)
   const uint4* slot64;    // nnz (alias sampler)
+  // learned proposal: P_ij = |h_ij| g_j / S_i (CDF), weight factor S_i / g_j
+  const double* lcdf64;   // nnz
+  const float* lcdf32;    // nnz
+  const double* lfac64;   // nnz
+  const float* lfac32;    // nnz
 };
 
 enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
-enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1 };
+enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1, SAMPLER_LEARNED = 2 };
 
 // Work queue of the walk kernel: one counter per XCD slot, 128 bytes apart.
 constexpr int kQueueSlots = 8;
@@ -84,6 +90,23 @@ struct WalkConfig {
 
 // Zeroes the queue counters, then launches the walk.
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s);
+
+// Importance features: sums[0] = sum |a_jj|, sums[1] = sum |v_j| (part holds
+// 2 * kResidualPartials doubles), then X (n x 8) from A's rows and v.
+hipError_t launch_feature_sums(int n, const int* rp, const int* col, const void* val, bool f32, const double* v,
+                               double* part, double* sums, hipStream_t s);
+hipError_t launch_features(int n, const int* rp, const int* col, const void* val, bool f32, const double* v,
+                           const double* sums, float* X, hipStream_t s);
+// g = exp(clip(o, -4, 4)), 1 where mask (nullable) is 0
+hipError_t launch_importance(int n, const float* o, const signed char* mask, float* g, hipStream_t s);
+// T = log(clip(|x| / mean|x|, e^-4, e^4)), mean from sums[1] / n
+hipError_t launch_targets(int n, const double* x, const double* sums, float* T, hipStream_t s);
+
+// Learned-proposal tables from the importance g (n, FP32, > 0); a row that
+// reaches g <= 0 or a non-finite g sets FLAG_BAD_IMPORTANCE.
+hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, const double* h64, const float* g,
+                                 double* lcdf64, float* lcdf32, double* lfac64, float* lfac32, int* flags,
+                                 hipStream_t s);
 
 // Row headers, and (alias != 0) Vose alias tables packed into slot records.
 // Rows longer than 64 entries set FLAG_LONG_ROW when alias tables are built.

@@ -348,7 +348,31 @@ typedef struct {
   double* thr;   /* alias tables (sampler == ORC_SAMPLER_ALIAS) */
   float* thr32;
   int* alias;
+  double* lcdf;  /* proposal tables (sampler == ORC_SAMPLER_LEARNED) */
+  double* lfac;
+  float* lcdf32;
+  float* lfac32;
 } domain_view;
+
+int orc_build_proposal(int n, const int64_t* d_rp, const int* d_col,
+                       const double* d_h, const float* g, double* lcdf,
+                       double* lfac) {
+  for (int i = 0; i < n; ++i) {
+    int64_t beg = d_rp[i], end = d_rp[i + 1];
+    double acc = 0.0;
+    for (int64_t p = beg; p < end; ++p) {
+      double gj = (double)g[d_col[p]];
+      if (!(gj > 0.0)) return -(i + 1);
+      acc = fabs(d_h[p]) * gj + acc;
+      lcdf[p] = acc;
+    }
+    for (int64_t p = beg; p < end; ++p) {
+      lcdf[p] = lcdf[p] / acc;
+      lfac[p] = acc / (double)g[d_col[p]];
+    }
+  }
+  return 0;
+}
 
 typedef struct {
   const domain_view* d;
@@ -402,9 +426,15 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       }
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
-      int k = d->alias ? alias_pick32(r, len, d->thr32 + beg, d->alias + beg)
-                       : orc_lower_bound_f32(d->cdf32 + beg, len, orc_u01_f32(r));
-      w = w * d->W32[state];
+      int k;
+      if (d->lcdf32) {  /* importance-reshaped: P = |h| g / S, factor S / g */
+        k = orc_lower_bound_f32(d->lcdf32 + beg, len, orc_u01_f32(r));
+        w = w * d->lfac32[beg + k];
+      } else {
+        k = d->alias ? alias_pick32(r, len, d->thr32 + beg, d->alias + beg)
+                     : orc_lower_bound_f32(d->cdf32 + beg, len, orc_u01_f32(r));
+        w = w * d->W32[state];
+      }
       if (d->h32[beg + k] < 0.0f) w = -w;
       state = d->col[beg + k];
       ++steps;
@@ -426,9 +456,15 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       }
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
-      int k = d->alias ? alias_pick64(r, len, d->thr + beg, d->alias + beg)
-                       : orc_lower_bound_f64(d->cdf + beg, len, orc_u01_f64(r));
-      w = w * d->W[state];
+      int k;
+      if (d->lcdf) {
+        k = orc_lower_bound_f64(d->lcdf + beg, len, orc_u01_f64(r));
+        w = w * d->lfac[beg + k];
+      } else {
+        k = d->alias ? alias_pick64(r, len, d->thr + beg, d->alias + beg)
+                     : orc_lower_bound_f64(d->cdf + beg, len, orc_u01_f64(r));
+        w = w * d->W[state];
+      }
       if (d->h[beg + k] < 0.0) w = -w;
       state = d->col[beg + k];
       ++steps;
@@ -458,7 +494,7 @@ static void* walk_worker(void* arg) {
 static void domain_view_init(domain_view* d, int n, const int64_t* rp,
                              const int* col, const double* h,
                              const double* cdf, const double* W, int need32,
-                             int need_alias) {
+                             int need_alias, const float* importance) {
   d->n = n;
   d->rp = rp;
   d->col = col;
@@ -469,6 +505,20 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
   d->thr = NULL;
   d->thr32 = NULL;
   d->alias = NULL;
+  d->lcdf = d->lfac = NULL;
+  d->lcdf32 = d->lfac32 = NULL;
+  if (importance) {
+    int64_t nnz = rp[n];
+    d->lcdf = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    d->lfac = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    d->lcdf32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
+    d->lfac32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
+    orc_build_proposal(n, rp, col, h, importance, d->lcdf, d->lfac);
+    for (int64_t k = 0; k < nnz; ++k) {
+      d->lcdf32[k] = (float)d->lcdf[k];
+      d->lfac32[k] = (float)d->lfac[k];
+    }
+  }
   if (need_alias) {
     int64_t nnz = rp[n];
     d->thr = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
@@ -492,6 +542,10 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
 }
 
 static void domain_view_free(domain_view* d) {
+  free(d->lcdf);
+  free(d->lfac);
+  free(d->lcdf32);
+  free(d->lfac32);
   free(d->thr);
   free(d->thr32);
   free(d->alias);
@@ -509,7 +563,8 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
                      int* paths, double* weights) {
   domain_view d;
   domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32,
-                   p->sampler == ORC_SAMPLER_ALIAS);
+                   p->sampler == ORC_SAMPLER_ALIAS,
+                   p->sampler == ORC_SAMPLER_LEARNED ? p->importance : NULL);
   uint64_t* q = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* qcdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* ncdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));

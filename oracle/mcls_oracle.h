@@ -74,10 +74,20 @@ int orc_source_birth(int n, const double* b, const uint64_t* q,
 int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
                         const double* d_W, double* thr, int* alias);
 
+/* Importance-reshaped transition tables (build extension, the learned
+ * proposal): for domain row i with entries (j, h_ij) and importance g_j > 0,
+ * P_ij = |h_ij| g_j / S_i with S_i = sum_k |h_ik| g_k, sampled by CDF
+ * inversion of lcdf, and the weight factor fac_ij = S_i / g_j (times the
+ * sign of h_ij), so E[w'] = sum_j P_ij fac_ij sign(h_ij) = sum_j h_ij:
+ * unbiased for any g.  g is FP32 (the network's output). */
+int orc_build_proposal(int n, const int64_t* d_rp, const int* d_col,
+                       const double* d_h, const float* g, double* lcdf,
+                       double* lfac);
+
 /* ------------------------------------------------------------- MC solve -- */
 enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1 };
 enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
-enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1 };
+enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2 };
 
 typedef struct orc_mc_params {
   int64_t num_histories;   /* histories per MC solve                       */
@@ -89,6 +99,7 @@ typedef struct orc_mc_params {
   uint64_t seed;
   int64_t num_threads;     /* worker threads (tallies reduced in order)    */
   int64_t sampler;         /* ORC_SAMPLER_* (transition sampling)          */
+  const float* importance; /* g (n) for ORC_SAMPLER_LEARNED, else NULL      */
 } orc_mc_params;
 
 /* One adjoint MC solve over the domain: x = (1/nh) * sum of tallies (plus

@@ -18,6 +18,7 @@ SRC_RANDOM = 0
 SRC_STRATIFIED = 1
 SAMPLER_CDF = 0
 SAMPLER_ALIAS = 1
+SAMPLER_LEARNED = 2
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -37,6 +38,7 @@ class MCParams(C.Structure):
         ("seed", C.c_uint64),
         ("num_threads", C.c_int64),
         ("sampler", C.c_int64),
+        ("importance", C.c_void_p),
     ]
 
 
@@ -98,6 +100,8 @@ def lib():
     L.orc_build_domain.argtypes = [i32, _i64p, _i32p, _f64p, vp, f64, f64,
                                    _i64p, _i32p, _f64p, _f64p, _f64p]
     L.orc_build_domain.restype = i64
+    L.orc_build_proposal.argtypes = [i32, _i64p, _i32p, _f64p, _f32p, _f64p, _f64p]
+    L.orc_build_proposal.restype = i32
     L.orc_build_alias.argtypes = [i32, _i64p, _f64p, _f64p, _f64p, _i32p]
     L.orc_build_alias.restype = i64
     L.orc_jacobi_dinv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p]
@@ -215,10 +219,27 @@ def build_domain(A, dinv=None, omega=1.0, threshold=0.0):
 
 def mc_params(num_histories, history_length=10, weight_cutoff=0.0, precision=64,
               estimator=EST_COLLISION, source=SRC_RANDOM, seed=0, num_threads=1,
-              sampler=SAMPLER_CDF):
-    return MCParams(int(num_histories), int(history_length), float(weight_cutoff),
-                    int(precision), int(estimator), int(source), int(seed) & (2**64 - 1),
-                    int(num_threads), int(sampler))
+              sampler=SAMPLER_CDF, importance=None):
+    """importance: float32 g (n) for the learned proposal (kept referenced)."""
+    imp = None
+    if importance is not None:
+        imp = np.ascontiguousarray(importance, np.float32)
+        sampler = SAMPLER_LEARNED
+    p = MCParams(int(num_histories), int(history_length), float(weight_cutoff),
+                 int(precision), int(estimator), int(source), int(seed) & (2**64 - 1),
+                 int(num_threads), int(sampler), None if imp is None else imp.ctypes.data)
+    p._importance_ref = imp
+    return p
+
+
+def build_proposal(dom, g):
+    g = np.ascontiguousarray(g, np.float32)
+    lcdf = np.zeros(max(dom.nnz, 1))
+    lfac = np.zeros(max(dom.nnz, 1))
+    rc = lib().orc_build_proposal(dom.n, dom.rp, dom.col, dom.h, g, lcdf, lfac)
+    if rc != 0:
+        raise ValueError(f"non-positive importance reached from row {-rc - 1}")
+    return lcdf[: dom.nnz], lfac[: dom.nnz]
 
 
 def build_alias(dom):

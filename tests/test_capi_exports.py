@@ -23,6 +23,16 @@ def test_gpu_lib_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_mlp_exports_every_declared_symbol():
+    decl = _declared("mcls_mlp.h")
+    assert sorted(capi.MLP_SYMBOLS) == decl
+    lib = C.CDLL(capi.GPU_LIB)
+    for name in decl:
+        assert hasattr(lib, name), name
+    text = open(os.path.join(ROOT, "include", "mcls_mlp.h")).read()
+    assert f"MCLS_MLP_PARAMS {capi.MLP_PARAMS}" in text
+
+
 def test_problem_lib_exports_every_declared_symbol():
     decl = _declared("mcls_problems.h")
     assert sorted(capi.PROBLEM_SYMBOLS) == decl

@@ -1,0 +1,161 @@
+"""Learned proposal on the GPU: the MFMA network against its CPU oracle, the
+importance-reshaped walk against the CPU restatement (bit-exact paths), the
+features and targets against their definitions."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from mcls_amd import capi
+from oracle import mlp_oracle as mo
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(rows, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(rows, 8)).astype(np.float32)
+    X[:, 7] = 0.0
+    return X
+
+
+def test_mlp_forward_matches_oracle():
+    net = capi.Mlp(0, seed=3)
+    p = net.params().astype(np.float64)
+    p[mo.N_IN * mo.N_HID: mo.N_IN * mo.N_HID + mo.N_HID] = 0.1  # non-zero b1
+    net.set_params(p.astype(np.float32))
+    X = _inputs(1000, 1)  # not a multiple of 16: exercises the tail tile
+    got = net.forward(X)
+    want, _ = mo.forward(net.params(), X, emulate=True)
+    err = np.abs(got - want)
+    assert np.median(err) < 1e-4 * (1 + np.median(np.abs(want)))
+    assert np.max(err / (1 + np.abs(want))) < 3e-2
+
+
+def test_mlp_gradient_matches_oracle():
+    net = capi.Mlp(0, seed=5)
+    X = _inputs(777, 2)
+    T = np.random.default_rng(3).normal(size=777).astype(np.float32)
+    loss, g = net.loss_grad(X, T)
+    lo, go = mo.loss_grad(net.params(), X, T, emulate=True)
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert np.linalg.norm(g - go) <= 2e-2 * np.linalg.norm(go)
+    # per layer
+    k = 0
+    for name, shape in mo.SHAPES:
+        size = int(np.prod(shape))
+        a, b = g[k:k + size], go[k:k + size]
+        assert np.linalg.norm(a - b) <= 5e-2 * (np.linalg.norm(b) + 1e-12), name
+        k += size
+
+
+def test_mlp_training_reduces_loss():
+    net = capi.Mlp(0, seed=7)
+    X = _inputs(8192, 4)
+    T = (np.tanh(X[:, 0]) + 0.5 * X[:, 1] ** 2).astype(np.float32)
+    net.dataset_clear()
+    net.dataset_append(X, T)
+    assert net.dataset_rows() == 8192
+    l0, _ = net.loss_grad(X, T)
+    l1 = net.train(epochs=40, batch_rows=512, lr=3e-3, seed=1)
+    l2, _ = net.loss_grad(X, T)
+    assert l2 < 0.1 * l0 and l1 < 0.2 * l0
+
+
+def _varcoef(n=24, seed=9):
+    return capi.build_problem(capi.make_spec(capi.VARCOEF2D_5PT, n, n, seed=seed, p0=0.1, p1=1.0, p2=64))
+
+
+def _features_numpy(prob, v):
+    A = sp.csr_matrix((prob.values(), prob.col, prob.rp), shape=(prob.n, prob.n))
+    n = prob.n
+    X = np.zeros((n, 8))
+    d = A.diagonal()
+    for j in range(n):
+        for k in range(A.indptr[j], A.indptr[j + 1]):
+            c = A.indices[k]
+            if c == j:
+                continue
+            t = -A.data[k] / d[j]
+            cls = 0 if c < j - 1 else (1 if c == j - 1 else (2 if c == j + 1 else 3))
+            X[j, cls] += t
+            X[j, 4] += abs(t)
+    X[:, 5] = np.log(np.abs(d) / np.abs(d).mean())
+    X[:, 6] = np.log(np.abs(v) / np.abs(v).mean() + 1e-3)
+    return X.astype(np.float32)
+
+
+def test_features_match_definition():
+    prob = _varcoef()
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        v = prob.b - 0.3
+        X = ctx.importance_features(v)
+    assert np.allclose(X, _features_numpy(prob, v), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_learned_paths_bitexact(prec):
+    prob = _varcoef()
+    g = np.exp(np.random.default_rng(1).normal(size=prob.n)).astype(np.float32)
+    L = 120
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "Learned", "Walk Precision": f"FP{prec}", "History Length": L,
+                        "Weight Cutoff": 1e-3, "Number of Histories": 4000, "Source Sampling Type": "Stratified",
+                        "Random Seed": 77})
+        ctx.set_importance(g)
+        A = po.Csr(prob.rp, prob.col, prob.values())
+        dinv = po.jacobi_dinv(A)
+        rhs = prob.b * dinv
+        gp, gw = ctx.record_paths(rhs, 4, 0, 2000, L)
+        xg = ctx.mc_solve_raw(rhs, 4)
+    dom = po.build_domain(A, dinv)
+    mp = po.mc_params(4000, L, 1e-3, prec, source=po.SRC_STRATIFIED, seed=77, importance=g)
+    op, ow = po.record_paths(dom, rhs, mp, 0, 2000, L, solve_index=4)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    xo, _ = po.mc_solve(dom, rhs, mp, solve_index=4)
+    tol = 1e-10 if prec == 64 else 2e-5
+    assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
+
+
+def test_model_importance_and_mask():
+    prob = _varcoef()
+    net = capi.Mlp(0, seed=11)
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "Learned", "Walk Precision": "FP32", "History Length": 50,
+                        "Number of Histories": 4000, "Source Sampling Type": "Stratified", "Random Seed": 1})
+        ctx.set_importance_model(net)
+        A = po.Csr(prob.rp, prob.col, prob.values())
+        rhs = prob.b * po.jacobi_dinv(A)
+        ctx.mc_solve_raw(rhs, 0)
+        g = ctx.importance()
+        X = ctx.importance_features(rhs)
+        o, _ = mo.forward(net.params(), X, emulate=True)
+        want = np.exp(np.clip(o, -4, 4))
+        assert np.max(np.abs(g - want) / want) < 5e-2
+        mask = np.ones(prob.n, np.int8)
+        mask[::2] = 0
+        ctx.set_importance_mask(mask)
+        ctx.mc_solve_raw(rhs, 1)
+        g2 = ctx.importance()
+        assert np.all(g2[::2] == 1.0) and np.allclose(g2[1::2], g[1::2])
+
+
+def test_targets_from_tally():
+    prob = _varcoef()
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "CDF", "Walk Precision": "FP64", "History Length": 50,
+                        "Number of Histories": 8000, "Source Sampling Type": "Random", "Random Seed": 2})
+        A = po.Csr(prob.rp, prob.col, prob.values())
+        rhs = prob.b * po.jacobi_dinv(A)
+        T = ctx.importance_targets(rhs, 9)
+        x = ctx.mc_solve_raw(rhs, 9)
+    r = np.clip(np.abs(x) / np.abs(x).mean(), np.exp(-4), np.exp(4))
+    assert np.allclose(T, np.log(r), atol=1e-5)

@@ -65,6 +65,34 @@ def test_estimator_expectation(est, source, sampler, prec):
     assert np.max(np.abs(mean - wrong) / np.maximum(se, 1e-300)) > 8
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("est", [po.EST_COLLISION, po.EST_EXPECTED_VALUE])
+def test_learned_proposal_is_unbiased(prec, est):
+    # P_ij ∝ |h_ij| g_j with weight factor S_i / g_j keeps E[x] for any g > 0
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv)
+    H = np.eye(n) - dinv[:, None] * A
+    L = 6
+    terms = L + 1 if est == po.EST_EXPECTED_VALUE else L
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(terms))
+    g = np.exp(np.random.default_rng(4).normal(size=n)).astype(np.float32)
+    mp = po.mc_params(3000, L, 0.0, prec, est, po.SRC_STRATIFIED, seed=11, importance=g)
+    mean, se = _replicas(dom, b, mp, 60)
+    _check_z(mean, se, exact)
+    # the proposal tables are a distribution and the factors invert it
+    lcdf, lfac = po.build_proposal(dom, g)
+    for i in range(n):
+        s, e = dom.rp[i], dom.rp[i + 1]
+        if e == s:
+            continue
+        assert lcdf[e - 1] == 1.0
+        p = np.diff(np.concatenate([[0.0], lcdf[s:e]]))
+        assert np.allclose(p * lfac[s:e], np.abs(dom.h[s:e]), rtol=1e-12)
+
+
 def test_reference_semantics_unbiased():
     A, b = _system(10, seed=8)
     Ac = po.Csr.from_dense(A)
