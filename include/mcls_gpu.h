@@ -164,6 +164,13 @@ int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
  * estimator unbiased for any g.  g = 1 reproduces the plain transition
  * probabilities.  The tables are rebuilt on the device. */
 int mcls_gpu_set_importance(mcls_gpu_context* ctx, const float* g, int ptr_kind);
+/* Importance g_j = |a_jj| computed on the device.  For symmetric A under
+ * point Jacobi (H = D^-1 T) it makes P_ij proportional to |t_ij|, the
+ * symmetric random walk on the couplings, and the weight after k steps
+ * telescopes to S_0 / |a_kk| times the row ratios
+ * sum_j |t_ij| / |a_ii| of the states passed, each at most one for a
+ * diagonally dominant row. */
+int mcls_gpu_set_importance_diagonal(mcls_gpu_context* ctx);
 /* Current importance g (n floats): the one set, or the one the attached
  * network computed for the last MC source. */
 int mcls_gpu_get_importance(mcls_gpu_context* ctx, float* g, int ptr_kind);
@@ -186,8 +193,12 @@ int mcls_gpu_set_importance_mask(mcls_gpu_context* ctx, const signed char* mask,
  *   f7      0 */
 int mcls_gpu_importance_features(mcls_gpu_context* ctx, const double* v, float* X, int ptr_kind);
 /* Training targets from the solver's own adjoint tally: one MC solve of
- * source v (number `solve_index`, plain transitions), then
- * t_j = log(clip(|x_j| / mean |x|, e^-4, e^4)) (n floats). */
+ * source v (number `solve_index`, plain transitions) giving x, then
+ * t_j = log(clip(|a_jj x_j| / mean_i |a_ii x_i|, e^-4, e^4)) (n floats).
+ * For symmetric A with H = I - D^-1 A >= 0 and v >= 0, phi = D x solves
+ * phi = D v + H^T phi: it is the adjoint walk's value function for the
+ * functional <D v, .>, the importance under which the weight factors
+ * S_i / g_j = 1 - (D v)_i / phi_i stay below one along a walk. */
 int mcls_gpu_importance_targets(mcls_gpu_context* ctx, const double* v, uint64_t solve_index, float* T,
                                 int ptr_kind);
 

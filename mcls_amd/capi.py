@@ -36,7 +36,8 @@ GPU_SYMBOLS = [
     "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
     "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
     "mcls_gpu_mc_solve_raw", "mcls_gpu_set_systems", "mcls_gpu_get_system_residuals",
-    "mcls_gpu_set_importance", "mcls_gpu_get_importance", "mcls_gpu_set_importance_model",
+    "mcls_gpu_set_importance", "mcls_gpu_set_importance_diagonal", "mcls_gpu_get_importance",
+    "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
 ]
 MLP_SYMBOLS = [
@@ -130,6 +131,7 @@ def gpu_lib():
     L.mcls_gpu_set_systems.argtypes = [vp, i64, vp]
     L.mcls_gpu_get_system_residuals.argtypes = [vp, vp]
     L.mcls_gpu_set_importance.argtypes = [vp, vp, i32]
+    L.mcls_gpu_set_importance_diagonal.argtypes = [vp]
     L.mcls_gpu_get_importance.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_importance_model.argtypes = [vp, vp]
     L.mcls_gpu_set_importance_mask.argtypes = [vp, vp, i32]
@@ -375,6 +377,9 @@ class Context:
         g = np.empty(self.n, np.float32)
         self._check(self.L.mcls_gpu_get_importance(self.h, _p(g), MCLS_HOST))
         return g
+
+    def set_importance_diagonal(self):
+        self._check(self.L.mcls_gpu_set_importance_diagonal(self.h))
 
     def set_importance_model(self, mlp):
         self._model = mlp

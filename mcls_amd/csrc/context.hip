@@ -1177,6 +1177,19 @@ int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
   return MCLS_OK;
 }
 
+int mcls_gpu_set_importance_diagonal(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  MCLS_TRY(c->g_imp.ensure((size_t)c->n, "importance"));
+  HIP_TRY(mcls::launch_diag_abs(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->g_imp.p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->has_importance = true;
+  c->proposal_valid = false;
+  invalidate_graph(c);
+  if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
+  return MCLS_OK;
+}
+
 int mcls_gpu_get_importance(mcls_gpu_context* c, float* g, int ptr_kind) {
   MCLS_TRY(check_ctx(c));
   MCLS_TRY(check_kind(ptr_kind));
@@ -1242,9 +1255,13 @@ int mcls_gpu_importance_targets(mcls_gpu_context* c, const double* v, uint64_t s
   c->prm.sampler = sampler;
   MCLS_TRY(rc);
   MCLS_TRY(c->mlp_out.ensure((size_t)c->n, "targets"));
-  HIP_TRY(mcls::launch_feature_sums(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->x.p, c->part.p,
+  // y = diag(A) x: for symmetric A under point Jacobi the adjoint tally
+  // approximates the right Perron vector v of |H|, and D v is the left one,
+  // the importance that keeps walk weights from growing
+  HIP_TRY(mcls::launch_scale_by_diag(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->x.p, c->r.p, c->stream));
+  HIP_TRY(mcls::launch_feature_sums(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->r.p, c->part.p,
                                     c->scal.p + 4, c->stream));
-  HIP_TRY(mcls::launch_targets(c->n, c->x.p, c->scal.p + 4, c->mlp_out.p, c->stream));
+  HIP_TRY(mcls::launch_targets(c->n, c->r.p, c->scal.p + 4, c->mlp_out.p, c->stream));
   HIP_TRY(hipMemcpyAsync(T, c->mlp_out.p, sizeof(float) * (size_t)c->n, kind_from_dev(ptr_kind), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;

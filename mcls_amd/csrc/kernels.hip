@@ -759,6 +759,32 @@ __global__ void k_features(int n, const int* __restrict__ rp, const int* __restr
   }
 }
 
+// y_j = a_jj x_j
+template <typename AV>
+__global__ void k_scale_by_diag(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                                const double* __restrict__ x, double* __restrict__ y) {
+  const AV* __restrict__ a = (const AV*)val;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int k = rp[j]; k < rp[j + 1]; ++k)
+      if (col[k] == j) d = (double)a[k];
+    y[j] = d * x[j];
+  }
+}
+
+// g_j = |a_jj| (0 when the row has no diagonal, which the proposal builder flags)
+template <typename AV>
+__global__ void k_diag_abs(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                           float* __restrict__ g) {
+  const AV* __restrict__ a = (const AV*)val;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int k = rp[j]; k < rp[j + 1]; ++k)
+      if (col[k] == j) d = (double)a[k];
+    g[j] = (float)fabs(d);
+  }
+}
+
 // g_j = exp(clip(o_j, -4, 4)); states switched off by the mask use g = 1.
 __global__ void k_importance(int n, const float* __restrict__ o, const signed char* __restrict__ mask,
                              float* __restrict__ g) {
@@ -1007,6 +1033,24 @@ hipError_t launch_features(int n, const int* rp, const int* col, const void* val
     k_features<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, v, sums, X);
   else
     k_features<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, v, sums, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_by_diag(int n, const int* rp, const int* col, const void* val, bool f32, const double* x,
+                                double* y, hipStream_t s) {
+  if (f32)
+    k_scale_by_diag<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, x, y);
+  else
+    k_scale_by_diag<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_abs(int n, const int* rp, const int* col, const void* val, bool f32, float* g,
+                           hipStream_t s) {
+  if (f32)
+    k_diag_abs<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, g);
+  else
+    k_diag_abs<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, g);
   return hipGetLastError();
 }
 

@@ -97,6 +97,12 @@ hipError_t launch_feature_sums(int n, const int* rp, const int* col, const void*
                                double* part, double* sums, hipStream_t s);
 hipError_t launch_features(int n, const int* rp, const int* col, const void* val, bool f32, const double* v,
                            const double* sums, float* X, hipStream_t s);
+// y_j = a_jj x_j
+hipError_t launch_scale_by_diag(int n, const int* rp, const int* col, const void* val, bool f32, const double* x,
+                                double* y, hipStream_t s);
+// g_j = |a_jj| as float
+hipError_t launch_diag_abs(int n, const int* rp, const int* col, const void* val, bool f32, float* g,
+                           hipStream_t s);
 // g = exp(clip(o, -4, 4)), 1 where mask (nullable) is 0
 hipError_t launch_importance(int n, const float* o, const signed char* mask, float* g, hipStream_t s);
 // T = log(clip(|x| / mean|x|, e^-4, e^4)), mean from sums[1] / n

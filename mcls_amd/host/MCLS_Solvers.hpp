@@ -19,6 +19,8 @@
 
 #include <cstring>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "MCLS_Core.hpp"
 #include "MCLS_Traits.hpp"
@@ -159,6 +161,7 @@ class DeviceProblem {
                                                 A.colInd().data(), val, dt, MCLS_HOST));
       d_op = p.getOperator().get();
       d_pl = nullptr;
+      d_imp_dirty = true;  // a new operator drops the context's importance
     }
     if (p.isLeftPrec()) {
       if (d_pl != p.getLeftPrec().get()) {
@@ -172,7 +175,28 @@ class DeviceProblem {
       d_pl = nullptr;
     }
     d_ctx->setParameters(pl);
+    if (d_imp_dirty && d_imp_mode != ImportanceMode::None) {
+      if (d_imp_mode == ImportanceMode::Diagonal) {
+        GpuContext::check(mcls_gpu_set_importance_diagonal(d_ctx->get()));
+      } else {
+        MCLS_INSIST((int64_t)d_imp.size() == A.numRows(), "importance has the wrong length");
+        GpuContext::check(mcls_gpu_set_importance(d_ctx->get(), d_imp.data(), MCLS_HOST));
+      }
+    }
+    d_imp_dirty = false;
   }
+  // Importance of the learned proposal ("Transition Sampler" = "Learned"),
+  // applied at the next solve and kept across operator changes.
+  void setImportanceDiagonal() {
+    d_imp_mode = ImportanceMode::Diagonal;
+    d_imp_dirty = true;
+  }
+  void setImportance(std::vector<float> g) {
+    d_imp = std::move(g);
+    d_imp_mode = ImportanceMode::Explicit;
+    d_imp_dirty = true;
+  }
+  void setImportanceModel(mcls_mlp* m) { GpuContext::check(mcls_gpu_set_importance_model(d_ctx->get(), m)); }
   mcls_gpu_context* ctx() const { return d_ctx->get(); }
   mcls_gpu_stats stats() const {
     mcls_gpu_stats s;
@@ -181,9 +205,13 @@ class DeviceProblem {
   }
 
  private:
+  enum class ImportanceMode { None, Diagonal, Explicit };
   RCP<GpuContext> d_ctx;
   const void* d_op = nullptr;
   const void* d_pl = nullptr;
+  ImportanceMode d_imp_mode = ImportanceMode::None;
+  std::vector<float> d_imp;
+  bool d_imp_dirty = false;
 };
 
 inline RCP<ParameterList> mcValidParameters() {
@@ -246,6 +274,12 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
   }
   bool getConvergedStatus() const override { return true; }
   mcls_gpu_stats deviceStats() const { return d_dev.stats(); }
+  // Learned-proposal importance (no counterpart in the reference): the
+  // diagonal rule g = |a_jj|, an explicit g, or a network (include/mcls_mlp.h)
+  // evaluated on every MC source.  Used with "Transition Sampler" = "Learned".
+  void setImportanceDiagonal() { d_dev.setImportanceDiagonal(); }
+  void setImportance(std::vector<float> g) { d_dev.setImportance(std::move(g)); }
+  void setImportanceModel(mcls_mlp* m) { d_dev.setImportanceModel(m); }
 
  private:
   RCP<Problem> d_problem;
@@ -308,6 +342,12 @@ class MCSASolverManager : public SolverManager<Vector, Matrix> {
   }
   bool getConvergedStatus() const override { return d_stats.converged != 0; }
   mcls_gpu_stats deviceStats() const { return d_stats; }
+  // Learned-proposal importance (no counterpart in the reference): the
+  // diagonal rule g = |a_jj|, an explicit g, or a network (include/mcls_mlp.h)
+  // evaluated on every MC source.  Used with "Transition Sampler" = "Learned".
+  void setImportanceDiagonal() { d_dev.setImportanceDiagonal(); }
+  void setImportance(std::vector<float> g) { d_dev.setImportance(std::move(g)); }
+  void setImportanceModel(mcls_mlp* m) { d_dev.setImportanceModel(m); }
 
  private:
   RCP<Problem> d_problem;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "MCLS_Solvers.hpp"
 
@@ -157,11 +158,80 @@ static void test_factory_and_errors() {
   std::printf("ok SolverFactory names and error convention\n");
 }
 
+// 2D 5-point Laplacian with a varying coefficient per row; the diagonal
+// importance through the facade, kept across a change of operator.
+static RCP<MatrixType> build_varcoef(int m, double scale) {
+  auto A = std::make_shared<MatrixType>((int64_t)m * m);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) {
+      const int r = j * m + i;
+      const double k = scale * (1.0 + 0.5 * ((i * 7 + j * 3) % 5));
+      std::vector<int64_t> c;
+      std::vector<double> v;
+      if (j > 0) { c.push_back(r - m); v.push_back(-k); }
+      if (i > 0) { c.push_back(r - 1); v.push_back(-k); }
+      c.push_back(r); v.push_back(4.0 * k + 0.05);
+      if (i < m - 1) { c.push_back(r + 1); v.push_back(-k); }
+      if (j < m - 1) { c.push_back(r + m); v.push_back(-k); }
+      A->insertGlobalValues(r, c, v);
+    }
+  A->fillComplete();
+  return A;
+}
+
+static void test_mcsa_importance() {
+  const int m = 16;
+  auto A = build_varcoef(m, 1.0);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*x, 0.0);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, 1.0);
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-8);
+  plist->set<int>("Maximum Iterations", 60);
+  plist->set<double>("Sample Ratio", 64.0);
+  plist->set<double>("History Length", 100000);
+  plist->set<double>("Weight Cutoff", 1e-4);
+  plist->set<double>("Richardson Relaxation", 0.7);
+  plist->set<int>("Smoother Steps", 3);
+  plist->set<std::string>("Transition Sampler", "Learned");
+  plist->set<int>("Random Seed", 3);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  PointJacobiPreconditioner<double> pj;
+  pj.setOperator(A);
+  pj.buildPreconditioner();
+  problem->setLeftPrec(pj.getLeftPreconditioner());
+  MCSASolverManager<VectorType, MatrixType, AdjointTag> solver(problem, plist);
+  bool threw = false;
+  try {
+    solver.solve();  // "Learned" without an importance is a state error
+  } catch (const Assertion&) {
+    threw = true;
+  }
+  CHECK(threw);
+  solver.setImportanceDiagonal();
+  VT::putScalar(*x, 0.0);
+  CHECK(solver.solve() && solver.achievedTol() <= 1e-8);
+  CHECK(all_sign(*x, 1.0));
+  const int first_iters = solver.getNumIters();
+  // a new operator: the importance is recomputed for it
+  auto A2 = build_varcoef(m, 2.0);
+  problem->setOperator(A2);
+  pj.setOperator(A2);
+  pj.buildPreconditioner();
+  problem->setLeftPrec(pj.getLeftPreconditioner());
+  VT::putScalar(*x, 0.0);
+  CHECK(solver.solve() && solver.achievedTol() <= 1e-8);
+  CHECK(all_sign(*x, 1.0));
+  std::printf("ok MCSASolverManager.importance iters=%d,%d\n", first_iters, solver.getNumIters());
+}
+
 int main() {
   test_mcsa_adjoint(false);
   test_mcsa_adjoint(true);
   test_mc_adjoint();
   test_factory_and_errors();
+  test_mcsa_importance();
   std::printf("ALL %d checks passed\n", g_checks);
   return 0;
 }
