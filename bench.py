@@ -82,12 +82,13 @@ def bytes_per_step(prec, z, sampler):
 
     CDF:   16 (packed row header: start, len, W) + p z (CDF row) + 4 (packed
            column) + 2p (tally read-modify-write)
-    Alias: 16 (row header) + 16 (slot record) + 2p (tally)
+    Alias: FP32: 32 (slot record carrying both candidates' row headers) + 2p;
+           FP64: 16 (row header) + 16 (slot record) + 2p
     p = 4 (FP32 walk) or 8 (FP64 walk); z = mean domain row length.
     """
     p = 4 if prec == "FP32" else 8
     if sampler == "Alias":
-        return 16 + 16 + 2 * p
+        return 32 + 2 * p if p == 4 else 16 + 16 + 2 * p
     return 16 + p * z + 4 + 2 * p
 
 
@@ -128,6 +129,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 reference solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--form", default="chosen", choices=sorted(FORMS))
+    ap.add_argument("--no-tune", action="store_true", help="skip the walk autotuner (default launch shape)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -164,6 +166,10 @@ def main():
     st = ctx.stats()
     log(f"domain built: nnz={st.domain_nnz} in {time.perf_counter() - t0:.2f}s (kernel {st.ms_domain:.1f} ms)")
     z = st.domain_nnz / prob.n
+    tuning = None
+    if not args.no_tune:
+        tuning = ctx.autotune(prob.b)
+        log(f"autotune: {tuning}")
 
     for i in range(args.warmup):
         dt, st = solve_once(ctx, prob.b)
@@ -213,9 +219,15 @@ def main():
                      "kernel": f"k_walk<{'float' if prec == 'FP32' else 'double'}>",
                      "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},
+        "tuning": tuning,
     }
     if not args.no_plain and args.form == "chosen":
         ctx.set_params(FORMS["plain"])
+        ptune = None
+        if not args.no_tune:
+            ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64})
+            ptune = ctx.autotune(prob.b)
+            log(f"autotune (plain): {ptune}")
         dt, st = solve_once(ctx, prob.b)
         log(f"plain FP64: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
         out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
@@ -223,7 +235,7 @@ def main():
                         "walk_steps_per_s": st.steps / (st.ms_walk / 1e3),
                         "roofline_frac": bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
                         * st.steps / (st.ms_walk / 1e3) / 1e9
-                        / HBM_PEAK_GBS}
+                        / HBM_PEAK_GBS, "tuning": ptune}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
     ctx.close()
     if not args.no_cpu:

@@ -114,12 +114,15 @@ int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kin
  * Build-only keys:
  *   i64 "Number of Histories"  (0 = N * Sample Ratio)
  *   str "Walk Precision"       ("FP64" | "FP32")
- *   str "Transition Sampler"   ("CDF" | "Alias"): row CDF inversion (the
- *        reference's lower_bound) or Vose alias tables (rows <= 64 entries)
+ *   str "Transition Sampler"   ("CDF" | "Alias" | "Learned"): row CDF
+ *        inversion (the reference's lower_bound), Vose alias tables (rows
+ *        <= 64 entries) or the importance-reshaped rows (learned proposal)
  *   i64 "Random Seed"          (0)
  *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
  *   i64 "Verbosity"            (0)
- *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto)
+ *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto),
+ *   i64 "Walk Batch"           (64) history ids per work-queue grab
+ *        (these three are what mcls_gpu_autotune sets)
  * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
 int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
 int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);
@@ -154,6 +157,28 @@ int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            const double* x, double* r, double* norm2,
                            int ptr_kind);
 int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
+
+/* ----------------------------------------------------------- autotuning -- */
+/* Walk launch shape picked by timing (no counterpart in the reference, whose
+ * transport loop runs on CPU ranks).  Pilot MC solves of the source PL b with
+ * `pilot_histories` histories (0: 1/8 of the configured count, at least
+ * 2^20) time the block size (64, 128, 256) at 8, 16 and 32 resident waves
+ * per CU, then the history ids a wave takes per queue grab (64 .. 512) at
+ * the fastest shape.  The winner is stored in "Walk Block Size", "Walk
+ * Blocks Per CU" and "Walk Batch" and reported in `out`.  Estimates do not
+ * depend on the launch shape (each history's random stream depends only on
+ * its id); the order of the floating-point tally additions does.  The solve
+ * counter is not advanced. */
+typedef struct mcls_gpu_tuning {
+  int64_t block;          /* Walk Block Size                                 */
+  int64_t blocks_per_cu;  /* Walk Blocks Per CU                              */
+  int64_t batch;          /* Walk Batch                                      */
+  double ms_default;      /* pilot walk time with the configuration on entry */
+  double ms_best;         /* pilot walk time with the chosen configuration   */
+  int64_t candidates;     /* configurations timed                            */
+} mcls_gpu_tuning;
+int mcls_gpu_autotune(mcls_gpu_context* ctx, const double* b, int ptr_kind, int64_t pilot_histories,
+                      mcls_gpu_tuning* out);
 
 /* ------------------------------------------------------ learned proposal -- */
 /* Importance g (n FP32 values > 0) for "Transition Sampler" = "Learned"

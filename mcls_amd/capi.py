@@ -36,6 +36,7 @@ GPU_SYMBOLS = [
     "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
     "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
     "mcls_gpu_mc_solve_raw", "mcls_gpu_set_systems", "mcls_gpu_get_system_residuals",
+    "mcls_gpu_autotune",
     "mcls_gpu_set_importance", "mcls_gpu_set_importance_diagonal", "mcls_gpu_get_importance",
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
@@ -75,6 +76,20 @@ class Stats(C.Structure):
         ("domain_nnz", C.c_int64),
         ("solve_index", C.c_int64),
         ("last_walk_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("block", C.c_int64),
+        ("blocks_per_cu", C.c_int64),
+        ("batch", C.c_int64),
+        ("ms_default", C.c_double),
+        ("ms_best", C.c_double),
+        ("candidates", C.c_int64),
     ]
 
     def as_dict(self):
@@ -132,6 +147,7 @@ def gpu_lib():
     L.mcls_gpu_get_system_residuals.argtypes = [vp, vp]
     L.mcls_gpu_set_importance.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_importance_diagonal.argtypes = [vp]
+    L.mcls_gpu_autotune.argtypes = [vp, vp, i32, C.c_int64, C.POINTER(Tuning)]
     L.mcls_gpu_get_importance.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_importance_model.argtypes = [vp, vp]
     L.mcls_gpu_set_importance_mask.argtypes = [vp, vp, i32]
@@ -377,6 +393,12 @@ class Context:
         g = np.empty(self.n, np.float32)
         self._check(self.L.mcls_gpu_get_importance(self.h, _p(g), MCLS_HOST))
         return g
+
+    def autotune(self, b, pilot_histories=0):
+        b = np.ascontiguousarray(b, np.float64)
+        t = Tuning()
+        self._check(self.L.mcls_gpu_autotune(self.h, _p(b), MCLS_HOST, int(pilot_histories), C.byref(t)))
+        return t.as_dict()
 
     def set_importance_diagonal(self):
         self._check(self.L.mcls_gpu_set_importance_diagonal(self.h))

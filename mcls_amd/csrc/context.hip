@@ -89,6 +89,7 @@ struct Params {
   int64_t verbosity = 0;
   int64_t walk_block = 256;
   int64_t walk_blocks_per_cu = 0;
+  int64_t walk_batch = 64;
   int sampler = mcls::SAMPLER_CDF;
 };
 
@@ -124,8 +125,12 @@ struct mcls_gpu_context {
   DBuf<uint32_t> d_col;
   DBuf<double> d_cdf64, d_h64, d_W64;
   DBuf<float> d_cdf32, d_h32, d_W32;
-  DBuf<uint4> d_hdr32, d_hdr64, d_slot32, d_slot64;
+  DBuf<uint4> d_hdr32, d_hdr64, d_slot64;
+  DBuf<mcls::Rec32> d_rec32;
+  DBuf<uint4> d_tgt32;
+  DBuf<mcls::Rec32> d_tgt64;
   bool alias_valid = false;
+  bool tgt32_valid = false, tgt64_valid = false;
   // learned proposal: importance g and its transition tables
   DBuf<float> g_imp;
   bool has_importance = false, proposal_valid = false;
@@ -318,6 +323,8 @@ int build_domain(mcls_gpu_context* c) {
                                  c->d_hdr64.p, nullptr, nullptr, false, c->flags.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->alias_valid = false;
+  c->tgt32_valid = false;
+  c->tgt64_valid = false;
   c->proposal_valid = false;
   c->domain_valid = true;
   c->domain_omega = omega;
@@ -329,11 +336,11 @@ int build_domain(mcls_gpu_context* c) {
 int ensure_alias(mcls_gpu_context* c) {
   if (c->prm.sampler != mcls::SAMPLER_ALIAS || c->alias_valid) return MCLS_OK;
   const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
-  MCLS_TRY(c->d_slot32.ensure(m, "alias slots 32"));
+  MCLS_TRY(c->d_rec32.ensure(m, "alias records 32"));
   MCLS_TRY(c->d_slot64.ensure(m, "alias slots 64"));
   MCLS_TRY(reset_flags(c));
   HIP_TRY(mcls::launch_pack_rows(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, c->d_hdr32.p,
-                                 c->d_hdr64.p, c->d_slot32.p, c->d_slot64.p, true, c->flags.p, c->stream));
+                                 c->d_hdr64.p, c->d_rec32.p, c->d_slot64.p, true, c->flags.p, c->stream));
   int flags[mcls::FLAG_COUNT];
   MCLS_TRY(read_flags(c, flags));
   if (flags[mcls::FLAG_LONG_ROW] != INT_MAX)
@@ -398,10 +405,30 @@ int ensure_proposal(mcls_gpu_context* c) {
   return MCLS_OK;
 }
 
+// Target records of the walk precision in use, for the CDF and learned
+// samplers (alias walks read their own records).
+int ensure_targets(mcls_gpu_context* c) {
+  if (c->prm.sampler == mcls::SAMPLER_ALIAS) return MCLS_OK;
+  const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
+  if (c->prm.precision == 32 && !c->tgt32_valid) {
+    MCLS_TRY(c->d_tgt32.ensure(m, "target records 32"));
+    HIP_TRY(mcls::launch_pack_targets(c->d_nnz, c->d_rp.p, c->d_col.p, c->d_W64.p, c->d_W32.p, c->d_tgt32.p,
+                                      nullptr, c->stream));
+    c->tgt32_valid = true;
+  } else if (c->prm.precision == 64 && !c->tgt64_valid) {
+    MCLS_TRY(c->d_tgt64.ensure(m, "target records 64"));
+    HIP_TRY(mcls::launch_pack_targets(c->d_nnz, c->d_rp.p, c->d_col.p, c->d_W64.p, c->d_W32.p, nullptr,
+                                      c->d_tgt64.p, c->stream));
+    c->tgt64_valid = true;
+  }
+  return MCLS_OK;
+}
+
 int ensure_domain_impl(mcls_gpu_context* c);
 int ensure_domain(mcls_gpu_context* c) {
   MCLS_TRY(ensure_domain_impl(c));
   MCLS_TRY(ensure_alias(c));
+  MCLS_TRY(ensure_targets(c));
   return ensure_proposal(c);
 }
 
@@ -449,8 +476,10 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.h32 = c->d_h32.p;
   a.d.hdr32 = c->d_hdr32.p;
   a.d.hdr64 = c->d_hdr64.p;
-  a.d.slot32 = c->d_slot32.p;
+  a.d.rec32 = c->d_rec32.p;
   a.d.slot64 = c->d_slot64.p;
+  a.d.tgt32 = c->d_tgt32.p;
+  a.d.tgt64 = c->d_tgt64.p;
   a.d.lcdf64 = c->d_lcdf64.p;
   a.d.lcdf32 = c->d_lcdf32.p;
   a.d.lfac64 = c->d_lfac64.p;
@@ -468,6 +497,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.tally64 = c->prm.precision == 32 ? nullptr : c->t64.p;
   a.tally32 = c->prm.precision == 32 ? c->t32.p : nullptr;
   a.steps = c->steps.p;
+  a.batch = (int)c->prm.walk_batch;
   return a;
 }
 
@@ -798,6 +828,10 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
   } else if (key_is(key, "Walk Blocks Per CU")) {
     if (v < 0 || v > 64) return fail(MCLS_ERR_INVALID_ARG, "Walk Blocks Per CU must be in [0, 64]");
     p.walk_blocks_per_cu = v;
+  } else if (key_is(key, "Walk Batch")) {
+    if (v < 64 || v > 4096 || v % 64 != 0)
+      return fail(MCLS_ERR_INVALID_ARG, "Walk Batch must be a multiple of 64 in [64, 4096]");
+    p.walk_batch = v;
   } else if (key_is(key, "MC Check Frequency") || key_is(key, "MC Buffer Size") ||
              key_is(key, "Overlap Size") || key_is(key, "Number of Sets")) {
     // accepted; single domain, single set
@@ -1174,6 +1208,85 @@ int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
   c->proposal_valid = false;
   invalidate_graph(c);
   if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
+  return MCLS_OK;
+}
+
+int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_t pilot, mcls_gpu_tuning* out) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!b || !out) return fail(MCLS_ERR_INVALID_ARG, "null vector or output");
+  if (pilot < 0) return fail(MCLS_ERR_INVALID_ARG, "pilot history count must be >= 0");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(c->b.p, b, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), s));
+  HIP_TRY(mcls::launch_fill_f64(c->x.p, c->n, 0.0, s));
+  MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));  // pilot source PL b
+  const Params saved = c->prm;
+  const int64_t saved_nsys = c->nsys;
+  const int64_t nh = num_histories(c);
+  int64_t np = pilot > 0 ? pilot : std::max<int64_t>(nh / 8, (int64_t)1 << 20);
+  if (nh > 0 && np > nh) np = nh;
+  c->prm.num_histories = np;
+  c->nsys = 0;  // pilot births follow |PL b| over the whole operator
+  auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, double* ms) -> int {
+    c->prm.walk_block = blk;
+    c->prm.walk_blocks_per_cu = bpc;
+    c->prm.walk_batch = bat;
+    MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true));
+    HIP_TRY(hipStreamSynchronize(s));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, c->ev[3], c->ev[4]));
+    *ms = t;
+    return MCLS_OK;
+  };
+  int rc = MCLS_OK;
+  mcls_gpu_tuning best{};
+  best.block = saved.walk_block;
+  best.blocks_per_cu = saved.walk_blocks_per_cu > 0 ? saved.walk_blocks_per_cu : 8;
+  best.batch = saved.walk_batch;
+  double ms = 0.0;
+  // warm-up, then the configuration on entry
+  rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, &ms);
+  if (rc == MCLS_OK) rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, &ms);
+  best.ms_default = ms;
+  best.ms_best = ms;
+  best.candidates = 1;
+  // launch shape: 8, 16 or 32 resident waves per CU at each block size
+  const int64_t blocks[3] = {64, 128, 256};
+  const int64_t waves[3] = {8, 16, 32};
+  for (int i = 0; i < 3 && rc == MCLS_OK; ++i)
+    for (int j = 0; j < 3 && rc == MCLS_OK; ++j) {
+      const int64_t bpc = waves[j] * 64 / blocks[i];
+      rc = pilot_ms(blocks[i], bpc, best.batch, &ms);
+      ++best.candidates;
+      if (rc == MCLS_OK && ms < best.ms_best) {
+        best.ms_best = ms;
+        best.block = blocks[i];
+        best.blocks_per_cu = bpc;
+      }
+    }
+  // history ids per queue grab at the chosen shape
+  const int64_t batches[4] = {64, 128, 256, 512};
+  const int64_t shape_batch = best.batch;
+  for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
+    if (batches[k] == shape_batch) continue;
+    rc = pilot_ms(best.block, best.blocks_per_cu, batches[k], &ms);
+    ++best.candidates;
+    if (rc == MCLS_OK && ms < best.ms_best) {
+      best.ms_best = ms;
+      best.batch = batches[k];
+    }
+  }
+  c->prm = saved;
+  c->nsys = saved_nsys;
+  MCLS_TRY(rc);
+  c->prm.walk_block = best.block;
+  c->prm.walk_blocks_per_cu = best.blocks_per_cu;
+  c->prm.walk_batch = best.batch;
+  invalidate_graph(c);
+  *out = best;
   return MCLS_OK;
 }
 

@@ -116,6 +116,7 @@ template <> struct Tables<float> {
 template <typename Real> struct Row;
 template <> struct Row<double> {
   __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr64[i]; }
+  __device__ static uint32_t target(const DomainView& d, uint32_t k, uint32_t& beg, uint32_t& len, double& W);
   __device__ static double W(uint4 h) { return __hiloint2double((int)h.w, (int)h.z); }
   // alias draw: slot from the high 32 bits, keep-or-alias from the low 32
   __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r) {
@@ -128,26 +129,35 @@ template <> struct Row<double> {
 template <> struct Row<float> {
   __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr32[i]; }
   __device__ static float W(uint4 h) { return __uint_as_float(h.z); }
-  __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r) {
-    const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
-    const uint4 s = d.slot32[start + slot];
-    const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
-    return u2 < __uint_as_float(s.x) ? s.y : s.z;
+  // target record k: packed column, and that column's row header
+  __device__ static uint32_t target(const DomainView& d, uint32_t k, uint32_t& beg, uint32_t& len, float& W) {
+    const uint4 t = d.tgt32[k];
+    beg = t.y;
+    len = t.z;
+    W = __uint_as_float(t.w);
+    return t.x;
   }
 };
+__device__ uint32_t Row<double>::target(const DomainView& d, uint32_t k, uint32_t& beg, uint32_t& len, double& W) {
+  const Rec32 t = d.tgt64[k];
+  beg = t.a.y;
+  len = t.a.z;
+  W = __hiloint2double((int)t.b.y, (int)t.b.x);
+  return t.a.x;
+}
+
+
 
 // --------------------------------------------------------------------------
 // Walk kernel.  Each lane owns one live history at a time.  Lanes whose
 // history ended are refilled together from a per-XCD work queue: the wave
-// takes batches of kBatch consecutive history ids with one atomic, so the
+// takes batches of a.batch consecutive history ids with one atomic, so the
 // walks in flight on an XCD were born next to each other (the source emits
 // history ids in state order) and share L2 / MALL lines.  Histories follow
 // the reference loop order: tally at the current state, sample the
 // transition (row CDF inversion or alias slot), multiply the weight by
 // W_i sign(h_ij), stop when steps >= L or |w| < cutoff |w0|.
 // --------------------------------------------------------------------------
-constexpr int kBatch = 64;
-
 template <typename Real, int EST, int SRC, int SAMP, bool REC>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const DomainView& d = a.d;
@@ -177,11 +187,18 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   unsigned long long* ctr = a.queue + qs * kQueueStride;
   const int lane = __lane_id();
 
+  // The current row's header is carried from the record that chose the
+  // state (FP32 alias records, CDF / learned target records); FP64 alias
+  // walks load it per step.
+  constexpr bool FUSED = SAMP == SAMPLER_ALIAS && sizeof(Real) == 4;
+  constexpr bool CARRY = SAMP != SAMPLER_ALIAS || FUSED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
   bool exhausted = false;       // wave-uniform
   bool alive = false;
   int64_t hid = 0;
   int state = 0;
+  uint32_t beg = 0, len = 0;    // current domain row
+  Real Wrow = 0;                // its weight W_i
   Real w = 0, wc = 0;
   int64_t steps = 0;
   uint64_t rng = 0;
@@ -195,10 +212,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         if (qnext >= qend) {
           const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
           unsigned long long b = 0;
-          if (lane == leader) b = atomicAdd(ctr, (unsigned long long)kBatch);
+          if (lane == leader) b = atomicAdd(ctr, (unsigned long long)a.batch);
           b = __shfl(b, leader);
           qnext = sbeg + (int64_t)b;
-          qend = qnext + kBatch < send ? qnext + kBatch : send;
+          qend = qnext + a.batch < send ? qnext + a.batch : send;
           if (qnext >= send) {
             exhausted = true;
             break;
@@ -225,6 +242,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           wc = (Real)a.cutoff * (w < 0 ? -w : w);
           steps = 0;
           alive = true;
+          if (CARRY) {
+            const uint4 hd = Row<Real>::hdr(d, state);
+            beg = hd.x;
+            len = hd.y;
+            Wrow = Row<Real>::W(hd);
+          }
           if (REC) {
             path = a.paths + (size_t)(hid - a.first) * (size_t)(a.max_len + 1);
             path[0] = state;
@@ -236,8 +259,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     }
     if (!alive) break;  // queue exhausted and no history left on this lane
 
-    const uint4 hd = Row<Real>::hdr(d, state);
-    const uint32_t beg = hd.x, len = hd.y;
+    if (!CARRY) {
+      const uint4 hd = Row<Real>::hdr(d, state);
+      beg = hd.x;
+      len = hd.y;
+      Wrow = Row<Real>::W(hd);
+    }
     if (EST == EST_EXPECTED_VALUE) {
       for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], w * ht[beg + k]);
     } else {
@@ -247,19 +274,31 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if (!end) {
       const uint64_t r = xorshift64(rng);
       uint32_t cp;
-      if (SAMP == SAMPLER_LEARNED) {
+      if constexpr (FUSED) {
+        // alias draw: slot from the high 32 bits, keep-or-alias from the low
+        // 32; the record also holds both candidates' row headers
+        const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
+        const Rec32 sr = d.rec32[beg + slot];
+        const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
+        const bool keep = u2 < __uint_as_float(sr.a.x);
+        cp = keep ? sr.a.y : sr.a.z;
+        w = w * Wrow;
+        beg = keep ? sr.b.x : sr.b.y;
+        len = keep ? (sr.a.w & 0xFFFFu) : (sr.a.w >> 16);
+        Wrow = (Real)__uint_as_float(keep ? sr.b.z : sr.b.w);
+      } else if constexpr (SAMP == SAMPLER_LEARNED) {
         // importance-reshaped row: P = |h| g / S, weight factor S / g_j
-        const int k = lower_bound_cdf<Real>(Tables<Real>::lcdf(d) + beg, (int)len, Tables<Real>::u01(r));
-        cp = colp[beg + k];
-        w = w * Tables<Real>::lfac(d)[beg + k];
+        const uint32_t k = beg + (uint32_t)lower_bound_cdf<Real>(Tables<Real>::lcdf(d) + beg, (int)len,
+                                                                 Tables<Real>::u01(r));
+        w = w * Tables<Real>::lfac(d)[k];
+        cp = Row<Real>::target(d, k, beg, len, Wrow);
+      } else if constexpr (SAMP == SAMPLER_ALIAS) {
+        cp = Row<Real>::pick(d, beg, len, r);
+        w = w * Wrow;
       } else {
-        if (SAMP == SAMPLER_ALIAS) {
-          cp = Row<Real>::pick(d, beg, len, r);
-        } else {
-          const int k = lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
-          cp = colp[beg + k];
-        }
-        w = w * Row<Real>::W(hd);
+        const uint32_t k = beg + (uint32_t)lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
+        w = w * Wrow;
+        cp = Row<Real>::target(d, k, beg, len, Wrow);
       }
       if (cp & 0x80000000u) w = -w;
       state = (int)(cp & 0x7FFFFFFFu);
@@ -642,7 +681,7 @@ __global__ void k_domain_fill(int n, const int* __restrict__ t_rp, const int* __
 __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
                             const double* __restrict__ h64, const double* __restrict__ W64,
                             const float* __restrict__ W32, uint4* __restrict__ hdr32, uint4* __restrict__ hdr64,
-                            uint4* __restrict__ slot32, uint4* __restrict__ slot64, int alias, int* flags) {
+                            Rec32* __restrict__ rec32, uint4* __restrict__ slot64, int alias, int* flags) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int beg = d_rp[i];
     const int z = d_rp[i + 1] - beg;
@@ -684,8 +723,33 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
     }
     for (int k = 0; k < z; ++k) {
       const uint32_t self = col[beg + k], other = col[beg + al[k]];
-      slot32[beg + k] = make_uint4(__float_as_uint((float)thr[k]), self, other, 0u);
+      const int js = (int)(self & 0x7FFFFFFFu), jo = (int)(other & 0x7FFFFFFFu);
+      const int bs = d_rp[js], bo = d_rp[jo];
+      const uint32_t ls = (uint32_t)(d_rp[js + 1] - bs), lo = (uint32_t)(d_rp[jo + 1] - bo);
+      Rec32 r;
+      r.a = make_uint4(__float_as_uint((float)thr[k]), self, other, (ls & 0xFFFFu) | (lo << 16));
+      r.b = make_uint4((uint32_t)bs, (uint32_t)bo, __float_as_uint(W32[js]), __float_as_uint(W32[jo]));
+      rec32[beg + k] = r;
       slot64[beg + k] = make_uint4((uint32_t)__double2loint(thr[k]), (uint32_t)__double2hiint(thr[k]), self, other);
+    }
+  }
+}
+
+// Target records: for entry k, the packed column and that column's row header.
+__global__ void k_pack_targets(int64_t nnz, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                               const double* __restrict__ W64, const float* __restrict__ W32,
+                               uint4* __restrict__ tgt32, Rec32* __restrict__ tgt64) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = col[k];
+    const int j = (int)(c & 0x7FFFFFFFu);
+    const int b = d_rp[j];
+    const uint32_t l = (uint32_t)(d_rp[j + 1] - b);
+    if (tgt32) tgt32[k] = make_uint4(c, (uint32_t)b, l, __float_as_uint(W32[j]));
+    if (tgt64) {
+      Rec32 r;
+      r.a = make_uint4(c, (uint32_t)b, l, 0u);
+      r.b = make_uint4((uint32_t)__double2loint(W64[j]), (uint32_t)__double2hiint(W64[j]), 0u, 0u);
+      tgt64[k] = r;
     }
   }
 }
@@ -1071,10 +1135,17 @@ hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, co
   return hipGetLastError();
 }
 
+hipError_t launch_pack_targets(int64_t nnz, const int* d_rp, const uint32_t* col, const double* W64,
+                               const float* W32, uint4* tgt32, Rec32* tgt64, hipStream_t s) {
+  if (nnz <= 0) return hipSuccess;
+  k_pack_targets<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, d_rp, col, W64, W32, tgt32, tgt64);
+  return hipGetLastError();
+}
+
 hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
-                            const float* W32, uint4* hdr32, uint4* hdr64, uint4* slot32, uint4* slot64,
+                            const float* W32, uint4* hdr32, uint4* hdr64, Rec32* rec32, uint4* slot64,
                             bool alias, int* flags, hipStream_t s) {
-  k_pack_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, hdr32, hdr64, slot32, slot64,
+  k_pack_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, hdr32, hdr64, rec32, slot64,
                                                 alias ? 1 : 0, flags);
   return hipGetLastError();
 }

@@ -23,10 +23,24 @@ enum DeviceFlag : int {
 // Packed per-row headers (one 16-byte load per step):
 //   hdr32[i] = {start, len, bits(W32_i), 0}
 //   hdr64[i] = {start, len, lo(W64_i), hi(W64_i)}
-// Alias slot records (one 16-byte load per step), packed columns carry the
-// sign of h in bit 31:
-//   slot32[k] = {bits((float)thr_k), col_k, col_alias(k), 0}
-//   slot64[k] = {lo(thr_k), hi(thr_k), col_k, col_alias(k)}
+// Alias slot records, packed columns carry the sign of h in bit 31.
+//   slot64[k] = {lo(thr_k), hi(thr_k), col_k, col_alias(k)}   (16 bytes)
+// FP32 walks use 32-byte records that also carry the row header of both
+// possible next states, so a step reads one record instead of a record and
+// the next row's header:
+//   rec32[k].a = {bits((float)thr_k), col_k, col_alias(k), len(col_k) | len(col_alias(k)) << 16}
+//   rec32[k].b = {start(col_k), start(col_alias(k)), bits(W32(col_k)), bits(W32(col_alias(k)))}
+// Target records for the CDF and learned samplers: entry k of a row names
+// its column and that column's row header, read once the CDF search has
+// chosen k (the next step then needs no header load):
+//   tgt32[k]   = {col_k, start(col_k), len(col_k), bits(W32(col_k))}
+//   tgt64[k].a = {col_k, start(col_k), len(col_k), 0}
+//   tgt64[k].b = {lo(W64(col_k)), hi(W64(col_k)), 0, 0}
+struct __attribute__((aligned(32))) Rec32 {
+  uint4 a;
+  uint4 b;
+};
+
 struct DomainView {
   int n;
   const int* rp;          // n+1
@@ -39,8 +53,10 @@ struct DomainView {
   const float* h32;       // nnz
   const uint4* hdr32;     // n
   const uint4* hdr64;     // n
-  const uint4* slot32;    // nnz (alias sampler)
-  const uint4* slot64;    // nnz (alias sampler)
+  const Rec32* rec32;     // nnz (alias sampler, FP32 walks)
+  const uint4* slot64;    // nnz (alias sampler, FP64 walks)
+  const uint4* tgt32;     // nnz (CDF / learned samplers, FP32 walks)
+  const Rec32* tgt64;     // nnz (CDF / learned samplers, FP64 walks)
   // learned proposal: P_ij = |h_ij| g_j / S_i (CDF), weight factor S_i / g_j
   const double* lcdf64;   // nnz
   const float* lcdf32;    // nnz
@@ -76,6 +92,7 @@ struct WalkArgs {
   int* paths;              // recording: (last-first) x (max_len+1)
   double* wfinal;          // recording: final weight per history
   int max_len;
+  int batch;               // history ids a wave takes per queue grab (multiple of 64)
 };
 
 struct WalkConfig {
@@ -118,7 +135,11 @@ hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, co
 // Rows longer than 64 entries set FLAG_LONG_ROW when alias tables are built.
 hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64,
                             const double* W64, const float* W32, uint4* hdr32, uint4* hdr64,
-                            uint4* slot32, uint4* slot64, bool alias, int* flags, hipStream_t s);
+                            Rec32* rec32, uint4* slot64, bool alias, int* flags, hipStream_t s);
+
+// Target records (either output may be null).
+hipError_t launch_pack_targets(int64_t nnz, const int* d_rp, const uint32_t* col, const double* W64,
+                               const float* W32, uint4* tgt32, Rec32* tgt64, hipStream_t s);
 
 // ---------------------------------------------------------------- source
 // *key = splitmix64(seed ^ splitmix64(*counter)); ++*counter  (one thread)

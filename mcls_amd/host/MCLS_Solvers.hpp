@@ -348,6 +348,15 @@ class MCSASolverManager : public SolverManager<Vector, Matrix> {
   void setImportanceDiagonal() { d_dev.setImportanceDiagonal(); }
   void setImportance(std::vector<float> g) { d_dev.setImportance(std::move(g)); }
   void setImportanceModel(mcls_mlp* m) { d_dev.setImportanceModel(m); }
+  // Walk launch shape by timing pilot MC solves of PL b (mcls_gpu_autotune);
+  // kept by later solves unless the parameter list sets the walk keys.
+  mcls_gpu_tuning autotune(int64_t pilot_histories = 0) {
+    MCLS_INSIST(d_problem, "no problem set");
+    d_dev.sync(*d_problem, *d_plist);
+    mcls_gpu_tuning t{};
+    GpuContext::check(mcls_gpu_autotune(d_dev.ctx(), VT::view(*d_problem->getRHS()), MCLS_HOST, pilot_histories, &t));
+    return t;
+  }
 
  private:
   RCP<Problem> d_problem;
