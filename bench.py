@@ -92,6 +92,23 @@ def bytes_per_step(prec, z, sampler):
     return 16 + p * z + 4 + 2 * p
 
 
+def measured_traffic(workload, form, steps_per_launch):
+    """HBM bytes per launch of the dominant walk kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload and form
+    (calibrated against a copy kernel), when its transitions per launch match
+    this run's to 2%; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"walk_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("workload") != workload or rec.get("form") != form:
+        return None
+    if abs(rec["steps_per_launch"] - steps_per_launch) > 0.02 * steps_per_launch:
+        return None
+    return float(rec["traffic_bytes_per_launch"])
+
+
 def cpu_baseline(seconds_budget=20.0):
     """Oracle restatement of the MCLS algorithm on a 64^3 member of the family."""
     from oracle import pyoracle as po
@@ -215,7 +232,8 @@ def main():
         "steps_per_history": steps / max(hist, 1),
         "walk_steps_per_s": steps / (walk_ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, args.form, steps_per_launch),
                      "kernel": f"k_walk<{'float' if prec == 'FP32' else 'double'}>",
                      "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},
