@@ -109,32 +109,35 @@ def measured_traffic(workload, form, steps_per_launch):
     return float(rec["traffic_bytes_per_launch"])
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """Oracle restatement of the MCLS algorithm on a 64^3 member of the family."""
+def cpu_baseline(prob, w, plain_iterations=None, nh_sample=1 << 24):
+    """The oracle's restatement of the MCLS algorithm (reference semantics:
+    FP64 CDF lower_bound, one Xorshift stream per set, LIFO emission, sets
+    summed / numSets) on the same operator, one set per host thread, with a
+    bounded history sample: one MCSA iteration of nh_sample histories.
+    Histories/s from its MC transport time; per-iteration time and time to
+    tolerance at the workload's history count are extrapolated from it."""
     from oracle import pyoracle as po
 
-    w = WORKLOADS["cfg4-64"]
-    prob = make_problem(w)
-    A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64))
+    A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64) if w["f32"] else prob.values())
     dinv = po.jacobi_dinv(A)
     cores = min(16, os.cpu_count() or 1)
-    nh = w["histories"]
-    mp = po.mc_params(nh // cores, 1000, 1e-4, 64, seed=1)
-    # one MCSA iteration first to size the bounded sample
+    nh_sample = min(nh_sample, w["histories"])
+    mp = po.mc_params(nh_sample // cores, 1000, 1e-4, 64, seed=1)
     sp = po.solver_params(tol=1e-30, max_iters=1, num_sets=cores, reference_mode=1, mc=mp)
     x, res, _ = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
-    per_iter = max(res.seconds_mc, 1e-3)
-    iters = int(max(1, min(8, seconds_budget // per_iter)))
-    sp = po.solver_params(tol=1e-30, max_iters=iters, num_sets=cores, reference_mode=1, mc=mp)
-    x, res, _ = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
     hps = res.histories / res.seconds_mc
+    per_iter = w["histories"] / hps
+    iters = plain_iterations or 7
     return {"value": hps, "unit": "histories/s", "cores": cores, "kind": "port",
-            "sample": f"{iters} MCSA iterations of the MCLS-semantics CPU restatement "
-                      f"(FP64, CDF lower_bound, {cores} sets x {nh // cores} histories, "
-                      f"weight cutoff 1e-4) on the 64^3 27-point member of the family "
-                      f"(262,144 unknowns); MC transport time only",
-            "seconds_per_iteration": res.seconds_mc / res.iters,
-            "steps_per_history": res.steps / max(res.histories, 1)}
+            "sample": f"1 MCSA iteration of the MCLS-semantics CPU restatement (FP64, CDF lower_bound, "
+                      f"{cores} sets x {nh_sample // cores} histories, weight cutoff 1e-4) on the same "
+                      f"operator ({prob.n} unknowns); MC transport time only",
+            "seconds_mc": res.seconds_mc,
+            "steps_per_history": res.steps / max(res.histories, 1),
+            "seconds_per_iteration_extrapolated": per_iter,
+            "time_to_tol_extrapolated_s": iters * per_iter,
+            "extrapolation": f"{w['histories']} histories per iteration x {iters} iterations "
+                             f"(the plain form's count), MC transport only"}
 
 
 def main():
@@ -258,7 +261,7 @@ def main():
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(prob, w, out.get("plain", {}).get("mcsa_iterations"))
         log(f"cpu baseline {out['cpu_baseline']['value']:.3e} histories/s")
     print(json.dumps(out), flush=True)
 
