@@ -122,7 +122,10 @@ int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kin
  *   i64 "Verbosity"            (0)
  *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto),
  *   i64 "Walk Batch"           (64) history ids per work-queue grab
- *        (these three are what mcls_gpu_autotune sets)
+ *   i64 "Tally Tile"           (0) LDS tally tile entries per wave (0 = off,
+ *        else a power of two in [64, 2048]; collision estimator): tallies
+ *        merge in LDS and are flushed into an FP64 tally, also for FP32 walks
+ *        (these four are what mcls_gpu_autotune sets)
  * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
 int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
 int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);
@@ -164,8 +167,10 @@ int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
  * `pilot_histories` histories (0: 1/8 of the configured count, at least
  * 2^20) time the block size (64, 128, 256) at 8, 16 and 32 resident waves
  * per CU, then the history ids a wave takes per queue grab (64 .. 512) at
- * the fastest shape.  The winner is stored in "Walk Block Size", "Walk
- * Blocks Per CU" and "Walk Batch" and reported in `out`.  Estimates do not
+ * the fastest shape, then (collision estimator) the LDS tally tile: off,
+ * 128 or 512 entries per wave.  The winner is stored in "Walk Block Size",
+ * "Walk Blocks Per CU", "Walk Batch" and "Tally Tile" and reported in
+ * `out`.  Estimates do not
  * depend on the launch shape (each history's random stream depends only on
  * its id); the order of the floating-point tally additions does.  The solve
  * counter is not advanced. */
@@ -173,6 +178,7 @@ typedef struct mcls_gpu_tuning {
   int64_t block;          /* Walk Block Size                                 */
   int64_t blocks_per_cu;  /* Walk Blocks Per CU                              */
   int64_t batch;          /* Walk Batch                                      */
+  int64_t tally_tile;     /* Tally Tile                                      */
   double ms_default;      /* pilot walk time with the configuration on entry */
   double ms_best;         /* pilot walk time with the chosen configuration   */
   int64_t candidates;     /* configurations timed                            */

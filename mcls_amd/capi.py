@@ -87,6 +87,7 @@ class Tuning(C.Structure):
         ("block", C.c_int64),
         ("blocks_per_cu", C.c_int64),
         ("batch", C.c_int64),
+        ("tally_tile", C.c_int64),
         ("ms_default", C.c_double),
         ("ms_best", C.c_double),
         ("candidates", C.c_int64),

@@ -90,6 +90,7 @@ struct Params {
   int64_t walk_block = 256;
   int64_t walk_blocks_per_cu = 0;
   int64_t walk_batch = 64;
+  int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
   int sampler = mcls::SAMPLER_CDF;
 };
 
@@ -240,18 +241,28 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->key_dev.ensure(1, "key"));
   MCLS_TRY(c->sidx_dev.ensure(1, "solve index"));
   MCLS_TRY(c->mag.ensure(n, "source magnitudes"));
-  if (c->prm.precision == 32) {
-    if (!c->t32.p || c->t32.n < n) {
-      MCLS_TRY(c->t32.ensure(n, "tally32"));
-      HIP_TRY(hipMemsetAsync(c->t32.p, 0, n * sizeof(float), c->stream));
-    }
-  } else {
-    if (!c->t64.p || c->t64.n < n) {
-      MCLS_TRY(c->t64.ensure(n, "tally64"));
-      HIP_TRY(hipMemsetAsync(c->t64.p, 0, n * sizeof(double), c->stream));
-    }
+  if (c->prm.precision == 32 && (!c->t32.p || c->t32.n < n)) {
+    MCLS_TRY(c->t32.ensure(n, "tally32"));
+    HIP_TRY(hipMemsetAsync(c->t32.p, 0, n * sizeof(float), c->stream));
+  }
+  // FP64 walks, and FP32 walks whose LDS tile flushes into an FP64 tally
+  if ((c->prm.precision == 64 || c->prm.tally_tile > 0) && (!c->t64.p || c->t64.n < n)) {
+    MCLS_TRY(c->t64.ensure(n, "tally64"));
+    HIP_TRY(hipMemsetAsync(c->t64.p, 0, n * sizeof(double), c->stream));
   }
   return MCLS_OK;
+}
+
+bool tile_on(const mcls_gpu_context* c, bool record) {
+  return !record && c->prm.tally_tile > 0 && c->prm.estimator == mcls::EST_COLLISION;
+}
+
+// The tally a walk writes: FP64 for FP64 walks and for tiled FP32 walks.
+double* tally64_of(mcls_gpu_context* c, bool record) {
+  return (c->prm.precision == 64 || tile_on(c, record)) ? c->t64.p : nullptr;
+}
+float* tally32_of(mcls_gpu_context* c, bool record) {
+  return (c->prm.precision == 32 && !tile_on(c, record)) ? c->t32.p : nullptr;
 }
 
 void invalidate_graph(mcls_gpu_context* c) {
@@ -452,6 +463,7 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   wc.source = c->prm.source;
   wc.sampler = c->prm.sampler;
   wc.record = record;
+  wc.tile = tile_on(c, record);
   wc.block = (int)c->prm.walk_block;
   int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
   int64_t cap = (int64_t)c->num_cus * per_cu;
@@ -494,8 +506,11 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.last = last;
   a.history_length = c->prm.history_length;
   a.cutoff = c->prm.weight_cutoff;
-  a.tally64 = c->prm.precision == 32 ? nullptr : c->t64.p;
-  a.tally32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  a.tally64 = c->t64.p;
+  a.tally32 = c->t32.p;
+  a.tile_slots = (int)c->prm.tally_tile;
+  a.tile_shift = 32;
+  for (int64_t t = c->prm.tally_tile; t > 1; t >>= 1) --a.tile_shift;
   a.steps = c->steps.p;
   a.batch = (int)c->prm.walk_batch;
   return a;
@@ -535,13 +550,18 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), 0, last);
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, last);
+  if (wc.tile) {
+    const size_t elem = sizeof(int) + (c->prm.precision == 32 ? sizeof(float) : sizeof(double));
+    if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
+      return fail(MCLS_ERR_INVALID_ARG, "Tally Tile x waves per block exceeds 64 KiB of LDS per workgroup");
+  }
   if (nh > 0) HIP_TRY(mcls::launch_walk(a, wc, s));
   if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
   const bool ev = c->prm.estimator == mcls::EST_EXPECTED_VALUE;
   double inv = nh > 0 ? 1.0 / (double)nh : 0.0;
   const uint64_t* mtot = strat ? c->ncdf.p + (c->n - 1) : nullptr;
-  double* t64 = c->prm.precision == 32 ? nullptr : c->t64.p;
-  float* t32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  double* t64 = tally64_of(c, false);
+  float* t32 = tally32_of(c, false);
   if (accumulate)
     HIP_TRY(mcls::launch_apply_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
   else
@@ -828,6 +848,11 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
   } else if (key_is(key, "Walk Blocks Per CU")) {
     if (v < 0 || v > 64) return fail(MCLS_ERR_INVALID_ARG, "Walk Blocks Per CU must be in [0, 64]");
     p.walk_blocks_per_cu = v;
+  } else if (key_is(key, "Tally Tile")) {
+    if (v != 0 && (v < 64 || v > 2048 || (v & (v - 1)) != 0))
+      return fail(MCLS_ERR_INVALID_ARG, "Tally Tile must be 0 or a power of two in [64, 2048]");
+    p.tally_tile = v;
+    if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
   } else if (key_is(key, "Walk Batch")) {
     if (v < 64 || v > 4096 || v % 64 != 0)
       return fail(MCLS_ERR_INVALID_ARG, "Walk Batch must be a multiple of 64 in [64, 4096]");
@@ -1003,8 +1028,8 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   WalkConfig wc = walk_config(c, true, count);
   HIP_TRY(mcls::launch_walk(a, wc, s));
   // clear the tally the recorded walks wrote
-  double* t64 = c->prm.precision == 32 ? nullptr : c->t64.p;
-  float* t32 = c->prm.precision == 32 ? c->t32.p : nullptr;
+  double* t64 = tally64_of(c, true);
+  float* t32 = tally32_of(c, true);
   HIP_TRY(mcls::launch_store_tally(c->n, c->x.p, t64, t32, c->rhs.p, false, 0.0, nullptr, s));
   HIP_TRY(hipMemcpyAsync(paths, dpaths.p, sizeof(int) * (size_t)count * (size_t)(max_len + 1),
                          hipMemcpyDeviceToHost, s));
@@ -1230,10 +1255,12 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   if (nh > 0 && np > nh) np = nh;
   c->prm.num_histories = np;
   c->nsys = 0;  // pilot births follow |PL b| over the whole operator
-  auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, double* ms) -> int {
+  auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double* ms) -> int {
     c->prm.walk_block = blk;
     c->prm.walk_blocks_per_cu = bpc;
     c->prm.walk_batch = bat;
+    c->prm.tally_tile = tile;
+    MCLS_TRY(ensure_vectors(c));
     MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true));
     HIP_TRY(hipStreamSynchronize(s));
     float t = 0.f;
@@ -1246,10 +1273,11 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   best.block = saved.walk_block;
   best.blocks_per_cu = saved.walk_blocks_per_cu > 0 ? saved.walk_blocks_per_cu : 8;
   best.batch = saved.walk_batch;
+  best.tally_tile = saved.tally_tile;
   double ms = 0.0;
   // warm-up, then the configuration on entry
-  rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, &ms);
-  if (rc == MCLS_OK) rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, &ms);
+  rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
+  if (rc == MCLS_OK) rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
   best.ms_default = ms;
   best.ms_best = ms;
   best.candidates = 1;
@@ -1259,7 +1287,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   for (int i = 0; i < 3 && rc == MCLS_OK; ++i)
     for (int j = 0; j < 3 && rc == MCLS_OK; ++j) {
       const int64_t bpc = waves[j] * 64 / blocks[i];
-      rc = pilot_ms(blocks[i], bpc, best.batch, &ms);
+      rc = pilot_ms(blocks[i], bpc, best.batch, best.tally_tile, &ms);
       ++best.candidates;
       if (rc == MCLS_OK && ms < best.ms_best) {
         best.ms_best = ms;
@@ -1272,11 +1300,25 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   const int64_t shape_batch = best.batch;
   for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
     if (batches[k] == shape_batch) continue;
-    rc = pilot_ms(best.block, best.blocks_per_cu, batches[k], &ms);
+    rc = pilot_ms(best.block, best.blocks_per_cu, batches[k], best.tally_tile, &ms);
     ++best.candidates;
     if (rc == MCLS_OK && ms < best.ms_best) {
       best.ms_best = ms;
       best.batch = batches[k];
+    }
+  }
+  // LDS tally tile (collision estimator): off, 128 or 512 entries per wave
+  if (saved.estimator == mcls::EST_COLLISION) {
+    const int64_t tiles[3] = {0, 128, 512};
+    const int64_t shape_tile = best.tally_tile;
+    for (int k = 0; k < 3 && rc == MCLS_OK; ++k) {
+      if (tiles[k] == shape_tile) continue;
+      rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, tiles[k], &ms);
+      ++best.candidates;
+      if (rc == MCLS_OK && ms < best.ms_best) {
+        best.ms_best = ms;
+        best.tally_tile = tiles[k];
+      }
     }
   }
   c->prm = saved;
@@ -1285,6 +1327,8 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   c->prm.walk_block = best.block;
   c->prm.walk_blocks_per_cu = best.blocks_per_cu;
   c->prm.walk_batch = best.batch;
+  c->prm.tally_tile = best.tally_tile;
+  MCLS_TRY(ensure_vectors(c));
   invalidate_graph(c);
   *out = best;
   return MCLS_OK;

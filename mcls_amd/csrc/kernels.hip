@@ -17,6 +17,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -158,13 +159,18 @@ __device__ uint32_t Row<double>::target(const DomainView& d, uint32_t k, uint32_
 // transition (row CDF inversion or alias slot), multiply the weight by
 // W_i sign(h_ij), stop when steps >= L or |w| < cutoff |w0|.
 // --------------------------------------------------------------------------
-template <typename Real, int EST, int SRC, int SAMP, bool REC>
+// With TILE, collision tallies first go to a per-wave direct-mapped tile of
+// states in LDS (a.tile_slots entries of {state, partial sum}); a state that
+// finds its slot taken by another goes straight to HBM.  A wave flushes its
+// tile into the FP64 HBM tally at every work-queue grab and when it exits.
+template <typename Real, int EST, int SRC, int SAMP, bool REC, bool TILE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
+  using Acc = typename std::conditional<TILE, double, Real>::type;
   const DomainView& d = a.d;
   const int n = d.n;
   const Real* __restrict__ cdf = Tables<Real>::cdf(d);
   const Real* __restrict__ ht = Tables<Real>::h(d);
-  Real* __restrict__ tally = Tables<Real>::tally(a);
+  Acc* __restrict__ tally = TILE ? (Acc*)a.tally64 : (Acc*)Tables<Real>::tally(a);
   const uint32_t* __restrict__ colp = d.col;
 
   int64_t hi_id = a.last;
@@ -186,6 +192,38 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const int64_t send = a.first + span * (qs + 1) / kQueueSlots;
   unsigned long long* ctr = a.queue + qs * kQueueStride;
   const int lane = __lane_id();
+
+  int* tkeys = nullptr;  // TILE: this wave's slots
+  Real* tvals = nullptr;
+  if constexpr (TILE) {
+    extern __shared__ __align__(16) unsigned char walk_lds[];
+    const int wv = threadIdx.x >> 6;
+    tkeys = (int*)walk_lds + wv * a.tile_slots;
+    tvals = (Real*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
+    for (int k = lane; k < a.tile_slots; k += 64) {
+      tkeys[k] = -1;
+      tvals[k] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // wave-uniform: every lane of the wave runs it together
+  auto flush_tile = [&]() {
+    if constexpr (TILE) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int k = lane; k < a.tile_slots; k += 64) {
+        const int key = tkeys[k];
+        if (key >= 0) {
+          tally_add(&tally[key], (double)tvals[k]);
+          tkeys[k] = -1;
+          tvals[k] = 0;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  };
 
   // The current row's header is carried from the record that chose the
   // state (FP32 alias records, CDF / learned target records); FP64 alias
@@ -210,6 +248,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       uint64_t need = __ballot(!alive);
       while (need != 0) {
         if (qnext >= qend) {
+          flush_tile();
           const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
           unsigned long long b = 0;
           if (lane == leader) b = atomicAdd(ctr, (unsigned long long)a.batch);
@@ -266,7 +305,18 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       Wrow = Row<Real>::W(hd);
     }
     if (EST == EST_EXPECTED_VALUE) {
-      for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], w * ht[beg + k]);
+      for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
+    } else if constexpr (TILE) {
+      const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
+      int k = tkeys[slot];
+      if (k < 0) {
+        k = atomicCAS(&tkeys[slot], -1, state);
+        if (k < 0) k = state;
+      }
+      if (k == state)
+        atomicAdd(&tvals[slot], w);
+      else
+        tally_add(&tally[state], (Acc)w);
     } else {
       tally_add(&tally[state], w);
     }
@@ -315,15 +365,24 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       }
     }
   }
+  flush_tile();  // the wave has reconverged after the loop
   if (my_steps) atomicAdd(a.steps, (unsigned long long)my_steps);
 }
 
 template <typename Real, int EST, int SRC, int SAMP>
 hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
-  if (c.record)
-    k_walk<Real, EST, SRC, SAMP, true><<<c.grid, c.block, 0, s>>>(a);
-  else
-    k_walk<Real, EST, SRC, SAMP, false><<<c.grid, c.block, 0, s>>>(a);
+  if (c.record) {
+    k_walk<Real, EST, SRC, SAMP, true, false><<<c.grid, c.block, 0, s>>>(a);
+  } else if constexpr (EST == EST_COLLISION) {
+    if (c.tile) {
+      const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * (sizeof(int) + sizeof(Real));
+      k_walk<Real, EST, SRC, SAMP, false, true><<<c.grid, c.block, lds, s>>>(a);
+    } else {
+      k_walk<Real, EST, SRC, SAMP, false, false><<<c.grid, c.block, 0, s>>>(a);
+    }
+  } else {
+    k_walk<Real, EST, SRC, SAMP, false, false><<<c.grid, c.block, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 

@@ -93,6 +93,8 @@ struct WalkArgs {
   double* wfinal;          // recording: final weight per history
   int max_len;
   int batch;               // history ids a wave takes per queue grab (multiple of 64)
+  int tile_slots;          // LDS tally tile entries per wave (power of two), 0 = none
+  int tile_shift;          // 32 - log2(tile_slots)
 };
 
 struct WalkConfig {
@@ -101,6 +103,7 @@ struct WalkConfig {
   int source;      // SourceMode
   int sampler;     // Sampler
   bool record;
+  bool tile;       // LDS tally tile (collision estimator; tallies go to tally64)
   int block;       // threads per block
   int grid;        // blocks (rounded up to a multiple of kQueueSlots)
 };

@@ -26,7 +26,8 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec):
     t = ctx.autotune(prob.b, pilot_histories=1 << 16)
     print("tuning", t)
     assert ctx.stats().solve_index == before
-    assert t["candidates"] == 13
+    assert t["candidates"] == 15
+    assert t["tally_tile"] in (0, 128, 512)
     assert t["block"] in (64, 128, 256) and t["batch"] in (64, 128, 256, 512)
     assert t["blocks_per_cu"] * t["block"] // 64 in (8, 16, 32)
     assert 0.0 < t["ms_best"] <= t["ms_default"]
@@ -39,7 +40,7 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec):
         x = ctx.mc_solve_raw(rhs, 6)
         tol = 1e-12 if prec == 64 else 1e-4  # FP32 atomics: summation order
         assert np.linalg.norm(x - ref_x) <= tol * np.linalg.norm(ref_x)
-    ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64})
+    ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
 
 
 def test_walk_batch_validation(ctx):
@@ -47,3 +48,43 @@ def test_walk_batch_validation(ctx):
     for bad in (0, 32, 100, 8192):
         with pytest.raises(capi.GpuError):
             ctx.set_params({"Walk Batch": bad})
+
+
+@pytest.mark.parametrize("sampler,prec", [("Alias", 32), ("CDF", 64), ("CDF", 32)])
+def test_tally_tile_matches_direct_tally(ctx, sampler, prec):
+    """The LDS tile changes only where the additions happen: same paths, same
+    tallies to the rounding of the summation order (FP32 walks accumulate in
+    FP64 through the tile, so closer to the FP64 answer)."""
+    prob, rhs = _poisson(ctx, sampler, prec)
+    ctx.set_params({"Tally Tile": 0})
+    ref = ctx.mc_solve_raw(rhs, 3)
+    for tile, blk in ((64, 256), (512, 128), (1024, 64)):
+        ctx.set_params({"Tally Tile": tile, "Walk Block Size": blk, "Walk Blocks Per CU": 0})
+        x = ctx.mc_solve_raw(rhs, 3)
+        tol = 1e-12 if prec == 64 else 1e-4
+        assert np.linalg.norm(x - ref) <= tol * np.linalg.norm(ref), tile
+    # FP32 walks with the tile flush into an FP64 tally: the result equals the
+    # FP64 accumulation of the same FP32 paths to ~FP32 weight rounding
+    ctx.set_params({"Tally Tile": 0, "Walk Block Size": 256})
+    with pytest.raises(capi.GpuError):
+        ctx.set_params({"Tally Tile": 100})
+    if prec == 64:
+        ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
+        with pytest.raises(capi.GpuError):
+            ctx.mc_solve_raw(rhs, 4)  # 4 waves x 2048 x 12 bytes > 64 KiB
+    ctx.set_params({"Tally Tile": 0})
+
+
+def test_mcsa_with_tally_tile_converges(ctx):
+    prob = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 16, seed=3, p0=0.6, p1=0.9, p2=0.8, p3=1.2),
+                              want64=False, want32=True)
+    ctx.set_problem(prob, "f32")
+    ctx.set_point_jacobi()
+    ctx.set_params({"Walk Precision": "FP32", "Transition Sampler": "Alias", "Source Sampling Type": "Stratified",
+                    "History Length": 1000, "Weight Cutoff": 1e-2, "Number of Histories": 16 * prob.n,
+                    "Random Seed": 3, "Convergence Tolerance": 1e-8, "Maximum Iterations": 50,
+                    "Tally Tile": 256, "Walk Block Size": 256, "Walk Blocks Per CU": 0})
+    x = ctx.mcsa_solve(prob.b)
+    st = ctx.stats()
+    ctx.set_params({"Tally Tile": 0})
+    assert st.converged and st.achieved_tol <= 1e-8
