@@ -120,6 +120,8 @@ int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kin
  *   i64 "Random Seed"          (0)
  *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
  *   i64 "Verbosity"            (0)
+ *   i64 "Compute Convergence Criteria" (0) print mcls_gpu_convergence_criteria
+ *        after each domain build (src/MCLS_AlmostOptimalDomain_impl.hpp:95-112)
  *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto),
  *   i64 "Walk Batch"           (64) history ids per work-queue grab
  *   i64 "Tally Tile"           (0) LDS tally tile entries per wave (0 = off,
@@ -160,6 +162,28 @@ int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            const double* x, double* r, double* norm2,
                            int ptr_kind);
 int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
+
+/* ------------------------------------------------- convergence criteria -- */
+/* AlmostOptimalDomain::computeConvergenceCriteria
+ * (src/MCLS_AlmostOptimalDomain_impl.hpp:439-576) for the domain operator
+ * (rows of H^T, built first if needed): the spectral radii of H, |H| and
+ * H* (H*_ij = |h_ij| W_i; rho(H*) < 1 is the condition for a finite
+ * estimator variance), ||H||_inf, the smallest row sum of |H|, and the
+ * balance ratios max_k |h_ik| / max_k |h_ki| over non-empty rows.  The
+ * reference uses Anasazi's Generalized Davidson; here power iteration on
+ * the device, the growth factor averaged over windows of 32 steps, stopped
+ * when two windows agree to `tol` (relative) or after `max_iters` (>= 64)
+ * steps.  With the i64 parameter "Compute Convergence Criteria" = 1 they
+ * are also printed after every domain build, as the reference does. */
+typedef struct mcls_gpu_criteria {
+  double rho_h, rho_habs, rho_hstar;   /* rho(H), rho(|H|) ("H+"), rho(H*)   */
+  double inf_norm;                     /* ||H||_inf = max_i W_i              */
+  double min_row_sum;                  /* min_i W_i                          */
+  double balance_min, balance_max, balance_ave;
+  int64_t iterations[3];               /* power steps per radius             */
+  int64_t converged[3];                /* 1 when two windows agreed to tol   */
+} mcls_gpu_criteria;
+int mcls_gpu_convergence_criteria(mcls_gpu_context* ctx, int64_t max_iters, double tol, mcls_gpu_criteria* out);
 
 /* ----------------------------------------------------------- autotuning -- */
 /* Walk launch shape picked by timing (no counterpart in the reference, whose

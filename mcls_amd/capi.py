@@ -36,7 +36,7 @@ GPU_SYMBOLS = [
     "mcls_gpu_get_domain", "mcls_gpu_mc_solve", "mcls_gpu_mcsa_solve",
     "mcls_gpu_prec_residual", "mcls_gpu_get_stats", "mcls_gpu_record_paths",
     "mcls_gpu_mc_solve_raw", "mcls_gpu_set_systems", "mcls_gpu_get_system_residuals",
-    "mcls_gpu_autotune",
+    "mcls_gpu_autotune", "mcls_gpu_convergence_criteria",
     "mcls_gpu_set_importance", "mcls_gpu_set_importance_diagonal", "mcls_gpu_get_importance",
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
@@ -97,6 +97,27 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Criteria(C.Structure):
+    _fields_ = [
+        ("rho_h", C.c_double),
+        ("rho_habs", C.c_double),
+        ("rho_hstar", C.c_double),
+        ("inf_norm", C.c_double),
+        ("min_row_sum", C.c_double),
+        ("balance_min", C.c_double),
+        ("balance_max", C.c_double),
+        ("balance_ave", C.c_double),
+        ("iterations", C.c_int64 * 3),
+        ("converged", C.c_int64 * 3),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_[:8]}
+        d["iterations"] = list(self.iterations)
+        d["converged"] = list(self.converged)
+        return d
+
+
 class ProblemSpec(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
@@ -149,6 +170,7 @@ def gpu_lib():
     L.mcls_gpu_set_importance.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_importance_diagonal.argtypes = [vp]
     L.mcls_gpu_autotune.argtypes = [vp, vp, i32, C.c_int64, C.POINTER(Tuning)]
+    L.mcls_gpu_convergence_criteria.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(Criteria)]
     L.mcls_gpu_get_importance.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_importance_model.argtypes = [vp, vp]
     L.mcls_gpu_set_importance_mask.argtypes = [vp, vp, i32]
@@ -400,6 +422,11 @@ class Context:
         t = Tuning()
         self._check(self.L.mcls_gpu_autotune(self.h, _p(b), MCLS_HOST, int(pilot_histories), C.byref(t)))
         return t.as_dict()
+
+    def convergence_criteria(self, max_iters=5000, tol=1e-6):
+        c = Criteria()
+        self._check(self.L.mcls_gpu_convergence_criteria(self.h, int(max_iters), float(tol), C.byref(c)))
+        return c.as_dict()
 
     def set_importance_diagonal(self):
         self._check(self.L.mcls_gpu_set_importance_diagonal(self.h))

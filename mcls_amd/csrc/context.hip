@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -91,6 +92,7 @@ struct Params {
   int64_t walk_blocks_per_cu = 0;
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
+  int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int sampler = mcls::SAMPLER_CDF;
 };
 
@@ -165,6 +167,9 @@ struct mcls_gpu_context {
   DBuf<int64_t> sys_off;
   DBuf<int> sys_of, sys_conv, sys_unconv;
   DBuf<double> sys_b2, sys_r2, sys_l1, mag;
+  // convergence-criteria work (power iteration vectors, scalars, column maxima)
+  DBuf<double> pw_x, pw_y, pw_scal;
+  DBuf<unsigned long long> pw_cmax, pw_mm;
   std::vector<int64_t> sys_off_h;
   std::vector<double> sys_rel;  // per-system relative residuals of the last solve
 };
@@ -274,6 +279,8 @@ void invalidate_graph(mcls_gpu_context* c) {
 }
 
 // --------------------------------------------------------------- domain
+int convergence_criteria(mcls_gpu_context* c, int64_t max_iters, double tol, mcls_gpu_criteria* out);
+
 int build_domain(mcls_gpu_context* c) {
   MCLS_TRY(check_matrix(c));
   const int n = c->n;
@@ -340,6 +347,17 @@ int build_domain(mcls_gpu_context* c) {
   c->domain_valid = true;
   c->domain_omega = omega;
   c->domain_threshold = thr;
+  if (c->prm.compute_criteria) {
+    mcls_gpu_criteria cr{};
+    MCLS_TRY(convergence_criteria(c, 5000, 1e-6, &cr));
+    // src/MCLS_AlmostOptimalDomain_impl.hpp:95-112 and :519-531
+    std::printf("\n||H||_inf = %.10g\nsum|H|_min = %.10g\n\nH Balance Parameters\nmin: %.10g\nmax: %.10g\nave: %.10g\n",
+                cr.inf_norm, cr.min_row_sum, cr.balance_min, cr.balance_max, cr.balance_ave);
+    std::printf("\nNeumann/Ulam Convergence Criteria\n---------------------------------\n"
+                "rho(H)  = %.10g\nrho(H+) = %.10g\nrho(H*) = %.10g\n---------------------------------\n\n",
+                cr.rho_h, cr.rho_habs, cr.rho_hstar);
+    std::fflush(stdout);
+  }
   return MCLS_OK;
 }
 
@@ -656,6 +674,86 @@ int mcsa_iteration(mcls_gpu_context* c, bool timed, const uint64_t* key_dev, uin
   return MCLS_OK;
 }
 
+// Spectral radii by power iteration on the domain operator (the growth
+// factor averaged over windows of 32 steps in log space, which also holds
+// when the dominant eigenvalues are a +-rho or complex pair) and the row /
+// column statistics of |H|; src/MCLS_AlmostOptimalDomain_impl.hpp:439-576
+// (Anasazi Generalized Davidson there).
+int convergence_criteria(mcls_gpu_context* c, int64_t max_iters, double tol, mcls_gpu_criteria* out) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  MCLS_TRY(c->pw_x.ensure((size_t)n, "power vector"));
+  MCLS_TRY(c->pw_y.ensure((size_t)n, "power vector"));
+  MCLS_TRY(c->pw_scal.ensure(8, "power scalars"));
+  MCLS_TRY(c->pw_cmax.ensure((size_t)n, "column maxima"));
+  MCLS_TRY(c->pw_mm.ensure(4, "row statistics"));
+  MCLS_TRY(ensure_vectors(c));
+  constexpr int kWindow = 32;
+  double* nrm2 = c->pw_scal.p;
+  double* logsum = c->pw_scal.p + 1;
+  double rho[3] = {0, 0, 0};
+  for (int mode = 0; mode < 3; ++mode) {
+    double* x = c->pw_x.p;
+    double* y = c->pw_y.p;
+    HIP_TRY(mcls::launch_power_start(n, 0x5EEDULL + (uint64_t)mode, x, s));
+    HIP_TRY(mcls::launch_fill_f64(logsum, 1, 0.0, s));
+    double prev_log = 0.0, prev_est = -1.0, est = 0.0;
+    int64_t it = 0;
+    bool conv = false;
+    while (it < max_iters) {
+      HIP_TRY(mcls::launch_domain_power(n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, mode, x,
+                                        it == 0 ? nullptr : nrm2, y, c->part.p, nrm2, logsum, s));
+      std::swap(x, y);
+      ++it;
+      if (it % kWindow == 0) {
+        double L = 0.0;
+        HIP_TRY(hipMemcpyAsync(&L, logsum, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!std::isfinite(L)) {  // the iterate vanished: nilpotent on the start vector
+          est = 0.0;
+          conv = true;
+          break;
+        }
+        if (it > kWindow) {
+          est = std::exp((L - prev_log) / kWindow);
+          if (prev_est >= 0.0 && std::fabs(est - prev_est) <= tol * est) {
+            conv = true;
+            prev_log = L;
+            break;
+          }
+          prev_est = est;
+        }
+        prev_log = L;
+      }
+    }
+    rho[mode] = est;
+    out->iterations[mode] = it;
+    out->converged[mode] = conv ? 1 : 0;
+  }
+  out->rho_h = rho[0];
+  out->rho_habs = rho[1];
+  out->rho_hstar = rho[2];
+  double* sums = c->pw_scal.p + 2;
+  HIP_TRY(mcls::launch_balance(n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->pw_cmax.p, c->pw_mm.p,
+                               c->part.p, sums, s));
+  unsigned long long mm[4];
+  double sm[2];
+  HIP_TRY(hipMemcpyAsync(mm, c->pw_mm.p, sizeof(mm), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(sm, sums, sizeof(sm), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  auto as_double = [](unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof(d));
+    return d;
+  };
+  out->inf_norm = as_double(mm[0]);
+  out->min_row_sum = as_double(mm[1]);
+  out->balance_min = sm[1] > 0 ? as_double(mm[2]) : 0.0;
+  out->balance_max = sm[1] > 0 ? as_double(mm[3]) : 0.0;
+  out->balance_ave = sm[1] > 0 ? sm[0] / sm[1] : 0.0;
+  return MCLS_OK;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -848,6 +946,8 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
   } else if (key_is(key, "Walk Blocks Per CU")) {
     if (v < 0 || v > 64) return fail(MCLS_ERR_INVALID_ARG, "Walk Blocks Per CU must be in [0, 64]");
     p.walk_blocks_per_cu = v;
+  } else if (key_is(key, "Compute Convergence Criteria")) {
+    p.compute_criteria = v ? 1 : 0;
   } else if (key_is(key, "Tally Tile")) {
     if (v != 0 && (v < 64 || v > 2048 || (v & (v - 1)) != 0))
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile must be 0 or a power of two in [64, 2048]");
@@ -1234,6 +1334,16 @@ int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
   invalidate_graph(c);
   if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
   return MCLS_OK;
+}
+
+int mcls_gpu_convergence_criteria(mcls_gpu_context* c, int64_t max_iters, double tol, mcls_gpu_criteria* out) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  if (!out) return fail(MCLS_ERR_INVALID_ARG, "null output");
+  if (max_iters < 64 || !(tol > 0.0)) return fail(MCLS_ERR_INVALID_ARG, "max_iters must be >= 64 and tol > 0");
+  MCLS_TRY(ensure_domain(c));
+  *out = mcls_gpu_criteria{};
+  return convergence_criteria(c, max_iters, tol, out);
 }
 
 int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_t pilot, mcls_gpu_tuning* out) {

@@ -1032,6 +1032,110 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __re
   if (threadIdx.x == 0) out[0] = sm[0];
 }
 
+// ------------------------------------------------- convergence criteria
+// y = M (x / sqrt(*nrm2_prev)) for the domain operator M (rows of H^T):
+// mode 0: M = H, 1: |H|, 2: H* with H*_ij = |h_ij| W_i; per-block partial
+// sums of y^2 into part (grid-stride, fixed grid: deterministic).
+__global__ __launch_bounds__(kRedThreads) void k_domain_power(int n, const int* __restrict__ d_rp,
+                                                              const uint32_t* __restrict__ col,
+                                                              const double* __restrict__ h64,
+                                                              const double* __restrict__ W64, int mode,
+                                                              const double* __restrict__ x,
+                                                              const double* __restrict__ nrm2_prev,
+                                                              double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sm[kRedThreads];
+  const double p = nrm2_prev ? *nrm2_prev : 1.0;
+  const double scale = p > 0.0 ? 1.0 / sqrt(p) : 1.0;
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double v = 0.0;
+    const double wi = mode == 2 ? W64[i] : 1.0;
+    for (int k = d_rp[i]; k < d_rp[i + 1]; ++k) {
+      const double h = h64[k];
+      const double m = mode == 0 ? h : fabs(h) * wi;
+      v = v + m * x[col[k] & 0x7FFFFFFFu];
+    }
+    v = v * scale;
+    y[i] = v;
+    acc = acc + v * v;
+  }
+  sm[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// *logsum += log(sqrt(*nrm2)) (one thread)
+__global__ void k_log_accum(const double* nrm2, double* logsum) {
+  const double v = *nrm2;
+  *logsum = *logsum + (v > 0.0 ? 0.5 * log(v) : -INFINITY);
+}
+
+// Deterministic positive start vector x_i in [0.5, 1.5).
+__global__ void k_power_start(int n, uint64_t seed, double* x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] = 0.5 + u01_f64(splitmix64(seed ^ splitmix64((uint64_t)i)));
+}
+
+// Column maxima of |h| (bits of non-negative doubles order as integers).
+__global__ void k_col_absmax(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                             const double* __restrict__ h64, unsigned long long* __restrict__ cmax) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int k = d_rp[i]; k < d_rp[i + 1]; ++k)
+      atomicMax(&cmax[col[k] & 0x7FFFFFFFu], (unsigned long long)__double_as_longlong(fabs(h64[k])));
+}
+
+// Per non-empty row i: ratio_i = max_k |h_ik| / colmax_i; out[0] = max W,
+// out[1] = min W, out[2] = min ratio, out[3] = max ratio, out[4] = sum of
+// ratios, out[5] = rows counted (block results combined by atomics on the
+// bits of non-negative doubles for min/max; sums per block into part).
+__global__ __launch_bounds__(kRedThreads) void k_balance(int n, const int* __restrict__ d_rp,
+                                                         const double* __restrict__ h64,
+                                                         const double* __restrict__ W64,
+                                                         const unsigned long long* __restrict__ cmax,
+                                                         unsigned long long* __restrict__ mm,
+                                                         double* __restrict__ part) {
+  __shared__ double ssum[kRedThreads], scnt[kRedThreads];
+  double sum = 0.0, cnt = 0.0;
+  unsigned long long wmax = 0, wmin = ~0ULL, rmin = ~0ULL, rmax = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned long long wb = (unsigned long long)__double_as_longlong(W64[i]);
+    if (wb > wmax) wmax = wb;
+    if (wb < wmin) wmin = wb;
+    if (d_rp[i + 1] == d_rp[i]) continue;
+    double rm = 0.0;
+    for (int k = d_rp[i]; k < d_rp[i + 1]; ++k) rm = fmax(rm, fabs(h64[k]));
+    const double cm = __longlong_as_double((long long)cmax[i]);
+    const double r = rm / cm;
+    const unsigned long long rb = (unsigned long long)__double_as_longlong(r);
+    if (rb < rmin) rmin = rb;
+    if (rb > rmax) rmax = rb;
+    sum = sum + r;
+    cnt = cnt + 1.0;
+  }
+  if (wmax) atomicMax(&mm[0], wmax);
+  atomicMin(&mm[1], wmin);
+  atomicMin(&mm[2], rmin);
+  if (rmax) atomicMax(&mm[3], rmax);
+  ssum[threadIdx.x] = sum;
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + off];
+      scnt[threadIdx.x] = scnt[threadIdx.x] + scnt[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = ssum[0];
+    part[kResidualPartials + blockIdx.x] = scnt[0];
+  }
+}
+
 // ------------------------------------------------------- batched systems
 // One block per system s (rows [off[s], off[s+1])): out[s] = sum_i (d_i v_i)^2
 // (d = dinv or 1) when square, else sum_i |v_i|; fixed-order block tree.
@@ -1353,6 +1457,35 @@ hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col,
                                 hipStream_t s) {
   return f32 ? prec_residual_t<float>(n, lanes, rp, col, val, dinv, b, x, r, part, s)
              : prec_residual_t<double>(n, lanes, rp, col, val, dinv, b, x, r, part, s);
+}
+
+hipError_t launch_domain_power(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                               int mode, const double* x, const double* nrm2_prev, double* y, double* part,
+                               double* nrm2, double* logsum, hipStream_t s) {
+  const int g = residual_blocks(n, 1);
+  k_domain_power<<<g, kRedThreads, 0, s>>>(n, d_rp, col, h64, W64, mode, x, nrm2_prev, y, part);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part, g, nrm2);
+  k_log_accum<<<1, 1, 0, s>>>(nrm2, logsum);
+  return hipGetLastError();
+}
+
+hipError_t launch_power_start(int n, uint64_t seed, double* x, hipStream_t s) {
+  k_power_start<<<grid_for(n, 256), 256, 0, s>>>(n, seed, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_balance(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                          unsigned long long* cmax, unsigned long long* mm, double* part, double* sums,
+                          hipStream_t s) {
+  k_fill<unsigned long long><<<grid_for(n, 256), 256, 0, s>>>(cmax, n, 0ULL);
+  k_col_absmax<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, cmax);
+  const unsigned long long init[4] = {0ULL, ~0ULL, ~0ULL, 0ULL};
+  for (int k = 0; k < 4; ++k) k_fill<unsigned long long><<<1, 1, 0, s>>>(mm + k, 1, init[k]);
+  const int g = residual_blocks(n, 1);
+  k_balance<<<g, kRedThreads, 0, s>>>(n, d_rp, h64, W64, cmax, mm, part);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part, g, sums);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(part + kResidualPartials, g, sums + 1);
+  return hipGetLastError();
 }
 
 hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s) {

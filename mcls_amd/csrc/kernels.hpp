@@ -199,6 +199,22 @@ hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col,
                                 double* r, double* part, hipStream_t s);
 // out[0] = sum of part[0..m) in a fixed order; out_host (pinned, nullable)
 hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s);
+// Convergence criteria of the domain operator M (rows of H^T).
+// One power step: y = M (x / sqrt(*nrm2_prev)) (nrm2_prev nullable = 1),
+// *nrm2 = ||y||^2, *logsum += log ||y||.  mode 0: H, 1: |H|, 2: H* (|h_ij| W_i).
+// part holds kResidualPartials doubles.
+hipError_t launch_domain_power(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                               int mode, const double* x, const double* nrm2_prev, double* y, double* part,
+                               double* nrm2, double* logsum, hipStream_t s);
+// x_i = 0.5 + u(splitmix(seed ^ splitmix(i)))
+hipError_t launch_power_start(int n, uint64_t seed, double* x, hipStream_t s);
+// Row-sum and balance statistics: mm (4 words) = bits of {max W, min W,
+// min ratio, max ratio}, sums[0] = sum of ratios, sums[1] = rows counted;
+// ratio_i = max_k |h_ik| / max_k |h_ki| over non-empty rows; cmax: n words,
+// part: 2 * kResidualPartials doubles.
+hipError_t launch_balance(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                          unsigned long long* cmax, unsigned long long* mm, double* part, double* sums,
+                          hipStream_t s);
 // ||PL b||_2^2 partials
 hipError_t launch_scaled_sq(int n, const double* dinv, const double* b, double* part,
                             hipStream_t s);
