@@ -93,6 +93,7 @@ struct Params {
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
+  int forward = 0;               // "Monte Carlo Type" = "Forward"
   int sampler = mcls::SAMPLER_CDF;
 };
 
@@ -123,6 +124,7 @@ struct mcls_gpu_context {
   // domain
   bool domain_valid = false;
   double domain_omega = 0.0, domain_threshold = 0.0;
+  int domain_forward = 0;  // the built domain holds rows of H (forward) or of H^T
   int64_t d_nnz = 0;
   DBuf<int> d_rp;
   DBuf<uint32_t> d_col;
@@ -259,7 +261,7 @@ int ensure_vectors(mcls_gpu_context* c) {
 }
 
 bool tile_on(const mcls_gpu_context* c, bool record) {
-  return !record && c->prm.tally_tile > 0 && c->prm.estimator == mcls::EST_COLLISION;
+  return !record && c->prm.tally_tile > 0 && c->prm.estimator == mcls::EST_COLLISION && !c->prm.forward;
 }
 
 // The tally a walk writes: FP64 for FP64 walks and for tiled FP32 walks.
@@ -300,10 +302,17 @@ int build_domain(mcls_gpu_context* c) {
   HIP_TRY(mcls::launch_fill_i32(cnt.p, n, 0, s));
   HIP_TRY(mcls::launch_fill_i32(fill.p, n, 0, s));
   const double* dinv = c->has_prec ? c->dinv.p : nullptr;
-  HIP_TRY(mcls::launch_count_cols(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, cnt.p, c->flags.p, s));
-  HIP_TRY(mcls::launch_scan_i32_excl(cnt.p, t_rp.p, n, itmp.p, s));
-  HIP_TRY(mcls::launch_scatter_transpose(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, fill.p,
-                                         t_src.p, t_val.p, s));
+  const bool forward = c->prm.forward != 0;
+  if (forward) {
+    // src/MCLS_MonteCarloSolverManager_impl.hpp:243-248: rows of PL A
+    HIP_TRY(mcls::launch_copy_rows(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, t_src.p, t_val.p,
+                                   c->flags.p, s));
+  } else {
+    HIP_TRY(mcls::launch_count_cols(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, cnt.p, c->flags.p, s));
+    HIP_TRY(mcls::launch_scan_i32_excl(cnt.p, t_rp.p, n, itmp.p, s));
+    HIP_TRY(mcls::launch_scatter_transpose(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, fill.p,
+                                           t_src.p, t_val.p, s));
+  }
   const double omega = c->prm.neumann_omega, thr = c->prm.threshold;
   HIP_TRY(mcls::launch_domain_rows(n, t_rp.p, t_src.p, t_val.p, omega, thr, c->has_prec, keep.p, c->flags.p, s));
   MCLS_TRY(c->d_rp.ensure((size_t)n + 1, "domain rp"));
@@ -347,6 +356,7 @@ int build_domain(mcls_gpu_context* c) {
   c->domain_valid = true;
   c->domain_omega = omega;
   c->domain_threshold = thr;
+  c->domain_forward = forward ? 1 : 0;
   if (c->prm.compute_criteria) {
     mcls_gpu_criteria cr{};
     MCLS_TRY(convergence_criteria(c, 5000, 1e-6, &cr));
@@ -462,13 +472,27 @@ int ensure_domain(mcls_gpu_context* c) {
 }
 
 int ensure_domain_impl(mcls_gpu_context* c) {
-  if (c->domain_valid && c->domain_omega == c->prm.neumann_omega && c->domain_threshold == c->prm.threshold)
+  if (c->domain_valid && c->domain_omega == c->prm.neumann_omega && c->domain_threshold == c->prm.threshold &&
+      c->domain_forward == c->prm.forward)
     return MCLS_OK;
   return build_domain(c);
 }
 
 // ------------------------------------------------------------- MC solve
+// Forward source (src/MCLS_UniformForwardSource_impl.hpp:78-91): every
+// state starts ceil(Sample Ratio) histories, or ceil(histories / N) when the
+// count is given.
+int64_t forward_per_state(const mcls_gpu_context* c) {
+  int64_t k;
+  if (c->prm.num_histories > 0)
+    k = (c->prm.num_histories + c->n - 1) / (c->n > 0 ? c->n : 1);
+  else
+    k = (int64_t)std::ceil(c->prm.sample_ratio);
+  return k > 0 ? k : 1;
+}
+
 int64_t num_histories(const mcls_gpu_context* c) {
+  if (c->prm.forward) return (int64_t)c->n * forward_per_state(c);
   if (c->prm.num_histories > 0) return c->prm.num_histories;
   // src/MCLS_UniformAdjointSource_impl.hpp:82-86: N * Sample Ratio (int)
   return (int64_t)((double)c->n * c->prm.sample_ratio);
@@ -477,7 +501,7 @@ int64_t num_histories(const mcls_gpu_context* c) {
 WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   WalkConfig wc;
   wc.precision = c->prm.precision;
-  wc.estimator = c->prm.estimator;
+  wc.estimator = c->prm.forward ? mcls::EST_FORWARD : c->prm.estimator;
   wc.source = c->prm.source;
   wc.sampler = c->prm.sampler;
   wc.record = record;
@@ -531,6 +555,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   for (int64_t t = c->prm.tally_tile; t > 1; t >>= 1) --a.tile_shift;
   a.steps = c->steps.p;
   a.batch = (int)c->prm.walk_batch;
+  a.per_state = forward_per_state(c);
   return a;
 }
 
@@ -561,9 +586,12 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
   if (c->prm.sampler == mcls::SAMPLER_LEARNED && c->model) MCLS_TRY(model_importance(c, rhs));
-  MCLS_TRY(build_source(c, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
+  const bool forward = c->prm.forward != 0;
+  if (forward && c->prm.estimator != mcls::EST_COLLISION)
+    return fail(MCLS_ERR_UNSUPPORTED, "the forward method tallies collisions only (Estimator Type 'Collision')");
+  if (!forward) MCLS_TRY(build_source(c, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
   if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
-  const bool strat = c->prm.source == mcls::SRC_STRATIFIED;
+  const bool strat = !forward && c->prm.source == mcls::SRC_STRATIFIED;
   const int64_t last = strat ? nh + c->n : nh;
   WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), 0, last);
   a.key_dev = key_dev;
@@ -575,8 +603,9 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   }
   if (nh > 0) HIP_TRY(mcls::launch_walk(a, wc, s));
   if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
-  const bool ev = c->prm.estimator == mcls::EST_EXPECTED_VALUE;
-  double inv = nh > 0 ? 1.0 / (double)nh : 0.0;
+  const bool ev = !forward && c->prm.estimator == mcls::EST_EXPECTED_VALUE;
+  // forward: each state's sum over its own histories (ForwardTally::finalize)
+  double inv = nh > 0 ? 1.0 / (double)(forward ? forward_per_state(c) : nh) : 0.0;
   const uint64_t* mtot = strat ? c->ncdf.p + (c->n - 1) : nullptr;
   double* t64 = tally64_of(c, false);
   float* t32 = tally32_of(c, false);
@@ -599,6 +628,10 @@ int accumulate_mc_timing(mcls_gpu_context* c) {
 }
 
 int histories_done(mcls_gpu_context* c, int64_t* out) {
+  if (c->prm.forward) {
+    *out = num_histories(c);
+    return MCLS_OK;
+  }
   if (c->prm.source == mcls::SRC_STRATIFIED) {
     uint64_t M = 0;
     HIP_TRY(hipMemcpy(&M, c->ncdf.p + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
@@ -1006,6 +1039,11 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     if (key_is(v, "Collision")) p.estimator = mcls::EST_COLLISION;
     else if (key_is(v, "Expected Value")) p.estimator = mcls::EST_EXPECTED_VALUE;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Estimator Type '") + v + "'");
+  } else if (key_is(key, "Monte Carlo Type")) {
+    // ForwardTag / AdjointTag of the reference's managers
+    if (key_is(v, "Adjoint")) p.forward = 0;
+    else if (key_is(v, "Forward")) p.forward = 1;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Monte Carlo Type '") + v + "'");
   } else if (key_is(key, "Transition Sampler")) {
     if (key_is(v, "CDF")) p.sampler = mcls::SAMPLER_CDF;
     else if (key_is(v, "Alias")) p.sampler = mcls::SAMPLER_ALIAS;
@@ -1120,7 +1158,8 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
   HIP_TRY(hipMemcpyAsync(c->rhs.p, rhs, bytes, hipMemcpyHostToDevice, s));
   const int64_t nh = num_histories(c);
-  MCLS_TRY(build_source(c, c->rhs.p, nh));
+  if (!c->prm.forward) MCLS_TRY(build_source(c, c->rhs.p, nh));
+  else if (first + count > nh) return fail(MCLS_ERR_INVALID_ARG, "history range beyond the forward source");
   WalkArgs a = walk_args(c, c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
   a.paths = dpaths.p;
   a.wfinal = dw.p;

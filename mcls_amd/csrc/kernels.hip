@@ -175,7 +175,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
 
   int64_t hi_id = a.last;
   uint64_t total = 0;  // Q (random) or M (stratified)
-  if (SRC == SRC_RANDOM) {
+  if (EST == EST_FORWARD) {
+    // every state starts per_state histories; the source tables are unused
+  } else if (SRC == SRC_RANDOM) {
     total = a.qcdf[n - 1];
     if (total == 0) return;
   } else {
@@ -238,6 +240,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   uint32_t beg = 0, len = 0;    // current domain row
   Real Wrow = 0;                // its weight W_i
   Real w = 0, wc = 0;
+  int start = 0;                // forward: start state
+  double fsum = 0.0;            // forward: the history's sum of w rhs[state]
   int64_t steps = 0;
   uint64_t rng = 0;
   uint64_t my_steps = 0;
@@ -268,7 +272,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           hid = qnext + rank;
           rng = history_state(key, (uint64_t)hid);
           double w0;
-          if (SRC == SRC_RANDOM) {
+          if (EST == EST_FORWARD) {
+            state = (int)(hid / a.per_state);
+            start = state;
+            fsum = 0.0;
+            w0 = 1.0;
+          } else if (SRC == SRC_RANDOM) {
             const uint64_t t = mul_u53(xorshift64(rng), total);
             state = upper_bound_u64(a.qcdf, n, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
@@ -304,7 +313,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       len = hd.y;
       Wrow = Row<Real>::W(hd);
     }
-    if (EST == EST_EXPECTED_VALUE) {
+    if (EST == EST_FORWARD) {
+      fsum = fsum + (double)w * a.rhs[state];
+    } else if (EST == EST_EXPECTED_VALUE) {
       for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
     } else if constexpr (TILE) {
       const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
@@ -359,6 +370,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     }
     if (end) {
       alive = false;
+      if (EST == EST_FORWARD) tally_add(&tally[start], (Acc)fsum);
       if (REC) {
         for (int k = (int)steps + 1; k <= a.max_len; ++k) path[k] = -1;
         a.wfinal[hid - a.first] = (double)w;
@@ -395,6 +407,7 @@ hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
 
 template <typename Real>
 hipError_t launch_walk_p(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if (c.estimator == EST_FORWARD) return launch_walk_t<Real, EST_FORWARD, SRC_RANDOM>(a, c, s);
   if (c.estimator == EST_EXPECTED_VALUE) {
     return c.source == SRC_STRATIFIED ? launch_walk_t<Real, EST_EXPECTED_VALUE, SRC_STRATIFIED>(a, c, s)
                                       : launch_walk_t<Real, EST_EXPECTED_VALUE, SRC_RANDOM>(a, c, s);
@@ -647,6 +660,24 @@ __global__ void k_scatter_transpose(int n, const int* __restrict__ rp, const int
       t_src[p] = r;
       double a = aval<AV>(val, k);
       t_val[p] = dinv ? a * dinv[r] : a;  // composite (PL A)_{rc}
+    }
+  }
+}
+
+template <typename AV>
+__global__ void k_copy_rows(int n, const int* __restrict__ rp, const int* __restrict__ col, const void* val,
+                            const double* __restrict__ dinv, int* __restrict__ t_rp, int* __restrict__ t_src,
+                            double* __restrict__ t_val, int* flags) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    t_rp[r] = rp[r];
+    if (r == n - 1) t_rp[n] = rp[n];
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int c = col[k];
+      const double a = aval<AV>(val, k);
+      if (!(fabs(a) <= DBL_MAX)) set_flag(flags, FLAG_NONFINITE, r);
+      if (c < 0 || c >= n) set_flag(flags, FLAG_BAD_COLUMN, r);
+      t_src[k] = c;
+      t_val[k] = dinv ? a * dinv[r] : a;  // composite (PL A)_{rc}
     }
   }
 }
@@ -1404,6 +1435,15 @@ hipError_t launch_scatter_transpose(int n, const int* rp, const int* col, const 
     k_scatter_transpose<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, fill, t_src, t_val);
   else
     k_scatter_transpose<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, fill, t_src, t_val);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_rows(int n, const int* rp, const int* col, const void* val, bool f32, const double* dinv,
+                            int* t_rp, int* t_src, double* t_val, int* flags, hipStream_t s) {
+  if (f32)
+    k_copy_rows<float><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, t_src, t_val, flags);
+  else
+    k_copy_rows<double><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, dinv, t_rp, t_src, t_val, flags);
   return hipGetLastError();
 }
 

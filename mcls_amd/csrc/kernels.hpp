@@ -64,7 +64,10 @@ struct DomainView {
   const float* lfac32;    // nnz
 };
 
-enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1 };
+// EST_FORWARD: the forward method on the forward domain (rows of H): history
+// h starts at state h / per_state with weight 1 and adds w rhs[state] to its
+// own sum at every collision; the sum goes to its start state at the end.
+enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1, EST_FORWARD = 2 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
 enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1, SAMPLER_LEARNED = 2 };
 
@@ -93,6 +96,7 @@ struct WalkArgs {
   double* wfinal;          // recording: final weight per history
   int max_len;
   int batch;               // history ids a wave takes per queue grab (multiple of 64)
+  int64_t per_state;       // forward method: histories per start state
   int tile_slots;          // LDS tally tile entries per wave (power of two), 0 = none
   int tile_shift;          // 32 - log2(tile_slots)
 };
@@ -179,6 +183,10 @@ hipError_t launch_count_cols(int n, const int* rp, const int* col, const void* v
 hipError_t launch_scatter_transpose(int n, const int* rp, const int* col, const void* val,
                                     bool f32, const double* dinv, const int* t_rp, int* fill,
                                     int* t_src, double* t_val, hipStream_t s);
+// Forward domain: rows of PL A as they are (t_rp = rp, t_src = col,
+// t_val = dinv_i a_ij), with the column and finiteness checks of count_cols.
+hipError_t launch_copy_rows(int n, const int* rp, const int* col, const void* val, bool f32, const double* dinv,
+                            int* t_rp, int* t_src, double* t_val, int* flags, hipStream_t s);
 hipError_t launch_domain_rows(int n, const int* t_rp, int* t_src, double* t_val, double omega,
                               double threshold, bool has_prec, int* keep, int* flags,
                               hipStream_t s);

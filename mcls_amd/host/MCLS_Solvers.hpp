@@ -4,8 +4,9 @@
 // as MCLS (paths relative to the MCLS source root):
 //   LinearProblem                     src/MCLS_LinearProblem.hpp:59-186
 //   SolverManager                     src/MCLS_SolverManager.hpp:60-113
-//   MonteCarloSolverManager<..,AdjointTag>  src/MCLS_MonteCarloSolverManager.hpp:67-220
-//   MCSASolverManager<..,AdjointTag>  src/MCLS_MCSASolverManager.hpp:70-181
+//   MonteCarloSolverManager<..,Tag>   src/MCLS_MonteCarloSolverManager.hpp:67-220
+//   MCSASolverManager<..,Tag>         src/MCLS_MCSASolverManager.hpp:70-181
+//   (Tag = AdjointTag or ForwardTag, the reference's two methods)
 //   MultiSetLinearProblem (1 set)     src/MCLS_MultiSetLinearProblem.hpp
 //   SolverFactory                     src/MCLS_SolverFactory_impl.hpp:58-146
 //   TpetraPointJacobiPreconditioner   tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
@@ -13,8 +14,8 @@
 // rebuilt only when the operator object changes (pointer identity, as
 // src/MCLS_MonteCarloSolverManager_impl.hpp:161-178).  The left
 // preconditioner must be diagonal (point Jacobi); a right preconditioner,
-// the forward method, Anderson and the stand-alone fixed-point manager are
-// not provided by this build and raise MCLS::Assertion.
+// Anderson and the stand-alone fixed-point manager are not provided by this
+// build and raise MCLS::Assertion.
 #pragma once
 
 #include <cstring>
@@ -229,8 +230,9 @@ inline RCP<ParameterList> mcValidParameters() {
 
 template <class Vector, class Matrix, class MonteCarloTag = AdjointTag>
 class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
-  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value,
-                "only the adjoint Monte Carlo method is provided by this build");
+  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value || std::is_same<MonteCarloTag, ForwardTag>::value,
+                "MonteCarloTag is AdjointTag or ForwardTag");
+  static constexpr const char* kMethod = std::is_same<MonteCarloTag, ForwardTag>::value ? "Forward" : "Adjoint";
 
  public:
   using Problem = LinearProblem<Vector, Matrix>;
@@ -268,6 +270,7 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
   bool solve() override {
     MCLS_INSIST(d_problem, "no problem set");
     d_dev.sync(*d_problem, *d_plist);
+    GpuContext::check(mcls_gpu_set_param_str(d_dev.ctx(), "Monte Carlo Type", kMethod));
     Vector& x = *d_problem->getLHS();
     GpuContext::check(mcls_gpu_mc_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
     return true;  // a direct solve (src/MCLS_MonteCarloSolverManager.hpp:170-171)
@@ -290,8 +293,9 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
 
 template <class Vector, class Matrix, class MonteCarloTag = AdjointTag>
 class MCSASolverManager : public SolverManager<Vector, Matrix> {
-  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value,
-                "only the adjoint Monte Carlo method is provided by this build");
+  static_assert(std::is_same<MonteCarloTag, AdjointTag>::value || std::is_same<MonteCarloTag, ForwardTag>::value,
+                "MonteCarloTag is AdjointTag or ForwardTag");
+  static constexpr const char* kMethod = std::is_same<MonteCarloTag, ForwardTag>::value ? "Forward" : "Adjoint";
 
  public:
   using Problem = LinearProblem<Vector, Matrix>;
@@ -335,6 +339,7 @@ class MCSASolverManager : public SolverManager<Vector, Matrix> {
   bool solve() override {
     MCLS_INSIST(d_problem, "no problem set");
     d_dev.sync(*d_problem, *d_plist);
+    GpuContext::check(mcls_gpu_set_param_str(d_dev.ctx(), "Monte Carlo Type", kMethod));
     Vector& x = *d_problem->getLHS();
     GpuContext::check(mcls_gpu_mcsa_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
     d_stats = d_dev.stats();
@@ -377,8 +382,9 @@ class SolverFactory {
     MCLS_INSIST(params, "null parameter list");
     if (name == "Adjoint MC") return std::make_shared<MonteCarloSolverManager<Vector, Matrix, AdjointTag>>(params);
     if (name == "Adjoint MCSA") return std::make_shared<MCSASolverManager<Vector, Matrix, AdjointTag>>(params);
-    if (name == "Forward MC" || name == "Forward MCSA" || name == "Adjoint Anderson" ||
-        name == "Forward Anderson" || name == "Fixed Point")
+    if (name == "Forward MC") return std::make_shared<MonteCarloSolverManager<Vector, Matrix, ForwardTag>>(params);
+    if (name == "Forward MCSA") return std::make_shared<MCSASolverManager<Vector, Matrix, ForwardTag>>(params);
+    if (name == "Adjoint Anderson" || name == "Forward Anderson" || name == "Fixed Point")
       throw Assertion("Solver type '" + name + "' is not provided by this build");
     throw Assertion("Solver type not supported!");
   }

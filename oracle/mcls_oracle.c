@@ -142,32 +142,55 @@ int orc_jacobi_dinv(int n, const int64_t* a_rp, const int* a_col,
  * src/MCLS_AlmostOptimalDomain_impl.hpp:296-364:
  *   h_ij = delta_ij - omega*m_ij, drop |h| < DBL_EPSILON,
  *   W_i = sum_k |h_ik| (sequential), cdf_k = (partial sum)_k / W_i. */
-int64_t orc_build_domain(int n, const int64_t* a_rp, const int* a_col,
-                         const double* a_val, const double* dinv, double omega,
-                         double threshold, int64_t* d_rp, int* d_col,
-                         double* d_h, double* d_cdf, double* d_W) {
+/* Domain rows of H^T (transpose = 1: the adjoint method,
+ * getTransposeCompositeOperator) or of H (transpose = 0: the forward method,
+ * getCompositeOperator, src/MCLS_LinearProblem_impl.hpp:141-172). */
+static int64_t build_domain_impl(int n, const int64_t* a_rp, const int* a_col,
+                                 const double* a_val, const double* dinv,
+                                 double omega, double threshold, int transpose,
+                                 int64_t* d_rp, int* d_col, double* d_h,
+                                 double* d_cdf, double* d_W) {
   int64_t a_nnz = a_rp[n];
   int64_t* cnt = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
   int64_t* pos = (int64_t*)malloc(((size_t)n + 1) * sizeof(int64_t));
   int* tsrc = (int*)malloc((size_t)(a_nnz > 0 ? a_nnz : 1) * sizeof(int));
   double* tval = (double*)malloc((size_t)(a_nnz > 0 ? a_nnz : 1) * sizeof(double));
   int64_t ret = 0;
-  /* transpose by counting sort over columns; rows of A are visited in
-   * ascending order, so each transposed row is ascending in source row. */
   for (int64_t k = 0; k < a_nnz; ++k) {
     int c = a_col[k];
     if (c < 0 || c >= n) { ret = -1; goto done; }
-    cnt[c + 1]++;
   }
-  for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-  memcpy(pos, cnt, ((size_t)n + 1) * sizeof(int64_t));
-  for (int r = 0; r < n; ++r) {
-    for (int64_t k = a_rp[r]; k < a_rp[r + 1]; ++k) {
-      int c = a_col[k];
-      int64_t p = pos[c]++;
-      tsrc[p] = r;
-      /* composite entry (PL A)_{r c} = dinv_r * a_rc */
-      tval[p] = dinv ? a_val[k] * dinv[r] : a_val[k];
+  if (transpose) {
+    /* transpose by counting sort over columns; rows of A are visited in
+     * ascending order, so each transposed row is ascending in source row. */
+    for (int64_t k = 0; k < a_nnz; ++k) cnt[a_col[k] + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    memcpy(pos, cnt, ((size_t)n + 1) * sizeof(int64_t));
+    for (int r = 0; r < n; ++r) {
+      for (int64_t k = a_rp[r]; k < a_rp[r + 1]; ++k) {
+        int c = a_col[k];
+        int64_t p = pos[c]++;
+        tsrc[p] = r;
+        /* composite entry (PL A)_{r c} = dinv_r * a_rc */
+        tval[p] = dinv ? a_val[k] * dinv[r] : a_val[k];
+      }
+    }
+  } else {
+    /* rows of PL A, each ordered by column (insertion sort) */
+    for (int i = 0; i <= n; ++i) cnt[i] = a_rp[i];
+    for (int r = 0; r < n; ++r) {
+      for (int64_t k = a_rp[r]; k < a_rp[r + 1]; ++k) {
+        int c = a_col[k];
+        double v = dinv ? a_val[k] * dinv[r] : a_val[k];
+        int64_t q = k - 1;
+        while (q >= a_rp[r] && tsrc[q] > c) {
+          tsrc[q + 1] = tsrc[q];
+          tval[q + 1] = tval[q];
+          --q;
+        }
+        tsrc[q + 1] = c;
+        tval[q + 1] = v;
+      }
     }
   }
   {
@@ -215,6 +238,23 @@ done:
   free(tsrc);
   free(tval);
   return ret;
+}
+
+int64_t orc_build_domain(int n, const int64_t* a_rp, const int* a_col,
+                         const double* a_val, const double* dinv, double omega,
+                         double threshold, int64_t* d_rp, int* d_col,
+                         double* d_h, double* d_cdf, double* d_W) {
+  return build_domain_impl(n, a_rp, a_col, a_val, dinv, omega, threshold, 1,
+                           d_rp, d_col, d_h, d_cdf, d_W);
+}
+
+int64_t orc_build_domain_forward(int n, const int64_t* a_rp, const int* a_col,
+                                 const double* a_val, const double* dinv,
+                                 double omega, double threshold, int64_t* d_rp,
+                                 int* d_col, double* d_h, double* d_cdf,
+                                 double* d_W) {
+  return build_domain_impl(n, a_rp, a_col, a_val, dinv, omega, threshold, 0,
+                           d_rp, d_col, d_h, d_cdf, d_W);
 }
 
 /* Vose's alias method per domain row (M. D. Vose, "A linear algorithm for
@@ -385,6 +425,7 @@ typedef struct {
   const uint64_t* ncdf;  /* stratified source */
   uint64_t M;
   int64_t first, last;   /* history range */
+  int64_t per_state;     /* forward: histories started in each state */
   double* x;             /* private tally, n entries */
   int64_t steps;
   /* optional path recording */
@@ -401,7 +442,14 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
   uint64_t s = orc_history_state(p->seed, J->solve_index, (uint64_t)hid);
   int state;
   double w0;
-  if (p->source == ORC_SRC_STRATIFIED) {
+  const int forward = p->estimator == ORC_EST_FORWARD;
+  double acc = 0.0; /* forward: sum of w b[state] along the history */
+  if (forward) {
+    /* UniformForwardSource::getHistory (src/MCLS_UniformForwardSource_impl.hpp:
+     * 115-150): per_state consecutive histories per state, weight 1 */
+    state = (int)(hid / J->per_state);
+    w0 = 1.0;
+  } else if (p->source == ORC_SRC_STRATIFIED) {
     int64_t k = orc_upper_bound_u64(J->ncdf, d->n, (uint64_t)hid);
     uint64_t nk = J->ncdf[k] - (k ? J->ncdf[k - 1] : 0);
     state = (int)k;
@@ -409,6 +457,7 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
   } else {
     state = orc_source_birth(d->n, J->b, J->q, J->qcdf, J->Q, &s, &w0);
   }
+  const int start = state;
   int64_t L = p->history_length;
   int64_t steps = 0;
   int plen = 0;
@@ -419,7 +468,10 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
     for (;;) {
       int64_t beg = d->rp[state];
       int len = (int)(d->rp[state + 1] - beg);
-      if (p->estimator == ORC_EST_EXPECTED_VALUE) {
+      if (forward) {
+        /* ForwardTally::tallyHistory (src/MCLS_ForwardTally.hpp:128-136) */
+        acc = acc + (double)w * J->b[state];
+      } else if (p->estimator == ORC_EST_EXPECTED_VALUE) {
         for (int k = 0; k < len; ++k) x[d->col[beg + k]] += (double)(w * d->h32[beg + k]);
       } else {
         x[state] += (double)w;
@@ -449,7 +501,9 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
     for (;;) {
       int64_t beg = d->rp[state];
       int len = (int)(d->rp[state + 1] - beg);
-      if (p->estimator == ORC_EST_EXPECTED_VALUE) {
+      if (forward) {
+        acc = acc + w * J->b[state];
+      } else if (p->estimator == ORC_EST_EXPECTED_VALUE) {
         for (int k = 0; k < len; ++k) x[d->col[beg + k]] += w * d->h[beg + k];
       } else {
         x[state] += w;
@@ -474,6 +528,8 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
     }
     if (wfinal) *wfinal = w;
   }
+  /* ForwardTally::postProcessHistory: the history's sum goes to its start */
+  if (forward) x[start] += acc;
   if (path)
     for (int i = plen; i < max_len + 1; ++i) path[i] = -1;
   return steps;
@@ -554,6 +610,13 @@ static void domain_view_free(domain_view* d) {
   free(d->W32);
 }
 
+/* Forward source: ceil(num_histories / n) histories per state, at least one
+ * (UniformForwardSource: Sample Ratio rounded up per state). */
+int64_t orc_forward_per_state(int n, int64_t num_histories) {
+  if (n <= 0 || num_histories <= 0) return 1;
+  return (num_histories + n - 1) / n;
+}
+
 /* Shared driver for orc_mc_solve and orc_record_paths. */
 static int run_walks(int n, const int64_t* d_rp, const int* d_col,
                      const double* d_h, const double* d_cdf, const double* d_W,
@@ -571,14 +634,20 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
   uint64_t Q = orc_source_quantize(n, b, q, qcdf);
   uint64_t nh = (uint64_t)p->num_histories;
   uint64_t M = 0;
-  if (p->source == ORC_SRC_STRATIFIED) {
+  const int forward = p->estimator == ORC_EST_FORWARD;
+  int64_t per_state = 1;
+  if (forward) {
+    per_state = orc_forward_per_state(n, p->num_histories);
+    nh = (uint64_t)n * (uint64_t)per_state;
+    Q = 1; /* every state starts histories, whatever b is */
+  } else if (p->source == ORC_SRC_STRATIFIED) {
     M = stratified_counts(n, q, Q, nh, ncdf);
     nh = M;
   }
   int64_t total_steps = 0;
   if (paths) {
     walk_job J = {&d, b, p, solve_index, q, qcdf, Q, ncdf, M,
-                  rec_first, rec_first + rec_count, NULL, 0, max_len, paths, weights};
+                  rec_first, rec_first + rec_count, per_state, NULL, 0, max_len, paths, weights};
     double* scratch = (double*)calloc((size_t)n, sizeof(double));
     J.x = scratch;
     if (Q) walk_worker(&J);
@@ -594,7 +663,7 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
       walk_job J = {&d, b, p, solve_index, q, qcdf, Q, ncdf, M,
                     (int64_t)(nh * (uint64_t)t / (uint64_t)nt),
                     (int64_t)(nh * (uint64_t)(t + 1) / (uint64_t)nt),
-                    NULL, 0, 0, NULL, NULL};
+                    per_state, NULL, 0, 0, NULL, NULL};
       J.x = (double*)calloc((size_t)n, sizeof(double));
       jobs[t] = J;
     }
@@ -610,9 +679,11 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
     }
     free(jobs);
     free(th);
-    /* src/MCLS_AdjointTally_impl.hpp:69-72: scale by 1/nh. */
+    /* src/MCLS_AdjointTally_impl.hpp:69-72: scale by 1/nh; forward:
+     * average over the histories of each start state
+     * (src/MCLS_ForwardTally_impl.hpp:174-222) */
     if (nh > 0) {
-      double inv = 1.0 / (double)nh;
+      double inv = 1.0 / (double)(forward ? (uint64_t)per_state : nh);
       for (int i = 0; i < n; ++i) x[i] = x[i] * inv;
     }
     if (p->estimator == ORC_EST_EXPECTED_VALUE)
@@ -842,8 +913,13 @@ int orc_mcsa_solve(int n, const int64_t* rp, const int* col, const double* val,
   set_job* sets = (set_job*)calloc((size_t)nsets, sizeof(set_job));
   pthread_t* th = (pthread_t*)calloc((size_t)nsets, sizeof(pthread_t));
   int rc = 0;
-  int64_t dnnz = orc_build_domain(n, rp, col, val, dinv, sp->neumann_omega,
-                                  sp->threshold, d_rp, d_col, d_h, d_cdf, d_W);
+  const int forward = sp->mc.estimator == ORC_EST_FORWARD;
+  if (forward && sp->reference_mode) { rc = -3; goto out; }
+  int64_t dnnz = forward
+                     ? orc_build_domain_forward(n, rp, col, val, dinv, sp->neumann_omega,
+                                                sp->threshold, d_rp, d_col, d_h, d_cdf, d_W)
+                     : orc_build_domain(n, rp, col, val, dinv, sp->neumann_omega,
+                                        sp->threshold, d_rp, d_col, d_h, d_cdf, d_W);
   if (dnnz < 0) { rc = (int)dnnz; goto out; }
   for (int s = 0; s < nsets; ++s) {
     sets[s].n = n;

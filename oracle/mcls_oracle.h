@@ -52,6 +52,15 @@ int64_t orc_build_domain(int n, const int64_t* a_rp, const int* a_col,
                          const double* a_val, const double* dinv, double omega,
                          double threshold, int64_t* d_rp, int* d_col,
                          double* d_h, double* d_cdf, double* d_W);
+/* The forward method's domain: rows of H = I - omega PL A (not transposed),
+ * each ordered by column, with the same filter and drop
+ * (src/MCLS_MonteCarloSolverManager_impl.hpp:243-248,
+ * src/MCLS_LinearProblem_impl.hpp:141-172). */
+int64_t orc_build_domain_forward(int n, const int64_t* a_rp, const int* a_col,
+                                 const double* a_val, const double* dinv,
+                                 double omega, double threshold, int64_t* d_rp,
+                                 int* d_col, double* d_h, double* d_cdf,
+                                 double* d_W);
 
 /* Point-Jacobi inverse diagonal: dinv_i = 1/a_ii.  Returns 0, or -(i+1)
  * for the first row with a zero / missing diagonal. */
@@ -85,7 +94,12 @@ int orc_build_proposal(int n, const int64_t* d_rp, const int* d_col,
                        double* lfac);
 
 /* ------------------------------------------------------------- MC solve -- */
-enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1 };
+/* ORC_EST_FORWARD: the forward method (ForwardTag) on the forward domain:
+ * each state starts orc_forward_per_state(n, num_histories) histories of
+ * weight 1, a history adds w b[state] at every collision to its own sum,
+ * and x[start] is the average of its histories' sums; the source type is
+ * ignored. */
+enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1, ORC_EST_FORWARD = 2 };
 enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
 enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2 };
 
@@ -109,6 +123,8 @@ int orc_mc_solve(int n, const int64_t* d_rp, const int* d_col,
                  const double* d_h, const double* d_cdf, const double* d_W,
                  const double* b, const orc_mc_params* p,
                  uint64_t solve_index, double* x, int64_t* steps_out);
+
+int64_t orc_forward_per_state(int n, int64_t num_histories);
 
 /* Record the state-index path of histories [first, first+count): paths is
  * count x (max_len+1) ints, padded with -1; weights the final weight. */

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
 EST_COLLISION = 0
 EST_EXPECTED_VALUE = 1
+EST_FORWARD = 2
 SRC_RANDOM = 0
 SRC_STRATIFIED = 1
 SAMPLER_CDF = 0
@@ -100,6 +101,10 @@ def lib():
     L.orc_build_domain.argtypes = [i32, _i64p, _i32p, _f64p, vp, f64, f64,
                                    _i64p, _i32p, _f64p, _f64p, _f64p]
     L.orc_build_domain.restype = i64
+    L.orc_build_domain_forward.argtypes = L.orc_build_domain.argtypes
+    L.orc_build_domain_forward.restype = i64
+    L.orc_forward_per_state.argtypes = [i32, i64]
+    L.orc_forward_per_state.restype = i64
     L.orc_build_proposal.argtypes = [i32, _i64p, _i32p, _f64p, _f32p, _f64p, _f64p]
     L.orc_build_proposal.restype = i32
     L.orc_build_alias.argtypes = [i32, _i64p, _f64p, _f64p, _f64p, _i32p]
@@ -201,7 +206,9 @@ def jacobi_dinv(A):
     return d
 
 
-def build_domain(A, dinv=None, omega=1.0, threshold=0.0):
+def build_domain(A, dinv=None, omega=1.0, threshold=0.0, forward=False):
+    """Adjoint domain (rows of H^T), or with forward=True the forward
+    method's domain (rows of H)."""
     nnz = int(A.rp[-1])
     cap = max(nnz, 1)
     rp = np.zeros(A.n + 1, np.int64)
@@ -210,8 +217,8 @@ def build_domain(A, dinv=None, omega=1.0, threshold=0.0):
     cdf = np.zeros(cap)
     W = np.zeros(A.n)
     dv = None if dinv is None else np.ascontiguousarray(dinv, np.float64)
-    got = lib().orc_build_domain(A.n, A.rp, A.col, A.val, _ptr(dv), omega, threshold,
-                                 rp, col, h, cdf, W)
+    fn = lib().orc_build_domain_forward if forward else lib().orc_build_domain
+    got = fn(A.n, A.rp, A.col, A.val, _ptr(dv), omega, threshold, rp, col, h, cdf, W)
     if got < 0:
         raise ValueError({-1: "column out of range", -2: "duplicate column"}.get(got, got))
     return Domain(A.n, rp, col[:got].copy(), h[:got].copy(), cdf[:got].copy(), W)

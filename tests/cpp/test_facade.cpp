@@ -111,6 +111,39 @@ static void test_mc_adjoint() {
   std::printf("ok MonteCarloSolverManager.adjoint tol=%.3f\n", solver.achievedTol());
 }
 
+// tpetra/test/tstTpetraMonteCarloSolverManager.cpp:360-467 (forward):
+// signs of the forward MC estimate for sources -1, 2, 2 (after setProblem), -2
+static void test_mc_forward() {
+  const int n = 10;
+  auto A = build_operator(n, 1.0, -0.14);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, -1.0);
+  auto plist = parameterList();
+  plist->set<double>("Sample Ratio", 10.0);
+  plist->set<int>("History Length", 10);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  MonteCarloSolverManager<VectorType, MatrixType, ForwardTag> solver(problem, plist, 0);
+  CHECK(solver.solve() && solver.getConvergedStatus() && solver.getNumIters() == 0);
+  CHECK(all_sign(*x, -1.0));
+  VT::putScalar(*const_cast<VectorType*>(b.get()), 2.0);
+  CHECK(solver.solve() && all_sign(*x, 1.0));
+  solver.setProblem(problem);
+  CHECK(solver.solve() && all_sign(*x, 1.0));
+  VT::putScalar(*const_cast<VectorType*>(b.get()), -2.0);
+  CHECK(solver.solve() && all_sign(*x, -1.0));
+  // forward MCSA converges like the adjoint one
+  auto mplist = parameterList();
+  mplist->set<double>("Convergence Tolerance", 1.0e-8);
+  mplist->set<int>("Maximum Iterations", 20);
+  mplist->set<int>("Random Seed", 12345);
+  VT::putScalar(*x, 0.0);
+  MCSASolverManager<VectorType, MatrixType, ForwardTag> mcsa(problem, mplist);
+  CHECK(mcsa.solve() && mcsa.getNumIters() < 20 && mcsa.achievedTol() <= 1e-8);
+  CHECK(all_sign(*x, -1.0));
+  std::printf("ok MonteCarloSolverManager.forward, MCSASolverManager.forward iters=%d\n", mcsa.getNumIters());
+}
+
 static void test_factory_and_errors() {
   SolverFactory<VectorType, MatrixType> factory;
   auto plist = parameterList();
@@ -120,9 +153,12 @@ static void test_factory_and_errors() {
   auto mcsa = factory.create("Adjoint MCSA", plist);
   auto mc = factory.create("Adjoint MC", plist);
   CHECK(mcsa && mc);
+  auto fmcsa = factory.create("Forward MCSA", plist);
+  auto fmc = factory.create("Forward MC", plist);
+  CHECK(fmcsa && fmc);
   bool threw = false;
   try {
-    factory.create("Forward MCSA", plist);
+    factory.create("Adjoint Anderson", plist);
   } catch (const Assertion&) {
     threw = true;
   }
@@ -237,6 +273,7 @@ int main() {
   test_mcsa_adjoint(false);
   test_mcsa_adjoint(true);
   test_mc_adjoint();
+  test_mc_forward();
   test_factory_and_errors();
   test_mcsa_importance();
   std::printf("ALL %d checks passed\n", g_checks);

@@ -43,7 +43,8 @@ def _setup(ctx, prob, f32, jacobi=True, params=None):
     base = {"Neumann Relaxation": 1.0, "Composite Operator Threshold": 0.0,
             "Walk Precision": "FP64", "Source Sampling Type": "Random",
             "Estimator Type": "Collision", "Weight Cutoff": 0.0, "History Length": 10,
-            "Number of Histories": 0, "Random Seed": 0, "Transition Sampler": "CDF"}
+            "Number of Histories": 0, "Random Seed": 0, "Transition Sampler": "CDF",
+            "Monte Carlo Type": "Adjoint"}
     base.update(params or {})
     ctx.set_params(base)
 
@@ -209,3 +210,77 @@ def test_alias_tables_bitexact(ctx):
     op, ow = po.record_paths(dom, rhs, po.mc_params(3000, 40, 0.0, 64, seed=5, sampler=po.SAMPLER_ALIAS),
                              0, 3000, 40)
     assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+
+
+# ------------------------------------------------------------ forward method
+@pytest.mark.parametrize("kind", ["poisson", "diff3d", "s27", "aniso"])
+@pytest.mark.parametrize("omega,thr", [(1.0, 0.0), (0.7, 0.05)])
+def test_forward_domain_bitexact(ctx, kind, omega, thr):
+    prob, f32 = _problem(kind)
+    _setup(ctx, prob, f32, params={"Neumann Relaxation": omega, "Composite Operator Threshold": thr,
+                                   "Monte Carlo Type": "Forward"})
+    ctx.build_domain()
+    rp, col, h, cdf, W = ctx.domain()
+    A = _csr(prob, f32)
+    dom = po.build_domain(A, po.jacobi_dinv(A), omega, thr, forward=True)
+    assert np.array_equal(rp, dom.rp) and np.array_equal(col, dom.col)
+    assert np.array_equal(h, dom.h) and np.array_equal(cdf, dom.cdf) and np.array_equal(W, dom.W)
+
+
+@pytest.mark.parametrize("sampler", ["CDF", "Alias"])
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("kind", ["poisson", "s27"])
+def test_forward_paths_and_tally_match_oracle(ctx, prec, kind, sampler):
+    prob, f32 = _problem(kind)
+    L, cutoff, nh = 200, 1e-3, 3 * prob.n
+    _setup(ctx, prob, f32, params={"Monte Carlo Type": "Forward", "Walk Precision": f"FP{prec}",
+                                   "History Length": L, "Weight Cutoff": cutoff, "Number of Histories": nh,
+                                   "Random Seed": 99, "Transition Sampler": sampler})
+    A = _csr(prob, f32)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv, forward=True)
+    rhs = prob.b * dinv
+    rhs[::5] *= -1.0
+    mp = po.mc_params(nh, L, cutoff, prec, po.EST_FORWARD, seed=99,
+                      sampler=po.SAMPLER_ALIAS if sampler == "Alias" else po.SAMPLER_CDF)
+    gp, gw = ctx.record_paths(rhs, 7, 0, 1500, L)
+    op, ow = po.record_paths(dom, rhs, mp, 0, 1500, L, solve_index=7)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    assert np.array_equal(gp[:, 0], np.arange(1500) // 3)  # three histories per state
+    xg = ctx.mc_solve_raw(rhs, 8)
+    st = ctx.stats()
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=8)
+    assert st.steps == steps and st.histories == nh
+    tol = 1e-12 if prec == 64 else 1e-6
+    assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
+
+
+def test_forward_mcsa_converges(ctx):
+    prob = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 12, seed=3, p0=0.6, p1=0.9, p2=0.8, p3=1.2))
+    _setup(ctx, prob, False, params={"Monte Carlo Type": "Forward", "History Length": 500, "Weight Cutoff": 1e-3,
+                                     "Number of Histories": 4 * prob.n, "Convergence Tolerance": 1e-8,
+                                     "Maximum Iterations": 60, "Random Seed": 5})
+    x = ctx.mcsa_solve(prob.b)
+    st = ctx.stats()
+    assert st.converged == 1 and st.achieved_tol <= 1e-8
+    S = sp.csr_matrix((prob.values(), prob.col, prob.rp), shape=(prob.n, prob.n))
+    xd = sla.spsolve(S.tocsc(), prob.b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    A = _csr(prob)
+    mp = po.mc_params(4 * prob.n, 500, 1e-3, 64, po.EST_FORWARD, seed=5)
+    xo, res, _ = po.mcsa_solve(A, prob.b, dinv=po.jacobi_dinv(A),
+                               params=po.solver_params(tol=1e-8, max_iters=60, mc=mp),
+                               solve_index0=st.solve_index - st.iterations)
+    assert res.converged == 1 and abs(res.iters - st.iterations) <= 2
+    ctx.set_params({"Monte Carlo Type": "Adjoint"})
+
+
+def test_forward_rejects_expected_value(ctx):
+    prob, f32 = _problem("poisson")
+    _setup(ctx, prob, f32, params={"Monte Carlo Type": "Forward", "Estimator Type": "Expected Value",
+                                   "Number of Histories": prob.n})
+    with pytest.raises(capi.GpuError):
+        ctx.mc_solve_raw(prob.b, 0)
+    with pytest.raises(capi.GpuError):
+        ctx.set_params({"Monte Carlo Type": "Sideways"})
+    ctx.set_params({"Monte Carlo Type": "Adjoint", "Estimator Type": "Collision"})

@@ -156,3 +156,69 @@ def test_mcsa_oracle_converges_to_direct_solution():
     x2, res2, _ = po.mcsa_solve(Ac, b, dinv=dinv, params=sp)
     assert res2.converged == 1
     assert np.linalg.norm(x2 - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("sampler", [po.SAMPLER_CDF, po.SAMPLER_ALIAS])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_forward_estimator_expectation(sampler, prec):
+    """Forward method (rows of H, histories from every state, each history's
+    sum of w b[state] averaged at its start state): E[x] = sum_{k<L} H^k b;
+    walking H^T instead (the adjoint domain) moves the mean away."""
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    omega = 0.9
+    dom = po.build_domain(Ac, dinv, omega, forward=True)
+    H = np.eye(n) - omega * dinv[:, None] * A
+    L = 6
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(L))
+    mp = po.mc_params(250 * n, L, 0.0, prec, po.EST_FORWARD, seed=11 + prec + sampler, sampler=sampler)
+    mean, se = _replicas(dom, b, mp, 60)
+    _check_z(mean, se, exact)
+    wrong = sum(np.linalg.matrix_power(H.T, k) @ b for k in range(L))
+    assert np.max(np.abs(mean - wrong) / np.maximum(se, 1e-300)) > 8
+
+
+def test_forward_domain_is_rows_of_h():
+    A, b = _system()
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv, 0.8, forward=True)
+    M = A * dinv[:, None]  # (PL A)_ij = a_ij / a_ii, rounded as the build does
+    H = -(0.8 * M)
+    H[np.diag_indices_from(H)] = 1.0 - 0.8 * np.diag(M)
+    dense = np.zeros_like(H)
+    for i in range(len(b)):
+        for k in range(dom.rp[i], dom.rp[i + 1]):
+            dense[i, dom.col[k]] = dom.h[k]
+        assert np.all(np.diff(dom.col[dom.rp[i]:dom.rp[i + 1]]) > 0)  # ascending columns
+    H[np.abs(H) < np.finfo(float).eps] = 0.0
+    assert np.array_equal(dense, H)
+    # and the adjoint domain is its transpose
+    adj = po.build_domain(Ac, dinv, 0.8)
+    dense_t = np.zeros_like(H)
+    for i in range(len(b)):
+        for k in range(adj.rp[i], adj.rp[i + 1]):
+            dense_t[i, adj.col[k]] = adj.h[k]
+    assert np.array_equal(dense_t, dense.T)
+
+
+def test_forward_source_and_one_step_tally():
+    """tstTpetraUniformForwardSource (histories = N x ceil(Sample Ratio),
+    emitted state by state, weight 1) and tstTpetraForwardTally (a history
+    that collides once tallies w b at its start state)."""
+    assert po.lib().orc_forward_per_state(10, 10) == 1
+    assert po.lib().orc_forward_per_state(10, 25) == 3
+    assert po.lib().orc_forward_per_state(10, 0) == 1
+    A = po.Csr.from_dense(np.array([[1.0, -0.25, 0.0], [-0.25, 1.0, -0.25], [0.0, -0.25, 1.0]]))
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv, forward=True)
+    b = np.array([3.0, -2.0, 0.5])
+    mp = po.mc_params(3 * 4, 1, 0.0, 64, po.EST_FORWARD, seed=9)
+    x, steps = po.mc_solve(dom, b, mp)
+    assert np.array_equal(x, b) and steps == 12
+    paths, w = po.record_paths(dom, b, mp, 0, 12, 1)
+    assert np.array_equal(paths[:, 0], np.repeat(np.arange(3), 4))
+    # one transition: w = W_start sign(h) with W = 0.25, 0.5, 0.25 and h > 0
+    assert np.array_equal(w, np.repeat([0.25, 0.5, 0.25], 4))
