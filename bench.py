@@ -56,10 +56,14 @@ WORKLOADS = {
 FORMS = {
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                "Estimator Type": "Collision", "Weight Cutoff": 1e-2, "History Length": 1000,
-               "Transition Sampler": "Alias"},
+               "Transition Sampler": "Alias", "Monte Carlo Type": "Adjoint"},
     "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random",
               "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000,
-              "Transition Sampler": "CDF"},
+              "Transition Sampler": "CDF", "Monte Carlo Type": "Adjoint"},
+    # the forward method with the chosen form's walk (no per-step tally atomics)
+    "forward": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
+                "Estimator Type": "Collision", "Weight Cutoff": 1e-2, "History Length": 1000,
+                "Transition Sampler": "Alias", "Monte Carlo Type": "Forward"},
 }
 
 
@@ -77,7 +81,7 @@ def solve_once(ctx, b):
     return dt, ctx.stats()
 
 
-def bytes_per_step(prec, z, sampler):
+def bytes_per_step(prec, z, sampler, forward=False):
     """Algorithmic bytes of one walk step (one tally + one transition).
 
     CDF:   16 (packed row header: start, len, W) + p z (CDF row) + 4 (packed
@@ -87,6 +91,8 @@ def bytes_per_step(prec, z, sampler):
     p = 4 (FP32 walk) or 8 (FP64 walk); z = mean domain row length.
     """
     p = 4 if prec == "FP32" else 8
+    if forward:  # the tally is one register add of w b[state]: read 8 bytes of b instead
+        return (32 if (sampler == "Alias" and p == 4) else 16 + (16 if sampler == "Alias" else p * z + 4)) + 8
     if sampler == "Alias":
         return 32 + 2 * p if p == 4 else 16 + 16 + 2 * p
     return 16 + p * z + 4 + 2 * p
@@ -150,6 +156,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--form", default="chosen", choices=sorted(FORMS))
     ap.add_argument("--no-tune", action="store_true", help="skip the walk autotuner (default launch shape)")
+    ap.add_argument("--no-forward", action="store_true", help="skip the forward-method solve")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -209,7 +216,8 @@ def main():
     total = sum(times)
     hps = hist / total
     prec = FORMS[args.form]["Walk Precision"]
-    bstep = bytes_per_step(prec, z, FORMS[args.form]["Transition Sampler"])
+    bstep = bytes_per_step(prec, z, FORMS[args.form]["Transition Sampler"],
+                           FORMS[args.form].get("Monte Carlo Type") == "Forward")
     avg_launch_s = walk_ms / 1e3 / max(launches, 1)
     steps_per_launch = steps / max(launches, 1)
     achieved = bstep * steps_per_launch / avg_launch_s / 1e9
@@ -258,6 +266,23 @@ def main():
                         * st.steps / (st.ms_walk / 1e3) / 1e9
                         / HBM_PEAK_GBS, "tuning": ptune}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
+        if not args.no_forward:
+            # the forward method with the chosen form's walk (reported beside
+            # the headline, which stays the adjoint solve)
+            ctx.set_params(FORMS["forward"])
+            ftune = None
+            if not args.no_tune:
+                ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
+                ftune = ctx.autotune(prob.b)
+            solve_once(ctx, prob.b)  # forward domain build and warm-up
+            fdt, fst = solve_once(ctx, prob.b)
+            log(f"forward FP32: {fdt:.3f}s iters={fst.iterations} tol={fst.achieved_tol:.2e}")
+            fbytes = bytes_per_step("FP32", z, "Alias", True)
+            out["forward"] = {"time_to_tol_s": fdt, "mcsa_iterations": fst.iterations,
+                              "achieved_tol": fst.achieved_tol, "histories_per_s": fst.histories / fdt,
+                              "walk_steps_per_s": fst.steps / (fst.ms_walk / 1e3),
+                              "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
+                              "bytes_per_step": fbytes, "speedup_vs_plain": dt / fdt, "tuning": ftune}
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline")

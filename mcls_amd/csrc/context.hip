@@ -701,7 +701,7 @@ int mcsa_iteration(mcls_gpu_context* c, bool timed, const uint64_t* key_dev, uin
     HIP_TRY(mcls::launch_batch_check(c->nsys, c->sys_r2.p, c->sys_b2.p, p.tol, c->sys_conv.p, c->sys_unconv.p, s));
     HIP_TRY(hipMemcpyAsync(c->host_scal + kHostUnconv, c->sys_unconv.p, sizeof(int), hipMemcpyDeviceToHost, s));
   }
-  if (p.source == mcls::SRC_STRATIFIED)
+  if (p.source == mcls::SRC_STRATIFIED && !p.forward)
     HIP_TRY(hipMemcpyAsync(c->host_scal + kHostM, c->ncdf.p + (n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   if (timed) HIP_TRY(hipEventRecord(c->ev[7], s));
   return MCLS_OK;
@@ -1306,8 +1306,8 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
       HIP_TRY(hipEventElapsedTime(&b2, c->ev[4], c->ev[7]));
       st.ms_residual += a + b2;
     }
-    int64_t hd = nh;
-    if (p.source == mcls::SRC_STRATIFIED) {
+    int64_t hd = nh;  // the forward source's count is nh exactly
+    if (p.source == mcls::SRC_STRATIFIED && !p.forward) {
       uint64_t M = 0;
       std::memcpy(&M, c->host_scal + kHostM, sizeof(M));
       hd = (int64_t)M;
@@ -1456,8 +1456,8 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
       best.batch = batches[k];
     }
   }
-  // LDS tally tile (collision estimator): off, 128 or 512 entries per wave
-  if (saved.estimator == mcls::EST_COLLISION) {
+  // LDS tally tile (adjoint collision estimator): off, 128 or 512 entries per wave
+  if (saved.estimator == mcls::EST_COLLISION && !saved.forward) {
     const int64_t tiles[3] = {0, 128, 512};
     const int64_t shape_tile = best.tally_tile;
     for (int k = 0; k < 3 && rc == MCLS_OK; ++k) {

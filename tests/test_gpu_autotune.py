@@ -19,14 +19,16 @@ def _poisson(ctx, sampler, prec):
     return prob, prob.b * ctx.get_left_prec_diag()
 
 
-@pytest.mark.parametrize("sampler,prec", [("Alias", 32), ("CDF", 64)])
-def test_autotune_choice_and_shape_independence(ctx, sampler, prec):
+@pytest.mark.parametrize("sampler,prec,method", [("Alias", 32, "Adjoint"), ("CDF", 64, "Adjoint"),
+                                                ("Alias", 32, "Forward")])
+def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
     prob, rhs = _poisson(ctx, sampler, prec)
+    ctx.set_params({"Monte Carlo Type": method})
     before = ctx.stats().solve_index
     t = ctx.autotune(prob.b, pilot_histories=1 << 16)
     print("tuning", t)
     assert ctx.stats().solve_index == before
-    assert t["candidates"] == 15
+    assert t["candidates"] == (15 if method == "Adjoint" else 13)  # no tally tile for forward walks
     assert t["tally_tile"] in (0, 128, 512)
     assert t["block"] in (64, 128, 256) and t["batch"] in (64, 128, 256, 512)
     assert t["blocks_per_cu"] * t["block"] // 64 in (8, 16, 32)
@@ -40,7 +42,8 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec):
         x = ctx.mc_solve_raw(rhs, 6)
         tol = 1e-12 if prec == 64 else 1e-4  # FP32 atomics: summation order
         assert np.linalg.norm(x - ref_x) <= tol * np.linalg.norm(ref_x)
-    ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
+    ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0,
+                    "Monte Carlo Type": "Adjoint"})
 
 
 def test_walk_batch_validation(ctx):
