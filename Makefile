@@ -15,7 +15,7 @@ CXXFLAGS := -O2 -std=c++17 -fPIC -Wall -Iinclude
 GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip $(SRC)/mlp.hip
 GPU_HDRS := $(SRC)/kernels.hpp $(SRC)/mlp.hpp include/mcls_gpu.h include/mcls_mlp.h
 
-all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade oracle
+all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade tests/cpp/bin/test_mm oracle
 
 $(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
 	mkdir -p $(LIBDIR)
@@ -29,6 +29,11 @@ tests/cpp/bin/test_facade: tests/cpp/test_facade.cpp mcls_amd/host/*.hpp include
 	mkdir -p tests/cpp/bin
 	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_facade.cpp -L$(LIBDIR) -lmcls_gpu \
 	    -L/opt/rocm/lib -Wl,-rpath,'$$ORIGIN/../../../$(LIBDIR)' -Wl,-rpath,/opt/rocm/lib
+
+# MatrixMarket reader driver: host-only, no GPU library
+tests/cpp/bin/test_mm: tests/cpp/test_mm.cpp mcls_amd/host/MCLS_MatrixMarket.hpp mcls_amd/host/MCLS_Core.hpp
+	mkdir -p tests/cpp/bin
+	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_mm.cpp
 
 oracle:
 	$(MAKE) -C oracle

@@ -5,11 +5,13 @@
 //   tpetra/test/tstTpetraMonteCarloSolverManager.cpp:73-197  (adjoint MC)
 //   src/MCLS_SolverFactory_impl.hpp:58-146                   (names)
 // Prints one line per check and exits non-zero on the first failure.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
+#include "MCLS_MatrixMarket.hpp"
 #include "MCLS_Solvers.hpp"
 
 using namespace MCLS;
@@ -144,6 +146,36 @@ static void test_mc_forward() {
   std::printf("ok MonteCarloSolverManager.forward, MCSASolverManager.forward iters=%d\n", mcsa.getNumIters());
 }
 
+// A system read from MatrixMarket files (run from the repository root).
+static void test_matrix_market_system() {
+  auto A = readMatrixMarket<double>("tests/data/mm/symmetric.mtx");
+  auto b = readMatrixMarketVector("tests/data/mm/vector.mtx");
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-10);
+  plist->set<int>("Maximum Iterations", 40);
+  plist->set<double>("Sample Ratio", 200.0);
+  plist->set<int>("Random Seed", 1);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  PointJacobiPreconditioner<double> pj;
+  pj.setOperator(A);
+  pj.buildPreconditioner();
+  problem->setLeftPrec(pj.getLeftPreconditioner());
+  MCSASolverManager<VectorType, MatrixType, AdjointTag> solver(problem, plist);
+  CHECK(solver.solve());
+  // residual of the unpreconditioned system
+  double r2 = 0.0, b2 = 0.0;
+  for (int64_t i = 0; i < A->numRows(); ++i) {
+    double ax = 0.0;
+    for (int64_t k = A->rowPtr()[(size_t)i]; k < A->rowPtr()[(size_t)i + 1]; ++k)
+      ax += A->values()[(size_t)k] * (*x)[A->colInd()[(size_t)k]];
+    r2 += ((*b)[i] - ax) * ((*b)[i] - ax);
+    b2 += (*b)[i] * (*b)[i];
+  }
+  CHECK(std::sqrt(r2 / b2) <= 1e-9);
+  std::printf("ok MatrixMarket system iters=%d\n", solver.getNumIters());
+}
+
 static void test_factory_and_errors() {
   SolverFactory<VectorType, MatrixType> factory;
   auto plist = parameterList();
@@ -275,6 +307,7 @@ int main() {
   test_mc_adjoint();
   test_mc_forward();
   test_factory_and_errors();
+  test_matrix_market_system();
   test_mcsa_importance();
   std::printf("ALL %d checks passed\n", g_checks);
   return 0;
