@@ -116,30 +116,25 @@ def measured_traffic(workload, form, steps_per_launch):
 
 
 def cpu_baseline(prob, w, plain_iterations=None, nh_sample=1 << 24):
-    """The oracle's restatement of the MCLS algorithm (reference semantics:
-    FP64 CDF lower_bound, one Xorshift stream per set, LIFO emission, sets
-    summed / numSets) on the same operator, one set per host thread, with a
-    bounded history sample: one MCSA iteration of nh_sample histories.
-    Histories/s from its MC transport time; per-iteration time and time to
-    tolerance at the workload's history count are extrapolated from it."""
-    from oracle import pyoracle as po
+    """The oracle's restatement of the MCLS algorithm in reference semantics
+    (oracle/cpu_baseline.py: FP64 CDF lower_bound, one Xorshift stream per
+    set, LIFO emission, sets summed / numSets) on the same operator, one set
+    per host thread, with a bounded history sample: one MCSA iteration of
+    nh_sample histories.  Histories/s from its MC transport time; the
+    per-iteration time and the time to tolerance at the workload's history
+    count are extrapolated from it."""
+    from oracle import cpu_baseline as cb
 
-    A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64) if w["f32"] else prob.values())
-    dinv = po.jacobi_dinv(A)
-    cores = min(16, os.cpu_count() or 1)
     nh_sample = min(nh_sample, w["histories"])
-    mp = po.mc_params(nh_sample // cores, 1000, 1e-4, 64, seed=1)
-    sp = po.solver_params(tol=1e-30, max_iters=1, num_sets=cores, reference_mode=1, mc=mp)
-    x, res, _ = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
-    hps = res.histories / res.seconds_mc
-    per_iter = w["histories"] / hps
+    r = cb.run(prob, w["f32"], nh_sample, iters=1, workload_histories=w["histories"])
     iters = plain_iterations or 7
-    return {"value": hps, "unit": "histories/s", "cores": cores, "kind": "port",
+    per_iter = r["extrapolated"]["seconds_mc_per_iteration"]
+    return {"value": r["histories_per_s"], "unit": "histories/s", "cores": r["cores"], "kind": "port",
             "sample": f"1 MCSA iteration of the MCLS-semantics CPU restatement (FP64, CDF lower_bound, "
-                      f"{cores} sets x {nh_sample // cores} histories, weight cutoff 1e-4) on the same "
+                      f"{r['sets']} sets x {r['histories_per_set']} histories, weight cutoff 1e-4) on the same "
                       f"operator ({prob.n} unknowns); MC transport time only",
-            "seconds_mc": res.seconds_mc,
-            "steps_per_history": res.steps / max(res.histories, 1),
+            "seconds_mc": r["seconds_mc"],
+            "steps_per_history": r["steps_per_history"],
             "seconds_per_iteration_extrapolated": per_iter,
             "time_to_tol_extrapolated_s": iters * per_iter,
             "extrapolation": f"{w['histories']} histories per iteration x {iters} iterations "

@@ -1,0 +1,108 @@
+"""CPU baseline harness: the oracle's restatement of MCLS's MCSA in the
+reference's semantics, timed on this host's cores.
+
+TEST AND MEASUREMENT INFRASTRUCTURE: it times the CPU restatement the GPU
+path is compared with; nothing in the product imports it.
+
+Reference semantics (oracle/mcls_oracle.c, orc_mcsa_solve with
+reference_mode = 1): FP64, per-row CDF inverted by lower_bound, one Xorshift
+stream per set, histories emitted from the highest state down, collision
+tally; "one MPI rank per core" is the reference's multi-set mode
+(MultiSetLinearProblem): `cores` sets, one thread each, every set with the
+whole domain and nh / cores histories, the set corrections summed and divided
+by the number of sets (src/MCLS_MCSASolverManager_impl.hpp:374-377).
+`--quirk` reproduces the reference's signed b[0] seed of the source CDF
+(src/MCLS_UniformAdjointSource_impl.hpp:185).
+
+Without --full, K MCSA iterations are timed and the per-iteration time and the
+time to tolerance at the workload's history count are extrapolated (and
+labelled so); with --full the solve runs to the tolerance or to K iterations.
+
+    python -m oracle.cpu_baseline --workload cfg4 --histories 16777216 --iters 1
+    python -m oracle.cpu_baseline --workload cfg0 --full --iters 200
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+if __package__ in (None, ""):
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from oracle import pyoracle as po  # noqa: E402
+
+# name: (problem kind, grid, spec parameters, FP32 values, histories per iteration, weight cutoff)
+WORKLOADS = {
+    "cfg0": ("POISSON2D_5PT", (100, 100), dict(seed=0), False, 10_000, 1e-4),
+    "cfg1": ("DIFFUSION3D_7PT", (128,), dict(seed=1, p0=8), False, 1 << 24, 1e-4),
+    "cfg3": ("VARCOEF2D_5PT", (256, 256), dict(seed=3, p0=0.1, p1=1.0, p2=128), False, 16 * 256 * 256, 1e-4),
+    "cfg4": ("STENCIL3D_27PT", (256,), dict(seed=4, p0=0.6, p1=0.9, p2=0.8, p3=1.2), True, 1 << 28, 1e-4),
+    "cfg4-64": ("STENCIL3D_27PT", (64,), dict(seed=4, p0=0.6, p1=0.9, p2=0.8, p3=1.2), True, 16 * 64 ** 3, 1e-4),
+}
+
+
+def make_problem(name, nthreads=None):
+    from mcls_amd import capi  # the problem generators (host-only library)
+
+    kind, grid, kw, f32, _, _ = WORKLOADS[name]
+    spec = capi.make_spec(getattr(capi, kind), *grid, **kw)
+    return capi.build_problem(spec, want64=not f32, want32=f32, nthreads=nthreads)
+
+
+def run(prob, f32, histories, cores=None, iters=1, tol=1e-8, cutoff=1e-4, history_length=1000, quirk=False,
+        full=False, workload_histories=None, seed=1):
+    """Time the reference-semantics MCSA on `prob`.  Returns a dict: histories/s
+    over the MC transport time, per-iteration time, steps per history,
+    residual history, convergence and (without `full`) extrapolations to
+    `workload_histories` per iteration."""
+    cores = int(cores or min(16, os.cpu_count() or 1))
+    A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64) if f32 else prob.values())
+    dinv = po.jacobi_dinv(A)
+    per_set = max(1, int(histories) // cores)
+    mp = po.mc_params(per_set, history_length, cutoff, 64, seed=seed)
+    sp = po.solver_params(tol=tol if full else 1e-300, max_iters=iters, num_sets=cores, reference_mode=1, mc=mp,
+                          ref_quirk=int(quirk))
+    x, res, hist = po.mcsa_solve(A, prob.b, dinv=dinv, params=sp)
+    hps = res.histories / max(res.seconds_mc, 1e-12)
+    out = {"histories_per_s": hps, "cores": cores, "sets": cores, "histories_per_set": per_set,
+           "histories": int(res.histories), "iterations": int(res.iters), "converged": bool(res.converged), "achieved_tol": float(res.achieved_tol),
+           "seconds_mc": res.seconds_mc, "seconds_total": res.seconds_total,
+           "seconds_mc_per_iteration": res.seconds_mc / max(res.iters, 1),
+           "steps_per_history": res.steps / max(res.histories, 1),
+           "residual_history": [float(v) for v in hist], "reference_quirk": bool(quirk)}
+    if not full and workload_histories:
+        per_iter = workload_histories / hps
+        out["extrapolated"] = {"histories_per_iteration": int(workload_histories),
+                               "seconds_mc_per_iteration": per_iter}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", default="cfg4-64", choices=sorted(WORKLOADS))
+    ap.add_argument("--cores", type=int, default=None, help="sets = threads (default min(16, cores))")
+    ap.add_argument("--histories", type=int, default=None, help="histories per MCSA iteration (default: the workload's)")
+    ap.add_argument("--iters", type=int, default=1, help="MCSA iterations (the cap with --full)")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--full", action="store_true", help="run to the tolerance instead of extrapolating")
+    ap.add_argument("--quirk", action="store_true", help="the reference's signed b[0] source CDF seed")
+    ap.add_argument("--time-to-tol-iterations", type=int, default=None,
+                    help="iteration count to extrapolate the time to tolerance with")
+    args = ap.parse_args(argv)
+    kind, grid, kw, f32, wl_hist, cutoff = WORKLOADS[args.workload]
+    prob = make_problem(args.workload)
+    hist = args.histories or wl_hist
+    out = run(prob, f32, hist, args.cores, args.iters, args.tol, cutoff, quirk=args.quirk, full=args.full,
+              workload_histories=wl_hist)
+    out["workload"] = args.workload
+    out["n"] = int(prob.n)
+    if "extrapolated" in out and args.time_to_tol_iterations:
+        out["extrapolated"]["time_to_tol_s"] = args.time_to_tol_iterations * out["extrapolated"]["seconds_mc_per_iteration"]
+        out["extrapolated"]["iterations"] = args.time_to_tol_iterations
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -879,6 +879,7 @@ typedef struct {
   const double* W;
   const double* rhs;
   orc_mc_params mc;
+  int quirk;
   uint64_t rng;
   double* x;
   int64_t steps;
@@ -888,7 +889,7 @@ typedef struct {
 static void* ref_set_worker(void* arg) {
   set_job* S = (set_job*)arg;
   S->nh = orc_ref_mc_solve(S->n, S->rp, S->col, S->h, S->cdf, S->W, S->rhs,
-                           &S->mc, S->mc.source == ORC_SRC_STRATIFIED, 0,
+                           &S->mc, S->mc.source == ORC_SRC_STRATIFIED, S->quirk,
                            &S->rng, S->x, &S->steps);
   return NULL;
 }
@@ -930,6 +931,7 @@ int orc_mcsa_solve(int n, const int64_t* rp, const int* col, const double* val,
     sets[s].W = d_W;
     sets[s].rhs = rhs;
     sets[s].mc = sp->mc;
+    sets[s].quirk = (int)sp->ref_quirk;
     /* one persistent stream per set, as the reference's per-rank PRNG */
     sets[s].rng = orc_splitmix64(sp->mc.seed ^ orc_splitmix64((uint64_t)s + 1));
     if (!sets[s].rng) sets[s].rng = 0x9E3779B97F4A7C15ULL;

@@ -157,6 +157,7 @@ typedef struct orc_solver_params {
   int64_t num_sets;         /* multi-set replication (>= 1)  */
   int64_t reference_mode;   /* 1 = MCLS reference MC semantics, 0 = build */
   orc_mc_params mc;
+  int64_t ref_quirk;        /* reference mode: 1 = the signed b[0] CDF seed */
 } orc_solver_params;
 
 typedef struct orc_solver_result {

@@ -55,6 +55,7 @@ class SolverParams(C.Structure):
         ("num_sets", C.c_int64),
         ("reference_mode", C.c_int64),
         ("mc", MCParams),
+        ("ref_quirk", C.c_int64),
     ]
 
 
@@ -285,10 +286,11 @@ def source_quantize(b):
 
 def solver_params(tol=1e-8, max_iters=1000, check_freq=1, smoother_steps=1,
                   richardson_omega=1.0, neumann_omega=1.0, threshold=0.0, num_sets=1,
-                  reference_mode=0, mc=None):
+                  reference_mode=0, mc=None, ref_quirk=0):
     return SolverParams(float(tol), int(max_iters), int(check_freq), int(smoother_steps),
                         float(richardson_omega), float(neumann_omega), float(threshold),
-                        int(num_sets), int(reference_mode), mc if mc is not None else mc_params(1))
+                        int(num_sets), int(reference_mode), mc if mc is not None else mc_params(1),
+                        int(ref_quirk))
 
 
 def mcsa_solve(A, b, x0=None, dinv=None, params=None, solve_index0=0):
