@@ -99,8 +99,8 @@ def bytes_per_step(prec, z, sampler, forward=False):
 
 
 def measured_traffic(workload, form, steps_per_launch):
-    """HBM bytes per launch of the dominant walk kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload and form
+    """HBM bytes per launch of a form's walk kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload
     (calibrated against a copy kernel), when its transitions per launch match
     this run's to 2%; None otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"walk_{workload}_traffic.json")
@@ -108,8 +108,9 @@ def measured_traffic(workload, form, steps_per_launch):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if rec.get("workload") != workload or rec.get("form") != form:
+    if rec.get("workload") != workload or form not in rec.get("forms", {}):
         return None
+    rec = rec["forms"][form]
     if abs(rec["steps_per_launch"] - steps_per_launch) > 0.02 * steps_per_launch:
         return None
     return float(rec["traffic_bytes_per_launch"])
@@ -273,7 +274,9 @@ def main():
             fdt, fst = solve_once(ctx, prob.b)
             log(f"forward FP32: {fdt:.3f}s iters={fst.iterations} tol={fst.achieved_tol:.2e}")
             fbytes = bytes_per_step("FP32", z, "Alias", True)
+            fsteps_per_launch = fst.steps / max(fst.walk_launches, 1)
             out["forward"] = {"time_to_tol_s": fdt, "mcsa_iterations": fst.iterations,
+                              "traffic": measured_traffic(args.workload, "forward", fsteps_per_launch),
                               "achieved_tol": fst.achieved_tol, "histories_per_s": fst.histories / fdt,
                               "walk_steps_per_s": fst.steps / (fst.ms_walk / 1e3),
                               "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
