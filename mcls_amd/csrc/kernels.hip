@@ -233,6 +233,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   constexpr bool FUSED = SAMP == SAMPLER_ALIAS && sizeof(Real) == 4;
   constexpr bool CARRY = SAMP != SAMPLER_ALIAS || FUSED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
+  int slo = 0, shi = 0;         // stratified: birth states of the current batch
   bool exhausted = false;       // wave-uniform
   bool alive = false;
   int64_t hid = 0;
@@ -263,6 +264,17 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             exhausted = true;
             break;
           }
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD) {
+            // the batch's ids are consecutive: bound its birth states once,
+            // so each birth searches only that window of the count table
+            int lo = 0, hi = 0;
+            if (lane == leader) {
+              lo = upper_bound_u64(a.ncdf, n, (uint64_t)qnext);
+              hi = upper_bound_u64(a.ncdf, n, (uint64_t)(qend - 1));
+            }
+            slo = __shfl(lo, leader);
+            shi = __shfl(hi, leader);
+          }
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
@@ -282,7 +294,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             state = upper_bound_u64(a.qcdf, n, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
           } else {
-            state = upper_bound_u64(a.ncdf, n, (uint64_t)hid);
+            // first k with ncdf[k] > hid lies in [slo, shi] for this batch
+            state = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
             const uint64_t nk = a.ncdf[state] - (state ? a.ncdf[state - 1] : 0);
             w0 = a.rhs[state] * (totald / (double)nk);
           }
