@@ -91,6 +91,24 @@ __device__ __forceinline__ int lower_bound_cdf(const Real* __restrict__ cdf, int
   return lo;
 }
 
+// The same index as lower_bound_cdf for a non-decreasing cdf whose last entry
+// is 1 (u in [0, 1)), found from the guess g = floor(u n): cdf[g - 1] and
+// cdf[g] are fetched together, which settles a near-uniform row in one round
+// trip; otherwise a bisection runs on the side of g the two values leave
+// open.  `overlap` runs between issuing the two loads and using them.
+template <typename Real, typename F>
+__device__ __forceinline__ int lower_bound_guess(const Real* __restrict__ cdf, int n, Real u, F&& overlap) {
+  int g = (int)(u * (Real)n);
+  g = g < n - 1 ? g : n - 1;
+  const Real c1 = cdf[g];
+  const Real c0r = cdf[g > 0 ? g - 1 : 0];
+  overlap();
+  const Real c0 = g > 0 ? c0r : (Real)-1;
+  if (c1 < u) return g + 1 + lower_bound_cdf<Real>(cdf + g + 1, n - g - 1, u);
+  if (c0 >= u) return lower_bound_cdf<Real>(cdf, g - 1, u);
+  return g;
+}
+
 __device__ __forceinline__ void tally_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void tally_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
@@ -119,12 +137,44 @@ template <> struct Row<double> {
   __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr64[i]; }
   __device__ static uint32_t target(const DomainView& d, uint32_t k, uint32_t& beg, uint32_t& len, double& W);
   __device__ static double W(uint4 h) { return __hiloint2double((int)h.w, (int)h.z); }
-  // alias draw: slot from the high 32 bits, keep-or-alias from the low 32
-  __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r) {
+  // alias draw: slot from the high 32 bits, keep-or-alias from the low 32;
+  // `overlap` runs between issuing the slot load and using it
+  template <typename F>
+  __device__ static uint32_t pick(const DomainView& d, uint32_t start, uint32_t len, uint64_t r, F&& overlap) {
     const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
     const uint4 s = d.slot64[start + slot];
+    overlap();
     const double u2 = (double)(uint32_t)r * 0x1.0p-32;
     return u2 < __hiloint2double((int)s.y, (int)s.x) ? s.z : s.w;
+  }
+  // CDF draw (lower_bound of u in the row's CDF) and transition from the
+  // target records, whose b.zw hold each entry's CDF value: the guess
+  // g = floor(u len) and its left neighbour come in one round trip, and when
+  // they bracket u the guess's record is also the transition.  Otherwise a
+  // bisection over cdf64 finds the index on the side they leave open.
+  template <typename F>
+  __device__ static uint32_t cdf_step(const DomainView& d, uint32_t& beg, uint32_t& len, double& W, double u,
+                                      F&& overlap) {
+    const Rec32* __restrict__ t = d.tgt64 + beg;
+    const double* __restrict__ c = d.cdf64 + beg;
+    const int n = (int)len;
+    int g = (int)(u * (double)n);
+    g = g < n - 1 ? g : n - 1;
+    Rec32 r = t[g];
+    const uint2 l = *(const uint2*)&t[g > 0 ? g - 1 : 0].b.z;
+    overlap();
+    const double c1 = __hiloint2double((int)r.b.w, (int)r.b.z);
+    const double c0 = g > 0 ? __hiloint2double((int)l.y, (int)l.x) : -1.0;
+    int k = g;
+    if (c1 < u)
+      k = g + 1 + lower_bound_cdf<double>(c + g + 1, n - g - 1, u);
+    else if (c0 >= u)
+      k = lower_bound_cdf<double>(c, g - 1, u);
+    if (k != g) r = t[k];
+    beg = r.a.y;
+    len = r.a.z;
+    W = __hiloint2double((int)r.b.y, (int)r.b.x);
+    return r.a.x;
   }
 };
 template <> struct Row<float> {
@@ -305,6 +355,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           alive = true;
           if (CARRY) {
             const uint4 hd = Row<Real>::hdr(d, state);
+            // Wait for the header here, on the refill path.  Left to the
+            // first use after the refill, the wait would also cover the
+            // previous step's tally atomic on every step of every lane.
+            asm volatile("" ::"v"(hd.x), "v"(hd.y), "v"(hd.z), "v"(hd.w));
             beg = hd.x;
             len = hd.y;
             Wrow = Row<Real>::W(hd);
@@ -326,26 +380,34 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       len = hd.y;
       Wrow = Row<Real>::W(hd);
     }
-    if (EST == EST_FORWARD) {
-      fsum = fsum + (double)w * a.rhs[state];
-    } else if (EST == EST_EXPECTED_VALUE) {
-      for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
-    } else if constexpr (TILE) {
-      const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
-      int k = tkeys[slot];
-      if (k < 0) {
-        k = atomicCAS(&tkeys[slot], -1, state);
-        if (k < 0) k = state;
+    // The step's tally at the current state, with the weight it arrived
+    // with.  It is issued after the transition's first load, so that the
+    // atomic's round trip overlaps that load's instead of preceding it (the
+    // counter that tracks outstanding memory operations drains in order).
+    auto tally_step = [&]() {
+      if (EST == EST_FORWARD) {
+        fsum = fsum + (double)w * a.rhs[state];
+      } else if (EST == EST_EXPECTED_VALUE) {
+        for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
+      } else if constexpr (TILE) {
+        const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
+        int k = tkeys[slot];
+        if (k < 0) {
+          k = atomicCAS(&tkeys[slot], -1, state);
+          if (k < 0) k = state;
+        }
+        if (k == state)
+          atomicAdd(&tvals[slot], w);
+        else
+          tally_add(&tally[state], (Acc)w);
+      } else {
+        tally_add(&tally[state], w);
       }
-      if (k == state)
-        atomicAdd(&tvals[slot], w);
-      else
-        tally_add(&tally[state], (Acc)w);
-    } else {
-      tally_add(&tally[state], w);
-    }
+    };
     bool end = len == 0;  // absorbing row (all of h dropped)
-    if (!end) {
+    if (end) {
+      tally_step();
+    } else {
       const uint64_t r = xorshift64(rng);
       uint32_t cp;
       if constexpr (FUSED) {
@@ -353,6 +415,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         // 32; the record also holds both candidates' row headers
         const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
         const Rec32 sr = d.rec32[beg + slot];
+        tally_step();
         const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
         const bool keep = u2 < __uint_as_float(sr.a.x);
         cp = keep ? sr.a.y : sr.a.z;
@@ -362,15 +425,20 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         Wrow = (Real)__uint_as_float(keep ? sr.b.z : sr.b.w);
       } else if constexpr (SAMP == SAMPLER_LEARNED) {
         // importance-reshaped row: P = |h| g / S, weight factor S / g_j
-        const uint32_t k = beg + (uint32_t)lower_bound_cdf<Real>(Tables<Real>::lcdf(d) + beg, (int)len,
-                                                                 Tables<Real>::u01(r));
+        const uint32_t k = beg + (uint32_t)lower_bound_guess<Real>(Tables<Real>::lcdf(d) + beg, (int)len,
+                                                                   Tables<Real>::u01(r), tally_step);
         w = w * Tables<Real>::lfac(d)[k];
         cp = Row<Real>::target(d, k, beg, len, Wrow);
       } else if constexpr (SAMP == SAMPLER_ALIAS) {
-        cp = Row<Real>::pick(d, beg, len, r);
+        cp = Row<Real>::pick(d, beg, len, r, tally_step);
         w = w * Wrow;
+      } else if constexpr (sizeof(Real) == 8) {
+        const Real wn = w * Wrow;  // W of the current row; the tally still sees w
+        cp = Row<double>::cdf_step(d, beg, len, Wrow, Tables<Real>::u01(r), tally_step);
+        w = wn;
       } else {
-        const uint32_t k = beg + (uint32_t)lower_bound_cdf<Real>(cdf + beg, (int)len, Tables<Real>::u01(r));
+        const uint32_t k = beg + (uint32_t)lower_bound_guess<Real>(cdf + beg, (int)len, Tables<Real>::u01(r),
+                                                                   tally_step);
         w = w * Wrow;
         cp = Row<Real>::target(d, k, beg, len, Wrow);
       }
@@ -841,7 +909,8 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
 // Target records: for entry k, the packed column and that column's row header.
 __global__ void k_pack_targets(int64_t nnz, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
                                const double* __restrict__ W64, const float* __restrict__ W32,
-                               uint4* __restrict__ tgt32, Rec32* __restrict__ tgt64) {
+                               const double* __restrict__ cdf64, uint4* __restrict__ tgt32,
+                               Rec32* __restrict__ tgt64) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t c = col[k];
     const int j = (int)(c & 0x7FFFFFFFu);
@@ -851,7 +920,8 @@ __global__ void k_pack_targets(int64_t nnz, const int* __restrict__ d_rp, const 
     if (tgt64) {
       Rec32 r;
       r.a = make_uint4(c, (uint32_t)b, l, 0u);
-      r.b = make_uint4((uint32_t)__double2loint(W64[j]), (uint32_t)__double2hiint(W64[j]), 0u, 0u);
+      r.b = make_uint4((uint32_t)__double2loint(W64[j]), (uint32_t)__double2hiint(W64[j]),
+                       (uint32_t)__double2loint(cdf64[k]), (uint32_t)__double2hiint(cdf64[k]));
       tgt64[k] = r;
     }
   }
@@ -1343,9 +1413,10 @@ hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, co
 }
 
 hipError_t launch_pack_targets(int64_t nnz, const int* d_rp, const uint32_t* col, const double* W64,
-                               const float* W32, uint4* tgt32, Rec32* tgt64, hipStream_t s) {
+                               const float* W32, const double* cdf64, uint4* tgt32, Rec32* tgt64,
+                               hipStream_t s) {
   if (nnz <= 0) return hipSuccess;
-  k_pack_targets<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, d_rp, col, W64, W32, tgt32, tgt64);
+  k_pack_targets<<<grid_for(nnz, 256), 256, 0, s>>>(nnz, d_rp, col, W64, W32, cdf64, tgt32, tgt64);
   return hipGetLastError();
 }
 

@@ -56,7 +56,7 @@ struct DomainView {
   const Rec32* rec32;     // nnz (alias sampler, FP32 walks)
   const uint4* slot64;    // nnz (alias sampler, FP64 walks)
   const uint4* tgt32;     // nnz (CDF / learned samplers, FP32 walks)
-  const Rec32* tgt64;     // nnz (CDF / learned samplers, FP64 walks)
+  const Rec32* tgt64;     // nnz (CDF / learned samplers, FP64 walks; b.zw = cdf64)
   // learned proposal: P_ij = |h_ij| g_j / S_i (CDF), weight factor S_i / g_j
   const double* lcdf64;   // nnz
   const float* lcdf32;    // nnz
@@ -144,9 +144,11 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
                             const double* W64, const float* W32, uint4* hdr32, uint4* hdr64,
                             Rec32* rec32, uint4* slot64, bool alias, int* flags, hipStream_t s);
 
-// Target records (either output may be null).
+// Target records (either output may be null).  FP64 records also carry the
+// entry's own CDF value, so one load serves the search and the transition.
 hipError_t launch_pack_targets(int64_t nnz, const int* d_rp, const uint32_t* col, const double* W64,
-                               const float* W32, uint4* tgt32, Rec32* tgt64, hipStream_t s);
+                               const float* W32, const double* cdf64, uint4* tgt32, Rec32* tgt64,
+                               hipStream_t s);
 
 // ---------------------------------------------------------------- source
 // *key = splitmix64(seed ^ splitmix64(*counter)); ++*counter  (one thread)
