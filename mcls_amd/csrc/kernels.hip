@@ -148,29 +148,38 @@ template <> struct Row<double> {
     return u2 < __hiloint2double((int)s.y, (int)s.x) ? s.z : s.w;
   }
   // CDF draw (lower_bound of u in the row's CDF) and transition from the
-  // target records, whose b.zw hold each entry's CDF value: the guess
-  // g = floor(u len) and its left neighbour come in one round trip, and when
-  // they bracket u the guess's record is also the transition.  Otherwise a
-  // bisection over cdf64 finds the index on the side they leave open.
+  // target records, whose b.zw hold each entry's CDF value.  The search
+  // starts at the guess k = floor(u len): record k and the CDF value of
+  // k - 1 come in one round trip, and when they bracket u (cdf[k-1] < u <=
+  // cdf[k]) record k is also the transition.  Otherwise k moves one entry
+  // towards u per round trip (the lines are already cached), up to
+  // kCdfWalk moves, after which a bisection over cdf64 settles the rest.
   template <typename F>
   __device__ static uint32_t cdf_step(const DomainView& d, uint32_t& beg, uint32_t& len, double& W, double u,
                                       F&& overlap) {
+    constexpr int kCdfWalk = 3;
     const Rec32* __restrict__ t = d.tgt64 + beg;
-    const double* __restrict__ c = d.cdf64 + beg;
     const int n = (int)len;
-    int g = (int)(u * (double)n);
-    g = g < n - 1 ? g : n - 1;
-    Rec32 r = t[g];
-    const uint2 l = *(const uint2*)&t[g > 0 ? g - 1 : 0].b.z;
+    int k = (int)(u * (double)n);
+    k = k < n - 1 ? k : n - 1;
+    Rec32 r = t[k];
+    uint2 l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
     overlap();
-    const double c1 = __hiloint2double((int)r.b.w, (int)r.b.z);
-    const double c0 = g > 0 ? __hiloint2double((int)l.y, (int)l.x) : -1.0;
-    int k = g;
-    if (c1 < u)
-      k = g + 1 + lower_bound_cdf<double>(c + g + 1, n - g - 1, u);
-    else if (c0 >= u)
-      k = lower_bound_cdf<double>(c, g - 1, u);
-    if (k != g) r = t[k];
+    for (int move = 0;; ++move) {
+      const double ck = __hiloint2double((int)r.b.w, (int)r.b.z);
+      const double cl = k > 0 ? __hiloint2double((int)l.y, (int)l.x) : -1.0;
+      const bool right = ck < u;  // then k < n - 1, as cdf[n - 1] = 1 > u
+      if (!right && !(cl >= u)) break;
+      if (move == kCdfWalk) {
+        const double* __restrict__ c = d.cdf64 + beg;
+        k = right ? k + 1 + lower_bound_cdf<double>(c + k + 1, n - k - 1, u) : lower_bound_cdf<double>(c, k - 1, u);
+        r = t[k];
+        break;
+      }
+      k += right ? 1 : -1;  // cl >= u implies k > 0
+      r = t[k];
+      l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
+    }
     beg = r.a.y;
     len = r.a.z;
     W = __hiloint2double((int)r.b.y, (int)r.b.x);

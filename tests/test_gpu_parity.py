@@ -111,6 +111,51 @@ def test_paths_bitexact(ctx, prec, source, kind, sampler):
     assert (gp[:, 1] >= 0).mean() > 0.9  # walks actually move
 
 
+def _skewed_problem(n=400, per_row=40, seed=21):
+    """Rows of ~40 entries whose magnitudes span six decades, so that CDF
+    inversion lands far from the uniform guess floor(u len) on both sides."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(n), per_row)
+    cols = rng.integers(0, n, n * per_row)
+    vals = -(10.0 ** rng.uniform(-3.0, 3.0, n * per_row))
+    S = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    S = (S + sp.diags(np.abs(S).sum(axis=1).A1 * 1.25 + 1.0)).tocsr()
+    S.sort_indices()
+    return capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(np.float64),
+                        None, rng.random(n))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_paths_bitexact_skewed_rows(ctx, prec):
+    prob = _skewed_problem()
+    L, nh = 40, 4000
+    _setup(ctx, prob, False, params={"Walk Precision": f"FP{prec}", "History Length": L, "Weight Cutoff": 1e-6,
+                                     "Number of Histories": nh, "Random Seed": 99, "Transition Sampler": "CDF"})
+    A = _csr(prob)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    # the draws must reach every branch of the search: far left, far right
+    lens = np.diff(dom.rp)
+    assert lens.mean() > 30
+    rng = np.random.default_rng(0)
+    errs = []
+    for i in rng.integers(0, len(lens), 2000):
+        c, u = dom.cdf[dom.rp[i]:dom.rp[i + 1]], rng.random()
+        errs.append(int(np.searchsorted(c, u, side="left")) - min(int(u * len(c)), len(c) - 1))
+    errs = np.array(errs)
+    assert (errs <= -4).mean() > 0.05 and (errs >= 4).mean() > 0.05
+    rhs = prob.b * dinv
+    first, count = 0, 2000
+    gp, gw = ctx.record_paths(rhs, 3, first, count, L)
+    mp = po.mc_params(nh, L, 1e-6, prec, po.EST_COLLISION, po.SRC_RANDOM, seed=99, sampler=po.SAMPLER_CDF)
+    op, ow = po.record_paths(dom, rhs, mp, first, count, L, solve_index=3)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gw, ow)
+    assert (gp[:, 5] >= 0).mean() > 0.5
+
+
 @pytest.mark.parametrize("sampler", ["CDF", "Alias"])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("est", ["Collision", "Expected Value"])
