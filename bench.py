@@ -257,12 +257,16 @@ def main():
             log(f"autotune (plain): {ptune}")
         dt, st = solve_once(ctx, prob.b)
         log(f"plain FP64: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
+        pbytes = bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
+        psteps_per_launch = st.steps / max(st.walk_launches, 1)
         out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
                         "achieved_tol": st.achieved_tol, "histories_per_s": st.histories / dt,
                         "walk_steps_per_s": st.steps / (st.ms_walk / 1e3),
-                        "roofline_frac": bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
-                        * st.steps / (st.ms_walk / 1e3) / 1e9
-                        / HBM_PEAK_GBS, "tuning": ptune}
+                        "roofline_frac": pbytes * st.steps / (st.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_step": pbytes, "steps_per_launch": psteps_per_launch,
+                        "avg_launch_ms": st.ms_walk / max(st.walk_launches, 1),
+                        "traffic": measured_traffic(args.workload, "plain", psteps_per_launch),
+                        "tuning": ptune}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
         if not args.no_forward:
             # the forward method with the chosen form's walk (reported beside
@@ -282,7 +286,9 @@ def main():
                               "achieved_tol": fst.achieved_tol, "histories_per_s": fst.histories / fdt,
                               "walk_steps_per_s": fst.steps / (fst.ms_walk / 1e3),
                               "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
-                              "bytes_per_step": fbytes, "speedup_vs_plain": dt / fdt, "tuning": ftune}
+                              "bytes_per_step": fbytes, "steps_per_launch": fsteps_per_launch,
+                              "avg_launch_ms": fst.ms_walk / max(fst.walk_launches, 1),
+                              "speedup_vs_plain": dt / fdt, "tuning": ftune}
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline")

@@ -714,6 +714,84 @@ int orc_record_paths(int n, const int64_t* d_rp, const int* d_col,
                    NULL, first, count, max_len, paths, weights);
 }
 
+/* =================================================== composite operator == */
+
+void orc_transpose(int n, const int64_t* rp, const int* col, const double* val,
+                   int64_t* t_rp, int* t_col, double* t_val) {
+  /* counting sort over columns, rows visited in ascending order */
+  for (int i = 0; i <= n; ++i) t_rp[i] = 0;
+  for (int64_t k = 0; k < rp[n]; ++k) t_rp[col[k] + 1]++;
+  for (int i = 0; i < n; ++i) t_rp[i + 1] += t_rp[i];
+  int64_t* pos = (int64_t*)malloc(((size_t)n + 1) * sizeof(int64_t));
+  memcpy(pos, t_rp, ((size_t)n + 1) * sizeof(int64_t));
+  for (int r = 0; r < n; ++r) {
+    for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+      int64_t p = pos[col[k]]++;
+      t_col[p] = r;
+      t_val[p] = val[k];
+    }
+  }
+  free(pos);
+}
+
+static int cmp_int(const void* a, const void* b) {
+  int x = *(const int*)a, y = *(const int*)b;
+  return (x > y) - (x < y);
+}
+
+int64_t orc_spgemm(int n, const int64_t* x_rp, const int* x_col, const double* x_val,
+                   const int64_t* y_rp, const int* y_col, const double* y_val,
+                   double threshold, int64_t cap, int64_t* c_rp, int* c_col,
+                   double* c_val) {
+  /* dense accumulator per row: mark[j] == i once row i touched column j */
+  int* mark = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  double* acc = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  int* list = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int j = 0; j < n; ++j) mark[j] = -1;
+  int64_t out = 0;
+  c_rp[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    int cnt = 0;
+    for (int64_t p = x_rp[i]; p < x_rp[i + 1]; ++p) {
+      const int k = x_col[p];
+      const double a = x_val[p];
+      for (int64_t q = y_rp[k]; q < y_rp[k + 1]; ++q) {
+        const int j = y_col[q];
+        const double prod = a * y_val[q];
+        if (mark[j] != i) {
+          mark[j] = i;
+          acc[j] = prod;
+          list[cnt++] = j;
+        } else {
+          acc[j] = acc[j] + prod;
+        }
+      }
+    }
+    qsort(list, (size_t)cnt, sizeof(int), cmp_int);
+    double thr = 0.0;
+    if (cnt > 0) {
+      double mx = acc[list[0]];
+      for (int t = 1; t < cnt; ++t)
+        if (acc[list[t]] > mx) mx = acc[list[t]];
+      thr = mx * threshold;
+    }
+    for (int t = 0; t < cnt; ++t) {
+      const double v = acc[list[t]];
+      if (fabs(v) <= thr || v == 0.0) continue;
+      if (out >= cap) { out = -1; goto done; }
+      c_col[out] = list[t];
+      c_val[out] = v;
+      ++out;
+    }
+    c_rp[i + 1] = out;
+  }
+done:
+  free(mark);
+  free(acc);
+  free(list);
+  return out;
+}
+
 /* =============================================================== linalg == */
 
 void orc_spmv(int n, const int64_t* rp, const int* col, const double* val,

@@ -134,6 +134,29 @@ int orc_record_paths(int n, const int64_t* d_rp, const int* d_col,
                      uint64_t solve_index, int64_t first, int64_t count,
                      int max_len, int* paths, double* weights);
 
+/* ---------------------------------------------------- composite operator -- */
+/* Transpose of an n x n CSR matrix (MT::copyTranspose,
+ * tpetra/src/MCLS_TpetraCrsMatrixAdapter.hpp:416-422): each transposed row
+ * lists its entries in ascending source row.  t_rp holds n+1 entries, t_col /
+ * t_val rp[n]. */
+void orc_transpose(int n, const int64_t* rp, const int* col, const double* val,
+                   int64_t* t_rp, int* t_col, double* t_val);
+
+/* C = filter(X Y, threshold) (MT::multiply,
+ * tpetra/src/MCLS_TpetraCrsMatrixAdapter.hpp:388-399, and
+ * TpetraMatrixHelpers::filter, tpetra/src/MCLS_TpetraHelpers.hpp:196-263).
+ * Row-by-row Gustavson product: for row i, the entries k of X's row in
+ * ascending column order, then the entries j of Y's row k; C_ij starts at the
+ * first product and adds the later ones in that order.  Rows of C are in
+ * ascending column order.  The filter drops, per row, every entry with
+ * |c| <= (signed row max) * threshold, and every exact zero (the reference
+ * removes zero values with std::remove).  X and Y rows must be ascending.
+ * Returns nnz(C), or -1 when it would exceed cap. */
+int64_t orc_spgemm(int n, const int64_t* x_rp, const int* x_col, const double* x_val,
+                   const int64_t* y_rp, const int* y_col, const double* y_val,
+                   double threshold, int64_t cap, int64_t* c_rp, int* c_col,
+                   double* c_val);
+
 /* --------------------------------------------------------------- linalg -- */
 void orc_spmv(int n, const int64_t* rp, const int* col, const double* val,
               const double* x, double* y);

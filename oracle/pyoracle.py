@@ -124,6 +124,11 @@ def lib():
     L.orc_record_paths.restype = i32
     L.orc_spmv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p, _f64p]
     L.orc_spmv.restype = None
+    L.orc_transpose.argtypes = [i32, _i64p, _i32p, _f64p, _i64p, _i32p, _f64p]
+    L.orc_transpose.restype = None
+    L.orc_spgemm.argtypes = [i32, _i64p, _i32p, _f64p, _i64p, _i32p, _f64p, f64, i64,
+                             _i64p, _i32p, _f64p]
+    L.orc_spgemm.restype = i64
     L.orc_prec_residual.argtypes = [i32, _i64p, _i32p, _f64p, vp, _f64p, _f64p, _f64p]
     L.orc_prec_residual.restype = None
     for nm in ("orc_norm2", "orc_norm1", "orc_norminf"):
@@ -362,3 +367,141 @@ def ref_mc_solve(dom, b, params, stratified=False, ref_quirk=False, rng_state=1)
                                 C.byref(params), int(stratified), int(ref_quirk), C.byref(st),
                                 x, C.byref(steps))
     return x, int(nh), int(steps.value)
+
+
+# ------------------------------------------------------ composite operator --
+def transpose(A):
+    """MT::copyTranspose: rows in ascending source row."""
+    nnz = int(A.rp[-1])
+    t_rp = np.zeros(A.n + 1, np.int64)
+    t_col = np.zeros(max(nnz, 1), np.int32)
+    t_val = np.zeros(max(nnz, 1))
+    lib().orc_transpose(A.n, A.rp, A.col, A.val, t_rp, t_col, t_val)
+    return Csr(t_rp, t_col[:nnz], t_val[:nnz], A.n)
+
+
+def spgemm(X, Y, threshold=0.0):
+    """MT::multiply(X, false, Y, false, threshold): Gustavson product in the
+    order of X's row entries, then the row filter (mcls_oracle.h)."""
+    ylen = np.diff(Y.rp)
+    cap = int(ylen[X.col].sum()) if len(X.col) else 0
+    c_rp = np.zeros(X.n + 1, np.int64)
+    c_col = np.zeros(max(cap, 1), np.int32)
+    c_val = np.zeros(max(cap, 1))
+    got = lib().orc_spgemm(X.n, X.rp, X.col, X.val, Y.rp, Y.col, Y.val, float(threshold), cap,
+                           c_rp, c_col, c_val)
+    if got < 0:
+        raise RuntimeError("spgemm capacity")
+    return Csr(c_rp, c_col[:got], c_val[:got], X.n)
+
+
+def composite(A, PL=None, PR=None, threshold=0.0, forward=False):
+    """LinearProblem::getCompositeOperator (forward) and
+    getTransposeCompositeOperator (adjoint), src/MCLS_LinearProblem_impl.hpp:139-221:
+    the same products in the same order, each filtered.  Without a
+    preconditioner: A (forward) or A^T (adjoint), unfiltered."""
+    if forward:
+        if PL is not None and PR is not None:
+            return spgemm(PL, spgemm(A, PR, threshold), threshold)
+        if PL is not None:
+            return spgemm(PL, A, threshold)
+        if PR is not None:
+            return spgemm(A, PR, threshold)
+        return A
+    AT = transpose(A)
+    if PL is not None and PR is not None:
+        return spgemm(transpose(PR), spgemm(AT, transpose(PL), threshold), threshold)
+    if PR is not None:
+        return spgemm(transpose(PR), AT, threshold)
+    if PL is not None:
+        return spgemm(AT, transpose(PL), threshold)
+    return AT
+
+
+def diag_csr(d):
+    n = len(d)
+    return Csr(np.arange(n + 1), np.arange(n), np.asarray(d, np.float64), n)
+
+
+def build_domain_general(A, PL=None, PR=None, omega=1.0, threshold=0.0, forward=False):
+    """Domain rows from the composite operator with general (sparse) left and
+    right preconditioners: the rows of the (transposed, for the adjoint
+    method) composite, with h = delta - omega m and the DBL_EPSILON drop
+    (src/MCLS_AlmostOptimalDomain_impl.hpp:296-364); the composite's filter
+    has already run inside its products."""
+    if PL is None and PR is None:
+        return build_domain(A, None, omega, threshold, forward)
+    Cm = composite(A, PL, PR, threshold, forward)
+    return build_domain(Cm, None, omega, 0.0, forward=True)
+
+
+class PrecSystem:
+    """A with a left preconditioner (sparse PL, or diagonal dinv) and an
+    optional sparse right preconditioner PR (src/MCLS_LinearProblem_impl.hpp)."""
+
+    def __init__(self, A, PL=None, PR=None, dinv=None):
+        self.A, self.PL, self.PR = A, PL, PR
+        self.dinv = None if dinv is None else np.asarray(dinv, np.float64)
+
+    def left(self, v):
+        if self.PL is not None:
+            return spmv(self.PL, v)
+        return v * self.dinv if self.dinv is not None else np.array(v, np.float64)
+
+    def right(self, v):
+        return spmv(self.PR, v) if self.PR is not None else np.array(v, np.float64)
+
+    def prec_residual(self, b, y):
+        """updatePrecResidual (:411-442): PL (b - A PR y)."""
+        return self.left(np.asarray(b, np.float64) - spmv(self.A, self.right(y)))
+
+    def left_csr(self):
+        if self.PL is not None:
+            return self.PL
+        return None if self.dinv is None else diag_csr(self.dinv)
+
+    def domain(self, omega=1.0, threshold=0.0, forward=False):
+        return build_domain_general(self.A, self.left_csr(), self.PR, omega, threshold, forward)
+
+
+def mc_solve_general(sys, b, params, omega=1.0, threshold=0.0, forward=False, solve_index=0):
+    """MonteCarloSolverManager::solve with preconditioners
+    (src/MCLS_MonteCarloSolverManager_impl.hpp:200-233, 266-380):
+    source omega PL b, walk over the composite's domain, then x = PR x."""
+    dom = sys.domain(omega, threshold, forward)
+    x, steps = mc_solve(dom, sys.left(b) * omega, params, solve_index)
+    return sys.right(x), steps
+
+
+def mcsa_solve_general(sys, b, x0=None, params=None, solve_index0=0, forward=False):
+    """MCSASolverManager::solve (src/MCLS_MCSASolverManager_impl.hpp:280-420)
+    with general preconditioners: iterate on y (x = PR y), residual
+    PL (b - A PR y), MC correction over the composite's domain with the
+    internal solver (no PL on its source, no PR on its result), and the
+    final recovery x = PR y.  Same stream indices as mcsa_solve."""
+    params = params if params is not None else solver_params()
+    y = np.zeros(sys.A.n) if x0 is None else np.array(x0, np.float64)
+    b = np.asarray(b, np.float64)
+    dom = sys.domain(params.neumann_omega, params.threshold, forward)
+    source_norm = float(np.sqrt(np.sum(sys.left(b) ** 2)))
+    if abs(source_norm) < 10.0 * np.finfo(float).eps:
+        source_norm = 1.0
+    crit = params.tol * source_norm
+    r = sys.prec_residual(b, y)
+    rnorm = float(np.linalg.norm(r))
+    hist = [rnorm / source_norm]
+    it = 0
+    go = True
+    while go:
+        it += 1
+        for _ in range(params.smoother_steps):
+            y = y + params.richardson_omega * r
+            r = sys.prec_residual(b, y)
+        dx, _ = mc_solve(dom, r * params.neumann_omega, params.mc, solve_index0 + it - 1)
+        y = y + dx
+        r = sys.prec_residual(b, y)
+        rnorm = float(np.linalg.norm(r))
+        hist.append(rnorm / source_norm)
+        if it % max(params.check_freq, 1) == 0:
+            go = rnorm > crit and it < params.max_iters
+    return sys.right(y), {"iters": it, "converged": rnorm <= crit, "achieved_tol": rnorm / source_norm}, hist
