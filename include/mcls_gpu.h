@@ -89,8 +89,28 @@ int mcls_gpu_set_left_prec_diag(mcls_gpu_context* ctx, const double* dinv,
 /* PL = diag(1/a_ii) built on the device (point Jacobi). */
 int mcls_gpu_set_left_prec_point_jacobi(mcls_gpu_context* ctx);
 int mcls_gpu_clear_left_prec(mcls_gpu_context* ctx);
-/* Copy PL's diagonal out (n doubles); MCLS_ERR_STATE without PL. */
+/* Copy PL's diagonal out (n doubles); MCLS_ERR_STATE without a diagonal PL. */
 int mcls_gpu_get_left_prec_diag(mcls_gpu_context* ctx, double* dinv, int ptr_kind);
+
+/* General (sparse) preconditioners, n x n CSR with the operator's n: int64
+ * row pointers, int32 columns strictly ascending in each row, FP64 values
+ * (host or device arrays, `ptr_kind`; copied).
+ *   LinearProblem::setLeftPrec / setRightPrec  src/MCLS_LinearProblem_impl.hpp:115-134
+ * A sparse PL replaces a diagonal one and vice versa.  With either present
+ * the domain is built from the reference's composite operator
+ * (getCompositeOperator / getTransposeCompositeOperator, :139-221): the
+ * products PL (A PR) (forward) or PR^T (A^T PL^T) (adjoint), each filtered
+ * by "Composite Operator Threshold" (tpetra/src/MCLS_TpetraHelpers.hpp:196-263).
+ * The residual is PL (b - A PR x) (updatePrecResidual, :411-442), the MC
+ * source omega PL b, and the solvers return x = PR y
+ * (src/MCLS_MonteCarloSolverManager_impl.hpp:220-229,
+ * src/MCLS_MCSASolverManager_impl.hpp:410-420).  Not with batched systems
+ * (MCLS_ERR_UNSUPPORTED at solve time). */
+int mcls_gpu_set_left_prec_matrix(mcls_gpu_context* ctx, const int64_t* row_ptr,
+                                  const int32_t* col, const double* val, int ptr_kind);
+int mcls_gpu_set_right_prec_matrix(mcls_gpu_context* ctx, const int64_t* row_ptr,
+                                   const int32_t* col, const double* val, int ptr_kind);
+int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
 
 /* ---------------------------------------------------------- parameters -- */
 /* Teuchos::ParameterList equivalents, by the reference's key names:

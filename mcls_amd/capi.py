@@ -40,6 +40,7 @@ GPU_SYMBOLS = [
     "mcls_gpu_set_importance", "mcls_gpu_set_importance_diagonal", "mcls_gpu_get_importance",
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
+    "mcls_gpu_set_left_prec_matrix", "mcls_gpu_set_right_prec_matrix", "mcls_gpu_clear_right_prec",
 ]
 MLP_SYMBOLS = [
     "mcls_mlp_create", "mcls_mlp_destroy", "mcls_mlp_get_params", "mcls_mlp_set_params",
@@ -153,6 +154,9 @@ def gpu_lib():
     L.mcls_gpu_set_left_prec_point_jacobi.argtypes = [vp]
     L.mcls_gpu_clear_left_prec.argtypes = [vp]
     L.mcls_gpu_get_left_prec_diag.argtypes = [vp, vp, i32]
+    L.mcls_gpu_set_left_prec_matrix.argtypes = [vp, vp, vp, vp, i32]
+    L.mcls_gpu_set_right_prec_matrix.argtypes = [vp, vp, vp, vp, i32]
+    L.mcls_gpu_clear_right_prec.argtypes = [vp]
     L.mcls_gpu_set_param_i64.argtypes = [vp, C.c_char_p, i64]
     L.mcls_gpu_set_param_f64.argtypes = [vp, C.c_char_p, f64]
     L.mcls_gpu_set_param_str.argtypes = [vp, C.c_char_p, C.c_char_p]
@@ -325,6 +329,24 @@ class Context:
 
     def clear_left_prec(self):
         self._check(self.L.mcls_gpu_clear_left_prec(self.h))
+
+    @staticmethod
+    def _prec_arrays(rp, col, val):
+        return (np.ascontiguousarray(rp, np.int64), np.ascontiguousarray(col, np.int32),
+                np.ascontiguousarray(val, np.float64))
+
+    def set_left_prec_matrix(self, rp, col, val):
+        """LinearProblem::setLeftPrec with a sparse PL (n x n CSR, ascending rows)."""
+        rp, col, val = self._prec_arrays(rp, col, val)
+        self._check(self.L.mcls_gpu_set_left_prec_matrix(self.h, _p(rp), _p(col), _p(val), MCLS_HOST))
+
+    def set_right_prec_matrix(self, rp, col, val):
+        """LinearProblem::setRightPrec (n x n CSR, ascending rows)."""
+        rp, col, val = self._prec_arrays(rp, col, val)
+        self._check(self.L.mcls_gpu_set_right_prec_matrix(self.h, _p(rp), _p(col), _p(val), MCLS_HOST))
+
+    def clear_right_prec(self):
+        self._check(self.L.mcls_gpu_clear_right_prec(self.h))
 
     def get_left_prec_diag(self):
         d = np.empty(self.n)

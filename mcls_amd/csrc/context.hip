@@ -120,6 +120,15 @@ struct mcls_gpu_context {
   DBuf<char> a_val;
   bool has_prec = false;
   DBuf<double> dinv;
+  // general (sparse) left / right preconditioners: FP64 CSR, ascending rows
+  struct PrecCsr {
+    DBuf<int> rp, col;
+    DBuf<double> val;
+    int64_t nnz = 0;
+    int dim = 0;
+    bool on = false;
+  } pl_csr, pr_csr;
+  DBuf<double> pt1, pt2;  // work vectors of PL (b - A PR x)
 
   // domain
   bool domain_valid = false;
@@ -192,6 +201,8 @@ uint64_t solve_key(uint64_t seed, uint64_t solve_index) {
   return splitmix64_h(seed ^ splitmix64_h(solve_index));
 }
 
+bool general_prec(const mcls_gpu_context* c) { return c->pl_csr.on || c->pr_csr.on; }
+
 int check_ctx(mcls_gpu_context* c) {
   if (!c) return fail(MCLS_ERR_INVALID_ARG, "null context");
   hipError_t e = hipSetDevice(c->device);
@@ -248,6 +259,8 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->key_dev.ensure(1, "key"));
   MCLS_TRY(c->sidx_dev.ensure(1, "solve index"));
   MCLS_TRY(c->mag.ensure(n, "source magnitudes"));
+  MCLS_TRY(c->pt1.ensure(n, "prec work 1"));
+  MCLS_TRY(c->pt2.ensure(n, "prec work 2"));
   if (c->prm.precision == 32 && (!c->t32.p || c->t32.n < n)) {
     MCLS_TRY(c->t32.ensure(n, "tally32"));
     HIP_TRY(hipMemsetAsync(c->t32.p, 0, n * sizeof(float), c->stream));
@@ -283,6 +296,181 @@ void invalidate_graph(mcls_gpu_context* c) {
 // --------------------------------------------------------------- domain
 int convergence_criteria(mcls_gpu_context* c, int64_t max_iters, double tol, mcls_gpu_criteria* out);
 
+// ------------------------------------------------------ composite operator
+// Read-only view of an FP64 CSR matrix, and an owned one.
+struct CsrRef {
+  const int* rp;
+  const int* col;
+  const double* val;
+  int64_t nnz;
+};
+struct DevCsr {
+  DBuf<int> rp, col;
+  DBuf<double> val;
+  int64_t nnz = 0;
+  CsrRef ref() const { return CsrRef{rp.p, col.p, val.p, nnz}; }
+};
+
+int scan_total(mcls_gpu_context* c, const int* cnt, int* out_rp, int64_t* total) {
+  // the row counts are summed on the host in 64 bits first, so that an
+  // operator past 2^31 entries is refused instead of wrapping the scan
+  const int n = c->n;
+  std::vector<int> h((size_t)n);
+  HIP_TRY(hipMemcpyAsync(h.data(), cnt, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  int64_t t = 0;
+  for (int v : h) t += v;
+  if (t > INT32_MAX) return fail(MCLS_ERR_UNSUPPORTED, "composite operator has more than 2^31-1 entries");
+  DBuf<int> tmp;
+  MCLS_TRY(tmp.ensure(mcls::scan_tmp_elems(n) + 1, "scan tmp"));
+  HIP_TRY(mcls::launch_scan_i32_excl(cnt, out_rp, n, tmp.p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *total = t;
+  return MCLS_OK;
+}
+
+int check_composite_flags(mcls_gpu_context* c) {
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_BAD_COLUMN] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(flags[mcls::FLAG_BAD_COLUMN] - 1));
+  if (flags[mcls::FLAG_DUPLICATE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "duplicate column in row " + std::to_string(flags[mcls::FLAG_DUPLICATE] - 1));
+  if (flags[mcls::FLAG_NONFINITE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "non-finite matrix value in row " + std::to_string(flags[mcls::FLAG_NONFINITE] - 1));
+  return MCLS_OK;
+}
+
+// MT::copyTranspose: rows of T ascending in source row.
+int csr_transpose(mcls_gpu_context* c, const int* rp, const int* col, const void* val, bool f32, int64_t nnz,
+                  DevCsr& T) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  DBuf<int> cnt, fill;
+  MCLS_TRY(cnt.ensure((size_t)n, "transpose counts"));
+  MCLS_TRY(fill.ensure((size_t)n, "transpose fill"));
+  MCLS_TRY(T.rp.ensure((size_t)n + 1, "transpose rp"));
+  MCLS_TRY(T.col.ensure((size_t)(nnz > 0 ? nnz : 1), "transpose col"));
+  MCLS_TRY(T.val.ensure((size_t)(nnz > 0 ? nnz : 1), "transpose val"));
+  HIP_TRY(mcls::launch_fill_i32(cnt.p, n, 0, s));
+  HIP_TRY(mcls::launch_fill_i32(fill.p, n, 0, s));
+  HIP_TRY(mcls::launch_count_cols(n, rp, col, val, f32, cnt.p, c->flags.p, s));
+  MCLS_TRY(scan_total(c, cnt.p, T.rp.p, &T.nnz));
+  HIP_TRY(mcls::launch_scatter_transpose(n, rp, col, val, f32, nullptr, T.rp.p, fill.p, T.col.p, T.val.p, s));
+  HIP_TRY(mcls::launch_sort_rows(n, T.rp.p, T.col.p, T.val.p, c->flags.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return MCLS_OK;
+}
+
+int csr_transpose(mcls_gpu_context* c, const CsrRef& M, DevCsr& T) {
+  return csr_transpose(c, M.rp, M.col, M.val, false, M.nnz, T);
+}
+
+// C = filter(X Y, threshold) (MT::multiply + TpetraMatrixHelpers::filter).
+int csr_multiply(mcls_gpu_context* c, const CsrRef& X, const CsrRef& Y, double threshold, DevCsr& C) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  DBuf<int> head, cnt, u_rp, u_col;
+  DBuf<double> u_val;
+  MCLS_TRY(head.ensure((size_t)(X.nnz > 0 ? X.nnz : 1), "product cursors"));
+  MCLS_TRY(cnt.ensure((size_t)n, "product counts"));
+  MCLS_TRY(u_rp.ensure((size_t)n + 1, "product rp"));
+  HIP_TRY(mcls::launch_spgemm_count(n, X.rp, X.col, Y.rp, Y.col, head.p, cnt.p, s));
+  int64_t unnz = 0;
+  MCLS_TRY(scan_total(c, cnt.p, u_rp.p, &unnz));
+  MCLS_TRY(u_col.ensure((size_t)(unnz > 0 ? unnz : 1), "product col"));
+  MCLS_TRY(u_val.ensure((size_t)(unnz > 0 ? unnz : 1), "product val"));
+  HIP_TRY(mcls::launch_spgemm_fill(n, X.rp, X.col, X.val, Y.rp, Y.col, Y.val, head.p, u_rp.p, u_col.p, u_val.p, s));
+  HIP_TRY(mcls::launch_filter_count(n, u_rp.p, u_val.p, threshold, cnt.p, s));
+  MCLS_TRY(C.rp.ensure((size_t)n + 1, "composite rp"));
+  MCLS_TRY(scan_total(c, cnt.p, C.rp.p, &C.nnz));
+  MCLS_TRY(C.col.ensure((size_t)(C.nnz > 0 ? C.nnz : 1), "composite col"));
+  MCLS_TRY(C.val.ensure((size_t)(C.nnz > 0 ? C.nnz : 1), "composite val"));
+  HIP_TRY(mcls::launch_filter_fill(n, u_rp.p, u_col.p, u_val.p, threshold, C.rp.p, C.col.p, C.val.p, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return MCLS_OK;
+}
+
+// The diagonal left preconditioner as a CSR matrix, for the general
+// composite that a sparse right preconditioner requires.
+int diag_as_csr(mcls_gpu_context* c, DevCsr& D) {
+  const int n = c->n;
+  std::vector<int> idx((size_t)n + 1);
+  for (int i = 0; i <= n; ++i) idx[(size_t)i] = i;
+  MCLS_TRY(D.rp.ensure((size_t)n + 1, "diag rp"));
+  MCLS_TRY(D.col.ensure((size_t)n, "diag col"));
+  MCLS_TRY(D.val.ensure((size_t)n, "diag val"));
+  HIP_TRY(hipMemcpy(D.rp.p, idx.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(D.col.p, idx.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(D.val.p, c->dinv.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice));
+  D.nnz = n;
+  return MCLS_OK;
+}
+
+// Rows the domain is built from when a sparse preconditioner is present:
+// the composite operator of src/MCLS_LinearProblem_impl.hpp:139-221, the
+// products taken in the reference's order and each filtered:
+//   forward:  PL (A PR), PL A, A PR
+//   adjoint:  PR^T (A^T PL^T), PR^T A^T, A^T PL^T  (rows of the transpose)
+int build_composite(mcls_gpu_context* c, bool forward, DevCsr& out) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  const double thr = c->prm.threshold;
+  const bool has_l = c->pl_csr.on || c->has_prec;
+  const bool has_r = c->pr_csr.on;
+  DevCsr Ld;  // a diagonal left preconditioner as CSR
+  CsrRef L{}, R{};
+  if (c->pl_csr.on) {
+    L = CsrRef{c->pl_csr.rp.p, c->pl_csr.col.p, c->pl_csr.val.p, c->pl_csr.nnz};
+  } else if (c->has_prec) {
+    MCLS_TRY(diag_as_csr(c, Ld));
+    L = Ld.ref();
+  }
+  if (has_r) R = CsrRef{c->pr_csr.rp.p, c->pr_csr.col.p, c->pr_csr.val.p, c->pr_csr.nnz};
+  MCLS_TRY(reset_flags(c));
+  if (forward) {
+    // A as FP64 with ascending rows
+    DevCsr A;
+    MCLS_TRY(A.rp.ensure((size_t)n + 1, "A64 rp"));
+    MCLS_TRY(A.col.ensure((size_t)(c->nnz > 0 ? c->nnz : 1), "A64 col"));
+    MCLS_TRY(A.val.ensure((size_t)(c->nnz > 0 ? c->nnz : 1), "A64 val"));
+    HIP_TRY(mcls::launch_copy_rows(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, nullptr, A.rp.p, A.col.p, A.val.p,
+                                   c->flags.p, s));
+    HIP_TRY(mcls::launch_sort_rows(n, A.rp.p, A.col.p, A.val.p, c->flags.p, s));
+    A.nnz = c->nnz;
+    MCLS_TRY(check_composite_flags(c));
+    if (has_l && has_r) {
+      DevCsr T;
+      MCLS_TRY(csr_multiply(c, A.ref(), R, thr, T));
+      MCLS_TRY(csr_multiply(c, L, T.ref(), thr, out));
+    } else if (has_l) {
+      MCLS_TRY(csr_multiply(c, L, A.ref(), thr, out));
+    } else {
+      MCLS_TRY(csr_multiply(c, A.ref(), R, thr, out));
+    }
+  } else {
+    DevCsr AT;
+    MCLS_TRY(csr_transpose(c, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->nnz, AT));
+    MCLS_TRY(check_composite_flags(c));
+    if (has_l && has_r) {
+      DevCsr LT, T, RT;
+      MCLS_TRY(csr_transpose(c, L, LT));
+      MCLS_TRY(csr_multiply(c, AT.ref(), LT.ref(), thr, T));
+      MCLS_TRY(csr_transpose(c, R, RT));
+      MCLS_TRY(csr_multiply(c, RT.ref(), T.ref(), thr, out));
+    } else if (has_r) {
+      DevCsr RT;
+      MCLS_TRY(csr_transpose(c, R, RT));
+      MCLS_TRY(csr_multiply(c, RT.ref(), AT.ref(), thr, out));
+    } else {
+      DevCsr LT;
+      MCLS_TRY(csr_transpose(c, L, LT));
+      MCLS_TRY(csr_multiply(c, AT.ref(), LT.ref(), thr, out));
+    }
+  }
+  return check_composite_flags(c);
+}
+
 int build_domain(mcls_gpu_context* c) {
   MCLS_TRY(check_matrix(c));
   const int n = c->n;
@@ -290,31 +478,54 @@ int build_domain(mcls_gpu_context* c) {
   invalidate_graph(c);
   HIP_TRY(hipEventRecord(c->ev[0], s));
   MCLS_TRY(reset_flags(c));
-  DBuf<int> cnt, t_rp, fill, t_src, keep, itmp;
+  DBuf<int> t_rp, t_src, keep, itmp;
   DBuf<double> t_val;
-  MCLS_TRY(cnt.ensure((size_t)n, "col counts"));
-  MCLS_TRY(t_rp.ensure((size_t)n + 1, "transpose rp"));
-  MCLS_TRY(fill.ensure((size_t)n, "fill"));
-  MCLS_TRY(t_src.ensure((size_t)c->nnz, "transpose src"));
-  MCLS_TRY(t_val.ensure((size_t)c->nnz, "transpose val"));
   MCLS_TRY(keep.ensure((size_t)n, "keep"));
   MCLS_TRY(itmp.ensure(mcls::scan_tmp_elems(n) + 1, "scan tmp"));
-  HIP_TRY(mcls::launch_fill_i32(cnt.p, n, 0, s));
-  HIP_TRY(mcls::launch_fill_i32(fill.p, n, 0, s));
   const double* dinv = c->has_prec ? c->dinv.p : nullptr;
   const bool forward = c->prm.forward != 0;
-  if (forward) {
-    // src/MCLS_MonteCarloSolverManager_impl.hpp:243-248: rows of PL A
-    HIP_TRY(mcls::launch_copy_rows(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, t_src.p, t_val.p,
-                                   c->flags.p, s));
+  // Source rows of the domain: the composite operator (rows of PL A, or of
+  // its transpose for the adjoint method), as (src_rp, src_col, src_val).
+  const bool general = general_prec(c);
+  DevCsr comp;
+  const int* src_rp;
+  int* src_col;
+  double* src_val;
+  if (general) {
+    MCLS_TRY(build_composite(c, forward, comp));
+    MCLS_TRY(reset_flags(c));
+    src_rp = comp.rp.p;
+    src_col = comp.col.p;
+    src_val = comp.val.p;
   } else {
-    HIP_TRY(mcls::launch_count_cols(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, cnt.p, c->flags.p, s));
-    HIP_TRY(mcls::launch_scan_i32_excl(cnt.p, t_rp.p, n, itmp.p, s));
-    HIP_TRY(mcls::launch_scatter_transpose(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, fill.p,
-                                           t_src.p, t_val.p, s));
+    DBuf<int> cnt, fill;
+    MCLS_TRY(cnt.ensure((size_t)n, "col counts"));
+    MCLS_TRY(fill.ensure((size_t)n, "fill"));
+    MCLS_TRY(t_rp.ensure((size_t)n + 1, "transpose rp"));
+    MCLS_TRY(t_src.ensure((size_t)c->nnz, "transpose src"));
+    MCLS_TRY(t_val.ensure((size_t)c->nnz, "transpose val"));
+    HIP_TRY(mcls::launch_fill_i32(cnt.p, n, 0, s));
+    HIP_TRY(mcls::launch_fill_i32(fill.p, n, 0, s));
+    if (forward) {
+      // src/MCLS_MonteCarloSolverManager_impl.hpp:243-248: rows of PL A
+      HIP_TRY(mcls::launch_copy_rows(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, t_src.p, t_val.p,
+                                     c->flags.p, s));
+    } else {
+      HIP_TRY(mcls::launch_count_cols(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, cnt.p, c->flags.p, s));
+      HIP_TRY(mcls::launch_scan_i32_excl(cnt.p, t_rp.p, n, itmp.p, s));
+      HIP_TRY(mcls::launch_scatter_transpose(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, t_rp.p, fill.p,
+                                             t_src.p, t_val.p, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    src_rp = t_rp.p;
+    src_col = t_src.p;
+    src_val = t_val.p;
   }
+  // with a diagonal PL the multiply's row filter runs here; the general
+  // composite has filtered inside its products
+  const bool filt = c->has_prec && !general;
   const double omega = c->prm.neumann_omega, thr = c->prm.threshold;
-  HIP_TRY(mcls::launch_domain_rows(n, t_rp.p, t_src.p, t_val.p, omega, thr, c->has_prec, keep.p, c->flags.p, s));
+  HIP_TRY(mcls::launch_domain_rows(n, src_rp, src_col, src_val, omega, thr, filt, keep.p, c->flags.p, s));
   MCLS_TRY(c->d_rp.ensure((size_t)n + 1, "domain rp"));
   HIP_TRY(mcls::launch_scan_i32_excl(keep.p, c->d_rp.p, n, itmp.p, s));
   int flags[mcls::FLAG_COUNT];
@@ -336,7 +547,7 @@ int build_domain(mcls_gpu_context* c) {
   MCLS_TRY(c->d_cdf32.ensure(m, "domain cdf32"));
   MCLS_TRY(c->d_W64.ensure((size_t)n, "domain W64"));
   MCLS_TRY(c->d_W32.ensure((size_t)n, "domain W32"));
-  HIP_TRY(mcls::launch_domain_fill(n, t_rp.p, t_src.p, t_val.p, omega, thr, c->has_prec, c->d_rp.p, c->d_col.p,
+  HIP_TRY(mcls::launch_domain_fill(n, src_rp, src_col, src_val, omega, thr, filt, c->d_rp.p, c->d_col.p,
                                    c->d_h64.p, c->d_h32.p, c->d_cdf64.p, c->d_cdf32.p, c->d_W64.p, c->d_W32.p, s));
   HIP_TRY(hipEventRecord(c->ev[1], s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -644,12 +855,61 @@ int histories_done(mcls_gpu_context* c, int64_t* out) {
   return MCLS_OK;
 }
 
-// r = PL (b - A x), norm^2 into scal[slot] and host_scal[slot]
+// r = PL (b - A PR x) (src/MCLS_LinearProblem_impl.hpp:411-442) with a
+// diagonal or sparse PL and an optional sparse PR; partial sums of r^2 go to
+// part, *nparts of them.
+int prec_residual(mcls_gpu_context* c, const double* b, const double* x, double* r, int* nparts) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  const double* xin = x;
+  if (c->pr_csr.on) {
+    HIP_TRY(mcls::launch_csr_spmv(n, c->pr_csr.rp.p, c->pr_csr.col.p, c->pr_csr.val.p, x, c->pt1.p, s));
+    xin = c->pt1.p;
+  }
+  if (c->pl_csr.on) {
+    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, nullptr, b, xin,
+                                       c->pt2.p, c->part.p, s));
+    HIP_TRY(mcls::launch_csr_spmv(n, c->pl_csr.rp.p, c->pl_csr.col.p, c->pl_csr.val.p, c->pt2.p, r, s));
+    HIP_TRY(mcls::launch_scaled_sq(n, nullptr, r, c->part.p, s));
+    *nparts = mcls::residual_blocks(n, 1);
+  } else {
+    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
+                                       c->has_prec ? c->dinv.p : nullptr, b, xin, r, c->part.p, s));
+    *nparts = mcls::residual_blocks(n, c->res_lanes);
+  }
+  return MCLS_OK;
+}
+
+// y = PL v (diagonal, sparse or none)
+int apply_left_prec(mcls_gpu_context* c, const double* v, double* y) {
+  hipStream_t s = c->stream;
+  if (c->pl_csr.on) {
+    HIP_TRY(mcls::launch_csr_spmv(c->n, c->pl_csr.rp.p, c->pl_csr.col.p, c->pl_csr.val.p, v, y, s));
+  } else if (c->has_prec) {
+    // dinv_i v_i, as PL (v - A 0) with x = 0
+    HIP_TRY(mcls::launch_fill_f64(c->pt1.p, c->n, 0.0, s));
+    HIP_TRY(mcls::launch_prec_residual(c->n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->dinv.p, v,
+                                       c->pt1.p, y, c->part.p, s));
+  } else {
+    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, v, y, s));
+  }
+  return MCLS_OK;
+}
+
+// x = PR x (the recovery of the original solution, right preconditioning)
+int recover_right_prec(mcls_gpu_context* c, double* x) {
+  if (!c->pr_csr.on) return MCLS_OK;
+  hipStream_t s = c->stream;
+  HIP_TRY(mcls::launch_csr_spmv(c->n, c->pr_csr.rp.p, c->pr_csr.col.p, c->pr_csr.val.p, x, c->pt1.p, s));
+  HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, c->pt1.p, x, s));
+  return MCLS_OK;
+}
+
+// r = PL (b - A PR x), norm^2 into scal[slot] and host_scal[slot]
 int residual(mcls_gpu_context* c, const double* b, const double* x, double* r, int slot) {
   hipStream_t s = c->stream;
-  const int nb = mcls::residual_blocks(c->n, c->res_lanes);
-  HIP_TRY(mcls::launch_prec_residual(c->n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
-                                     c->has_prec ? c->dinv.p : nullptr, b, x, r, c->part.p, s));
+  int nb = 0;
+  MCLS_TRY(prec_residual(c, b, x, r, &nb));
   HIP_TRY(mcls::launch_sum_partials(c->part.p, nb, c->scal.p + slot, s));
   HIP_TRY(hipMemcpyAsync(c->host_scal + slot, c->scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, s));
   return MCLS_OK;
@@ -676,13 +936,12 @@ int mcsa_iteration(mcls_gpu_context* c, bool timed, const uint64_t* key_dev, uin
   hipStream_t s = c->stream;
   const Params& p = c->prm;
   const int n = c->n;
-  const double* dinv = c->has_prec ? c->dinv.p : nullptr;
   if (timed) HIP_TRY(hipEventRecord(c->ev[5], s));
   for (int64_t l = 0; l < p.smoother_steps; ++l) {
     // RichardsonIteration::doOneIteration: x += omega rp; rp = PL(b - A x)
     HIP_TRY(mcls::launch_axpy(n, p.richardson_omega, c->r.p, c->x.p, s));
-    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, c->b.p,
-                                       c->x.p, c->r.p, c->part.p, s));
+    int np = 0;
+    MCLS_TRY(prec_residual(c, c->b.p, c->x.p, c->r.p, &np));
   }
   // buildMonteCarloSource with the internal solver: rhs = omega rp
   if (c->nsys > 0) {
@@ -895,6 +1154,9 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   c->has_importance = false;
   c->proposal_valid = false;
   if (c->has_prec && (int64_t)c->dinv.n < n) c->has_prec = false;
+  // sparse preconditioners of another size no longer apply
+  if (c->pl_csr.on && c->pl_csr.dim != n) c->pl_csr.on = false;
+  if (c->pr_csr.on && c->pr_csr.dim != n) c->pr_csr.on = false;
   MCLS_TRY(ensure_vectors(c));
   return MCLS_OK;
 }
@@ -907,6 +1169,7 @@ int mcls_gpu_set_left_prec_diag(mcls_gpu_context* c, const double* dinv, int ptr
   MCLS_TRY(c->dinv.ensure((size_t)c->n, "dinv"));
   HIP_TRY(hipMemcpy(c->dinv.p, dinv, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind)));
   c->has_prec = true;
+  c->pl_csr.on = false;
   c->domain_valid = false;
   invalidate_graph(c);
   return MCLS_OK;
@@ -925,6 +1188,7 @@ int mcls_gpu_set_left_prec_point_jacobi(mcls_gpu_context* c) {
     return fail(MCLS_ERR_MATRIX, "zero diagonal in row " + std::to_string(flags[mcls::FLAG_ZERO_DIAG] - 1));
   }
   c->has_prec = true;
+  c->pl_csr.on = false;
   c->domain_valid = false;
   invalidate_graph(c);
   return MCLS_OK;
@@ -933,6 +1197,80 @@ int mcls_gpu_set_left_prec_point_jacobi(mcls_gpu_context* c) {
 int mcls_gpu_clear_left_prec(mcls_gpu_context* c) {
   MCLS_TRY(check_ctx(c));
   c->has_prec = false;
+  c->pl_csr.on = false;
+  c->domain_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+
+namespace {
+// A sparse preconditioner: n x n (the operator's n), FP64, each row's
+// columns strictly ascending.  Validated on the host and on the device.
+int set_prec_csr(mcls_gpu_context* c, mcls_gpu_context::PrecCsr& P, const int64_t* row_ptr, const int32_t* col,
+                 const double* val, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  MCLS_TRY(check_matrix(c));
+  if (!row_ptr) return fail(MCLS_ERR_INVALID_ARG, "null CSR array");
+  const int n = c->n;
+  const hipMemcpyKind to_host = ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+  std::vector<int64_t> rp64((size_t)n + 1);
+  HIP_TRY(hipMemcpy(rp64.data(), row_ptr, sizeof(int64_t) * ((size_t)n + 1), to_host));
+  const int64_t nnz = rp64[(size_t)n];
+  if (rp64[0] != 0 || nnz < 0 || nnz > INT32_MAX) return fail(MCLS_ERR_MATRIX, "row_ptr[0] != 0 or row_ptr[n] out of range");
+  if (nnz > 0 && (!col || !val)) return fail(MCLS_ERR_INVALID_ARG, "null CSR array");
+  std::vector<int> rp32((size_t)n + 1), col_h((size_t)(nnz > 0 ? nnz : 1));
+  for (int i = 0; i <= n; ++i) {
+    if (i > 0 && rp64[(size_t)i] < rp64[(size_t)i - 1]) return fail(MCLS_ERR_MATRIX, "row_ptr not monotone");
+    rp32[(size_t)i] = (int)rp64[(size_t)i];
+  }
+  if (nnz > 0) HIP_TRY(hipMemcpy(col_h.data(), col, sizeof(int) * (size_t)nnz, to_host));
+  for (int i = 0; i < n; ++i)
+    for (int k = rp32[(size_t)i] + 1; k < rp32[(size_t)i + 1]; ++k)
+      if (col_h[(size_t)k] <= col_h[(size_t)k - 1])
+        return fail(MCLS_ERR_MATRIX, "preconditioner columns not strictly ascending in row " + std::to_string(i));
+  P.on = false;
+  MCLS_TRY(P.rp.ensure((size_t)n + 1, "prec rp"));
+  MCLS_TRY(P.col.ensure((size_t)(nnz > 0 ? nnz : 1), "prec col"));
+  MCLS_TRY(P.val.ensure((size_t)(nnz > 0 ? nnz : 1), "prec val"));
+  HIP_TRY(hipMemcpy(P.rp.p, rp32.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+  if (nnz > 0) {
+    HIP_TRY(hipMemcpy(P.col.p, col_h.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P.val.p, val, sizeof(double) * (size_t)nnz, kind_to_dev(ptr_kind)));
+  }
+  MCLS_TRY(reset_flags(c));
+  HIP_TRY(mcls::launch_validate_csr(n, P.rp.p, P.col.p, P.val.p, false, c->flags.p, c->stream));
+  int flags[mcls::FLAG_COUNT];
+  MCLS_TRY(read_flags(c, flags));
+  if (flags[mcls::FLAG_BAD_COLUMN] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(flags[mcls::FLAG_BAD_COLUMN] - 1));
+  if (flags[mcls::FLAG_NONFINITE] != INT_MAX)
+    return fail(MCLS_ERR_MATRIX, "non-finite value in row " + std::to_string(flags[mcls::FLAG_NONFINITE] - 1));
+  P.nnz = nnz;
+  P.dim = n;
+  P.on = true;
+  MCLS_TRY(ensure_vectors(c));
+  c->domain_valid = false;
+  invalidate_graph(c);
+  return MCLS_OK;
+}
+}  // namespace
+
+int mcls_gpu_set_left_prec_matrix(mcls_gpu_context* c, const int64_t* row_ptr, const int32_t* col, const double* val,
+                                  int ptr_kind) {
+  MCLS_TRY(set_prec_csr(c, c->pl_csr, row_ptr, col, val, ptr_kind));
+  c->has_prec = false;  // replaces a diagonal left preconditioner
+  return MCLS_OK;
+}
+
+int mcls_gpu_set_right_prec_matrix(mcls_gpu_context* c, const int64_t* row_ptr, const int32_t* col, const double* val,
+                                   int ptr_kind) {
+  return set_prec_csr(c, c->pr_csr, row_ptr, col, val, ptr_kind);
+}
+
+int mcls_gpu_clear_right_prec(mcls_gpu_context* c) {
+  MCLS_TRY(check_ctx(c));
+  c->pr_csr.on = false;
   c->domain_valid = false;
   invalidate_graph(c);
   return MCLS_OK;
@@ -941,7 +1279,8 @@ int mcls_gpu_clear_left_prec(mcls_gpu_context* c) {
 int mcls_gpu_get_left_prec_diag(mcls_gpu_context* c, double* dinv, int ptr_kind) {
   MCLS_TRY(check_ctx(c));
   MCLS_TRY(check_kind(ptr_kind));
-  if (!c->has_prec) return fail(MCLS_ERR_STATE, "no left preconditioner");
+  if (!c->has_prec) return fail(MCLS_ERR_STATE, c->pl_csr.on ? "the left preconditioner is not diagonal"
+                                                           : "no left preconditioner");
   if (!dinv) return fail(MCLS_ERR_INVALID_ARG, "null dinv");
   HIP_TRY(hipMemcpy(dinv, c->dinv.p, sizeof(double) * (size_t)c->n, kind_from_dev(ptr_kind)));
   return MCLS_OK;
@@ -1182,6 +1521,8 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   MCLS_TRY(check_ctx(c));
   MCLS_TRY(check_kind(ptr_kind));
   MCLS_TRY(check_matrix(c));
+  if (c->nsys > 0 && general_prec(c))
+    return fail(MCLS_ERR_UNSUPPORTED, "sparse preconditioners are not supported with batched systems");
   if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
@@ -1193,17 +1534,12 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), s));
   // buildMonteCarloSource (src/MCLS_MonteCarloSolverManager_impl.hpp:347-380):
   // rhs = omega * PL b
-  if (c->has_prec) {
-    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, c->b.p, c->rhs.p, s));
-    // rhs_i = dinv_i * b_i via the residual kernel would need x = 0; do it directly
-    HIP_TRY(mcls::launch_fill_f64(c->x.p, c->n, 0.0, s));
-    MCLS_TRY(residual(c, c->b.p, c->x.p, c->rhs.p, 1));  // PL (b - A 0) = PL b
-  } else {
-    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, c->b.p, c->rhs.p, s));
-  }
+  MCLS_TRY(apply_left_prec(c, c->b.p, c->rhs.p));
   if (c->prm.neumann_omega != 1.0) HIP_TRY(mcls::launch_scale_copy(c->n, c->prm.neumann_omega, c->rhs.p, c->rhs.p, s));
   const uint64_t sidx = c->solve_index++;
   MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, sidx));
+  // right preconditioning: x = PR x (src/MCLS_MonteCarloSolverManager_impl.hpp:220-229)
+  MCLS_TRY(recover_right_prec(c, c->x.p));
   HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
   unsigned long long steps = 0;
   HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));
@@ -1223,6 +1559,8 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   MCLS_TRY(check_ctx(c));
   MCLS_TRY(check_kind(ptr_kind));
   MCLS_TRY(check_matrix(c));
+  if (c->nsys > 0 && general_prec(c))
+    return fail(MCLS_ERR_UNSUPPORTED, "sparse preconditioners are not supported with batched systems");
   if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
@@ -1239,7 +1577,12 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   HIP_TRY(hipMemcpyAsync(c->x.p, x, bytes, kind_to_dev(ptr_kind), s));
 
   // src/MCLS_MCSASolverManager_impl.hpp:300-318: ||PL b||_2, homogeneous case
-  HIP_TRY(mcls::launch_scaled_sq(n, c->has_prec ? c->dinv.p : nullptr, c->b.p, c->part.p, s));
+  if (c->pl_csr.on) {
+    MCLS_TRY(apply_left_prec(c, c->b.p, c->pt2.p));
+    HIP_TRY(mcls::launch_scaled_sq(n, nullptr, c->pt2.p, c->part.p, s));
+  } else {
+    HIP_TRY(mcls::launch_scaled_sq(n, c->has_prec ? c->dinv.p : nullptr, c->b.p, c->part.p, s));
+  }
   HIP_TRY(mcls::launch_sum_partials(c->part.p, mcls::residual_blocks(n, 1), c->scal.p + 2, s));
   HIP_TRY(hipMemcpyAsync(c->host_scal + 2, c->scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
   MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
@@ -1331,6 +1674,8 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
     if (p.verbosity > 0 && ((p.print_freq > 0 && iters % p.print_freq == 0) || !go))
       std::printf("%18lld%18.4e\n", (long long)iters, rnorm / source_norm);
   }
+  // src/MCLS_MCSASolverManager_impl.hpp:410-420: x = PR x when right preconditioned
+  MCLS_TRY(recover_right_prec(c, c->x.p));
   HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
   unsigned long long steps = 0;
   HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -1356,6 +1357,123 @@ __global__ void k_tally_out(int n, double* __restrict__ x, double* __restrict__ 
   }
 }
 
+// --------------------------------------------------- composite operator
+// Composite operators with general left / right preconditioners
+// (src/MCLS_LinearProblem_impl.hpp:139-221): sorted transposes, row-by-row
+// sparse products and the per-row filter of MT::multiply.  One lane per
+// row: these run once per operator, not per MCSA iteration.
+
+// Sort each row's (col, val) pairs by column; a repeated column sets
+// FLAG_DUPLICATE.
+__global__ void k_sort_rows(int n, const int* __restrict__ rp, int* __restrict__ col, double* __restrict__ val,
+                            int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = rp[i], end = rp[i + 1];
+    for (int p = beg + 1; p < end; ++p) {
+      const int c = col[p];
+      const double v = val[p];
+      int q = p - 1;
+      while (q >= beg && col[q] > c) {
+        col[q + 1] = col[q];
+        val[q + 1] = val[q];
+        --q;
+      }
+      col[q + 1] = c;
+      val[q + 1] = v;
+    }
+    for (int p = beg + 1; p < end; ++p)
+      if (col[p] == col[p - 1]) set_flag(flags, FLAG_DUPLICATE, i);
+  }
+}
+
+// Row i of C = X Y (rows of X and Y ascending): a merge of the rows Y_k for
+// the entries k of X's row, smallest column first.  C_ij is the first
+// product X_ik Y_kj in the order of X's row, plus each later one in turn
+// (the Gustavson order of a row-by-row product).  head (nnz(X) entries) is
+// each list's cursor.  FILL = false counts the row's entries into cnt.
+template <bool FILL>
+__global__ void k_spgemm(int n, const int* __restrict__ x_rp, const int* __restrict__ x_col,
+                         const double* __restrict__ x_val, const int* __restrict__ y_rp,
+                         const int* __restrict__ y_col, const double* __restrict__ y_val, int* __restrict__ head,
+                         int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_col,
+                         double* __restrict__ c_val) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int xb = x_rp[i], xe = x_rp[i + 1];
+    for (int p = xb; p < xe; ++p) head[p] = y_rp[x_col[p]];
+    int out = FILL ? c_rp[i] : 0;
+    for (;;) {
+      int jmin = INT_MAX;
+      for (int p = xb; p < xe; ++p) {
+        const int h = head[p];
+        if (h < y_rp[x_col[p] + 1]) {
+          const int j = y_col[h];
+          jmin = j < jmin ? j : jmin;
+        }
+      }
+      if (jmin == INT_MAX) break;
+      double sum = 0.0;
+      bool first = true;
+      for (int p = xb; p < xe; ++p) {
+        const int h = head[p];
+        if (h < y_rp[x_col[p] + 1] && y_col[h] == jmin) {
+          if (FILL) {
+            const double prod = x_val[p] * y_val[h];
+            sum = first ? prod : sum + prod;
+          }
+          first = false;
+          head[p] = h + 1;
+        }
+      }
+      if (FILL) {
+        c_col[out] = jmin;
+        c_val[out] = sum;
+      }
+      ++out;
+    }
+    if (!FILL) cnt[i] = out;
+  }
+}
+
+// TpetraMatrixHelpers::filter (tpetra/src/MCLS_TpetraHelpers.hpp:196-263):
+// per row, drop |v| <= (signed row max) * threshold and exact zeros.
+template <bool FILL>
+__global__ void k_filter_rows(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                              const double* __restrict__ val, double threshold, int* __restrict__ keep,
+                              const int* __restrict__ out_rp, int* __restrict__ out_col,
+                              double* __restrict__ out_val) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = rp[i], end = rp[i + 1];
+    double thr = 0.0;
+    if (end > beg) {
+      double mx = val[beg];
+      for (int p = beg + 1; p < end; ++p)
+        if (val[p] > mx) mx = val[p];
+      thr = mx * threshold;
+    }
+    int out = FILL ? out_rp[i] : 0;
+    for (int p = beg; p < end; ++p) {
+      const double v = val[p];
+      if (fabs(v) <= thr || v == 0.0) continue;
+      if (FILL) {
+        out_col[out] = col[p];
+        out_val[out] = v;
+      }
+      ++out;
+    }
+    if (!FILL) keep[i] = out;
+  }
+}
+
+// y = P x for an FP64 CSR matrix, each row summed left to right.
+__global__ void k_csr_spmv(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                           const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) acc = acc + val[k] * x[col[k]];
+    y[i] = acc;
+  }
+}
+
 }  // namespace
 
 // ============================================================== launchers
@@ -1672,6 +1790,45 @@ hipError_t launch_apply_tally(int n, double* x, double* t64, float* t32, const d
 hipError_t launch_store_tally(int n, double* x, double* t64, float* t32, const double* rhs, bool add_rhs,
                               double inv, const uint64_t* mtotal, hipStream_t s) {
   k_tally_out<false><<<grid_for(n, 256), 256, 0, s>>>(n, x, t64, t32, rhs, add_rhs ? 1 : 0, inv, mtotal);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_rows(int n, const int* rp, int* col, double* val, int* flags, hipStream_t s) {
+  k_sort_rows<<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_spgemm_count(int n, const int* x_rp, const int* x_col, const int* y_rp, const int* y_col, int* head,
+                               int* cnt, hipStream_t s) {
+  k_spgemm<false><<<grid_for(n, 256), 256, 0, s>>>(n, x_rp, x_col, nullptr, y_rp, y_col, nullptr, head, cnt,
+                                                   nullptr, nullptr, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_spgemm_fill(int n, const int* x_rp, const int* x_col, const double* x_val, const int* y_rp,
+                              const int* y_col, const double* y_val, int* head, const int* c_rp, int* c_col,
+                              double* c_val, hipStream_t s) {
+  k_spgemm<true><<<grid_for(n, 256), 256, 0, s>>>(n, x_rp, x_col, x_val, y_rp, y_col, y_val, head, nullptr, c_rp,
+                                                  c_col, c_val);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_count(int n, const int* rp, const double* val, double threshold, int* keep, hipStream_t s) {
+  k_filter_rows<false><<<grid_for(n, 256), 256, 0, s>>>(n, rp, nullptr, val, threshold, keep, nullptr, nullptr,
+                                                        nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_fill(int n, const int* rp, const int* col, const double* val, double threshold,
+                              const int* out_rp, int* out_col, double* out_val, hipStream_t s) {
+  k_filter_rows<true><<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, threshold, nullptr, out_rp, out_col,
+                                                       out_val);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_spmv(int n, const int* rp, const int* col, const double* val, const double* x, double* y,
+                           hipStream_t s) {
+  k_csr_spmv<<<grid_for(n, 256), 256, 0, s>>>(n, rp, col, val, x, y);
   return hipGetLastError();
 }
 

@@ -197,6 +197,28 @@ hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const do
                               uint32_t* col, double* h64, float* h32, double* cdf64,
                               float* cdf32, double* W64, float* W32, hipStream_t s);
 
+// ---------------------------------------------------- composite operator
+// General left / right preconditioners (src/MCLS_LinearProblem_impl.hpp:
+// 139-221), one lane per row.  Sort each row by column (repeats set
+// FLAG_DUPLICATE).
+hipError_t launch_sort_rows(int n, const int* rp, int* col, double* val, int* flags, hipStream_t s);
+// C = X Y for row-sorted X, Y: count pass (cnt[i] = entries of row i), then
+// fill into c_rp's rows, ascending columns; C_ij accumulates the products in
+// the order of X's row.  head holds nnz(X) ints of scratch.
+hipError_t launch_spgemm_count(int n, const int* x_rp, const int* x_col, const int* y_rp, const int* y_col, int* head,
+                               int* cnt, hipStream_t s);
+hipError_t launch_spgemm_fill(int n, const int* x_rp, const int* x_col, const double* x_val, const int* y_rp,
+                              const int* y_col, const double* y_val, int* head, const int* c_rp, int* c_col,
+                              double* c_val, hipStream_t s);
+// Row filter of MT::multiply: keep entries with |v| > (signed row max) *
+// threshold that are not exact zeros.
+hipError_t launch_filter_count(int n, const int* rp, const double* val, double threshold, int* keep, hipStream_t s);
+hipError_t launch_filter_fill(int n, const int* rp, const int* col, const double* val, double threshold,
+                              const int* out_rp, int* out_col, double* out_val, hipStream_t s);
+// y = P x (FP64 CSR, rows summed left to right)
+hipError_t launch_csr_spmv(int n, const int* rp, const int* col, const double* val, const double* x, double* y,
+                           hipStream_t s);
+
 // ---------------------------------------------------------------- linalg
 // r = PL (b - A x); partial sums of r^2 per block into part
 // (residual_blocks(n, lanes) <= kResidualPartials entries).  lanes per row

@@ -12,10 +12,12 @@
 //   TpetraPointJacobiPreconditioner   tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
 // Non-zero C ABI status is rethrown as MCLS::Assertion.  The domain is
 // rebuilt only when the operator object changes (pointer identity, as
-// src/MCLS_MonteCarloSolverManager_impl.hpp:161-178).  The left
-// preconditioner must be diagonal (point Jacobi); a right preconditioner,
-// Anderson and the stand-alone fixed-point manager are not provided by this
-// build and raise MCLS::Assertion.
+// src/MCLS_MonteCarloSolverManager_impl.hpp:161-178).  A diagonal left
+// preconditioner goes to the device as its diagonal; any other left or right
+// preconditioner as a sparse matrix, and the domain is then built from the
+// composite operator (src/MCLS_LinearProblem_impl.hpp:139-221).  Anderson
+// and the stand-alone fixed-point manager are not provided by this build and
+// raise MCLS::Assertion.
 #pragma once
 
 #include <cstring>
@@ -150,9 +152,9 @@ class DeviceProblem {
   using Problem = LinearProblem<Vector, Matrix>;
   DeviceProblem() : d_ctx(std::make_shared<GpuContext>(0)) {}
 
-  // Upload the operator when its object changed; refresh PL each call.
+  // Upload the operator when its object changed; refresh PL and PR when
+  // their objects changed.
   void sync(const Problem& p, const ParameterList& pl) {
-    MCLS_INSIST(!p.isRightPrec(), "right preconditioning is not provided by this build");
     const Matrix& A = *p.getOperator();
     MCLS_INSIST(A.isFillComplete(), "operator must be fill-complete");
     if (d_op != p.getOperator().get()) {
@@ -162,18 +164,39 @@ class DeviceProblem {
                                                 A.colInd().data(), val, dt, MCLS_HOST));
       d_op = p.getOperator().get();
       d_pl = nullptr;
+      d_pr = nullptr;
       d_imp_dirty = true;  // a new operator drops the context's importance
     }
     if (p.isLeftPrec()) {
-      if (d_pl != p.getLeftPrec().get()) {
+      const Matrix& PL = *p.getLeftPrec();
+      if (d_pl != &PL) {
+        MCLS_INSIST(PL.isFillComplete() && PL.numRows() == A.numRows(), "left preconditioner must match the operator");
         std::vector<double> d;
-        MCLS_INSIST(p.getLeftPrec()->isDiagonal(&d), "only a diagonal (point Jacobi) left preconditioner is provided");
-        GpuContext::check(mcls_gpu_set_left_prec_diag(d_ctx->get(), d.data(), MCLS_HOST));
-        d_pl = p.getLeftPrec().get();
+        if (PL.isDiagonal(&d)) {
+          GpuContext::check(mcls_gpu_set_left_prec_diag(d_ctx->get(), d.data(), MCLS_HOST));
+        } else {
+          const std::vector<double> v = values64(PL);
+          GpuContext::check(mcls_gpu_set_left_prec_matrix(d_ctx->get(), PL.rowPtr().data(), PL.colInd().data(),
+                                                          v.data(), MCLS_HOST));
+        }
+        d_pl = &PL;
       }
     } else if (d_pl) {
       GpuContext::check(mcls_gpu_clear_left_prec(d_ctx->get()));
       d_pl = nullptr;
+    }
+    if (p.isRightPrec()) {
+      const Matrix& PR = *p.getRightPrec();
+      if (d_pr != &PR) {
+        MCLS_INSIST(PR.isFillComplete() && PR.numRows() == A.numRows(), "right preconditioner must match the operator");
+        const std::vector<double> v = values64(PR);
+        GpuContext::check(mcls_gpu_set_right_prec_matrix(d_ctx->get(), PR.rowPtr().data(), PR.colInd().data(),
+                                                         v.data(), MCLS_HOST));
+        d_pr = &PR;
+      }
+    } else if (d_pr) {
+      GpuContext::check(mcls_gpu_clear_right_prec(d_ctx->get()));
+      d_pr = nullptr;
     }
     d_ctx->setParameters(pl);
     if (d_imp_dirty && d_imp_mode != ImportanceMode::None) {
@@ -208,8 +231,12 @@ class DeviceProblem {
  private:
   enum class ImportanceMode { None, Diagonal, Explicit };
   RCP<GpuContext> d_ctx;
+  static std::vector<double> values64(const Matrix& M) {
+    return std::vector<double>(M.values().begin(), M.values().end());
+  }
   const void* d_op = nullptr;
   const void* d_pl = nullptr;
+  const void* d_pr = nullptr;
   ImportanceMode d_imp_mode = ImportanceMode::None;
   std::vector<float> d_imp;
   bool d_imp_dirty = false;

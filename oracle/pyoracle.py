@@ -435,6 +435,12 @@ def build_domain_general(A, PL=None, PR=None, omega=1.0, threshold=0.0, forward=
     return build_domain(Cm, None, omega, 0.0, forward=True)
 
 
+def norm2(v):
+    """Sequential 2-norm (orc_norm2), as the C solvers compute it."""
+    v = np.ascontiguousarray(v, np.float64)
+    return float(lib().orc_norm2(len(v), v))
+
+
 class PrecSystem:
     """A with a left preconditioner (sparse PL, or diagonal dinv) and an
     optional sparse right preconditioner PR (src/MCLS_LinearProblem_impl.hpp)."""
@@ -483,12 +489,12 @@ def mcsa_solve_general(sys, b, x0=None, params=None, solve_index0=0, forward=Fal
     y = np.zeros(sys.A.n) if x0 is None else np.array(x0, np.float64)
     b = np.asarray(b, np.float64)
     dom = sys.domain(params.neumann_omega, params.threshold, forward)
-    source_norm = float(np.sqrt(np.sum(sys.left(b) ** 2)))
+    source_norm = norm2(sys.left(b))
     if abs(source_norm) < 10.0 * np.finfo(float).eps:
         source_norm = 1.0
     crit = params.tol * source_norm
     r = sys.prec_residual(b, y)
-    rnorm = float(np.linalg.norm(r))
+    rnorm = norm2(r)
     hist = [rnorm / source_norm]
     it = 0
     go = True
@@ -500,7 +506,7 @@ def mcsa_solve_general(sys, b, x0=None, params=None, solve_index0=0, forward=Fal
         dx, _ = mc_solve(dom, r * params.neumann_omega, params.mc, solve_index0 + it - 1)
         y = y + dx
         r = sys.prec_residual(b, y)
-        rnorm = float(np.linalg.norm(r))
+        rnorm = norm2(r)
         hist.append(rnorm / source_norm)
         if it % max(params.check_freq, 1) == 0:
             go = rnorm > crit and it < params.max_iters

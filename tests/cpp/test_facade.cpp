@@ -2,6 +2,8 @@
 // (paths relative to the MCLS source root):
 //   tpetra/test/tstTpetraMCSASolverManager.cpp:72-203        (adjoint)
 //   tpetra/test/tstTpetraMCSASolverManager.cpp:206-355       (prec_adjoint)
+//   tpetra/test/tstTpetraMCSASolverManager.cpp:494-...        (prec_forward)
+//   tpetra/test/tstTpetraMonteCarloSolverManager.cpp:200-340 (adjoint_prec)
 //   tpetra/test/tstTpetraMonteCarloSolverManager.cpp:73-197  (adjoint MC)
 //   src/MCLS_SolverFactory_impl.hpp:58-146                   (names)
 // Prints one line per check and exits non-zero on the first failure.
@@ -301,6 +303,107 @@ static void test_mcsa_importance() {
               (long long)t.blocks_per_cu, (long long)t.batch);
 }
 
+static RCP<MatrixType> identity(int n, double v) {
+  auto I = std::make_shared<MatrixType>(n);
+  for (int i = 0; i < n; ++i) I->insertGlobalValues(i, {i}, {v});
+  I->fillComplete();
+  return I;
+}
+
+// prec_adjoint / prec_forward and adjoint_prec / forward_prec of the
+// reference: PL = PR = I, signs of x for sources -1, 2, 2 (after
+// setProblem), -2, fewer than 20 MCSA iterations.
+template <class Tag>
+static void test_identity_prec(const char* name) {
+  const int n = 10;
+  auto A = build_operator(n, 1.0, -0.14);
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*x, 0.0);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*b, -1.0);
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-8);
+  plist->set<int>("Maximum Iterations", 20);
+  plist->set<double>("Sample Ratio", 1.0);
+  plist->set<int>("Random Seed", 4321);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  auto I = identity(n, 1.0);
+  problem->setLeftPrec(I);
+  problem->setRightPrec(I);
+  MCSASolverManager<VectorType, MatrixType, Tag> solver(problem, plist);
+  CHECK(solver.solve() && solver.getConvergedStatus() && solver.getNumIters() < 20 && solver.achievedTol() > 0.0);
+  CHECK(all_sign(*x, -1.0));
+  VT::putScalar(*const_cast<VectorType*>(b.get()), 2.0);
+  VT::putScalar(*x, 0.0);
+  problem->setLHS(x);
+  CHECK(solver.solve() && solver.getNumIters() < 20 && all_sign(*x, 1.0));
+  VT::putScalar(*x, 0.0);
+  solver.setProblem(problem);
+  CHECK(solver.solve() && solver.getNumIters() < 20 && all_sign(*x, 1.0));
+  VT::putScalar(*const_cast<VectorType*>(b.get()), -2.0);
+  VT::putScalar(*x, 0.0);
+  CHECK(solver.solve() && solver.getNumIters() < 20 && all_sign(*x, -1.0));
+  auto mlist = parameterList();
+  mlist->set<double>("Sample Ratio", 10.0);
+  mlist->set<int>("History Length", 10);
+  MonteCarloSolverManager<VectorType, MatrixType, Tag> mc(problem, mlist, 0);
+  CHECK(mc.solve() && all_sign(*x, -1.0));
+  std::printf("ok %s with PL = PR = I iters=%d\n", name, solver.getNumIters());
+}
+
+// A sparse (block-inverse) PL and a non-diagonal PR: MCSA on PL A PR y =
+// PL b, x = PR y, checked by the unpreconditioned residual on the host.
+static void test_sparse_prec() {
+  const int n = 12;
+  auto A = build_operator(n, 2.0, -0.5);
+  // PL: inverse of the 2 x 2 diagonal blocks of A
+  auto PL = std::make_shared<MatrixType>(n);
+  for (int k = 0; k < n; k += 2) {
+    auto a = [&](int i, int j) {
+      for (int64_t p = A->rowPtr()[(size_t)i]; p < A->rowPtr()[(size_t)i + 1]; ++p)
+        if (A->colInd()[(size_t)p] == j) return A->values()[(size_t)p];
+      return 0.0;
+    };
+    const double a00 = a(k, k), a01 = a(k, k + 1), a10 = a(k + 1, k), a11 = a(k + 1, k + 1);
+    const double det = a00 * a11 - a01 * a10;
+    PL->insertGlobalValues(k, {k, k + 1}, {a11 / det, -a01 / det});
+    PL->insertGlobalValues(k + 1, {k, k + 1}, {-a10 / det, a00 / det});
+  }
+  PL->fillComplete();
+  auto PR = std::make_shared<MatrixType>(n);
+  for (int i = 0; i < n; ++i) {
+    if (i + 1 < n)
+      PR->insertGlobalValues(i, {i, i + 1}, {0.9 + 0.02 * i, 0.03});
+    else
+      PR->insertGlobalValues(i, {i}, {0.9 + 0.02 * i});
+  }
+  PR->fillComplete();
+  auto x = MT::cloneVectorFromMatrixRows(*A);
+  VT::putScalar(*x, 0.0);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  for (int i = 0; i < n; ++i) (*b)[i] = 1.0 + 0.1 * i;
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-10);
+  plist->set<int>("Maximum Iterations", 40);
+  plist->set<double>("Sample Ratio", 200.0);
+  plist->set<int>("Random Seed", 9);
+  auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+  problem->setLeftPrec(PL);
+  problem->setRightPrec(PR);
+  MCSASolverManager<VectorType, MatrixType, AdjointTag> solver(problem, plist);
+  CHECK(solver.solve() && solver.achievedTol() <= 1e-10);
+  double r2 = 0.0, b2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double ax = 0.0;
+    for (int64_t k = A->rowPtr()[(size_t)i]; k < A->rowPtr()[(size_t)i + 1]; ++k)
+      ax += A->values()[(size_t)k] * (*x)[A->colInd()[(size_t)k]];
+    r2 += ((*b)[i] - ax) * ((*b)[i] - ax);
+    b2 += (*b)[i] * (*b)[i];
+  }
+  CHECK(std::sqrt(r2 / b2) <= 1e-8);
+  std::printf("ok MCSASolverManager with sparse PL and PR iters=%d\n", solver.getNumIters());
+}
+
 int main() {
   test_mcsa_adjoint(false);
   test_mcsa_adjoint(true);
@@ -309,6 +412,9 @@ int main() {
   test_factory_and_errors();
   test_matrix_market_system();
   test_mcsa_importance();
+  test_identity_prec<AdjointTag>("MCSASolverManager.prec_adjoint");
+  test_identity_prec<ForwardTag>("MCSASolverManager.prec_forward");
+  test_sparse_prec();
   std::printf("ALL %d checks passed\n", g_checks);
   return 0;
 }
