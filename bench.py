@@ -8,10 +8,14 @@ tolerance, with the matrix, the domain and b resident in HBM.
 
 Forms:
   chosen: FP32 walks and tallies, FP64 residual/update/convergence test,
-          stratified source, alias transition sampling, relative weight
-          cutoff 1e-2.
+          stratified source, relative weight cutoff 1e-1, and the learned
+          proposal with plain almost-optimal (alias) sampling as the
+          fallback: before the timed solves a network is trained on this
+          operator's own adjoint tally and the lowest variance x walk time
+          of plain / diagonal importance / network on pilot solves is kept.
   plain:  FP64 walks and tallies, random source, CDF inversion (the
           reference's sampler), weight cutoff 1e-4.
+  forward: the forward method with the chosen walk (reported beside).
 value = histories/s of the chosen form over the K timed solves.
 
 The walk-kernel roofline uses the per-step algorithmic bytes of the walk
@@ -20,8 +24,8 @@ launch's HIP-event duration measured by the library on its own stream.
 
 cpu_baseline: the oracle's restatement of the MCLS reference algorithm
 (FP64, per-row CDF + lower_bound, one Xorshift stream per set, multi-set
-threads summed / numSets) timed for a bounded number of MCSA iterations on
-the same operator family at a reduced grid (see the printed sample).
+threads summed / numSets) timed on one MCSA iteration of a bounded history
+sample on the same operator (see the printed sample).
 """
 import argparse
 import json
@@ -97,7 +101,66 @@ def bytes_per_step(prec, z, sampler, forward=False):
         return (32 if (sampler == "Alias" and p == 4) else 16 + (16 if sampler == "Alias" else p * z + 4)) + 8
     if sampler == "Alias":
         return 32 + 2 * p if p == 4 else 16 + 16 + 2 * p
+    if sampler == "Learned":  # reshaped CDF row plus the entry's weight factor
+        return 16 + p * z + 4 + p + 2 * p
     return 16 + p * z + 4 + 2 * p
+
+
+def learned_stage(ctx, prob, w, pilot_histories=1 << 24, reps=4, train_rows=1 << 18, epochs=20):
+    """The chosen form's learned proposal on this instance, with plain
+    almost-optimal sampling as the fallback: train the network on the
+    solver's own adjoint tally of this operator (features and targets of a
+    row sample), then run `reps` pilot MC solves of the first MCSA source
+    with each candidate - plain (the chosen alias walk), the diagonal
+    importance g = |a_jj| and the network - and keep the one with the lowest
+    variance x walk time per solve (the cost of a fixed error).  Leaves the
+    context configured with it; returns the record for the bench line."""
+    from mcls_amd import learned
+
+    t0 = time.perf_counter()
+    nh = w["histories"]
+    rhs = prob.b * ctx.get_left_prec_diag()
+    ctx.set_params({"Number of Histories": min(pilot_histories, nh)})
+    X, T = learned.gather(ctx, rhs, solve_index=1 << 40)
+    rng = np.random.default_rng(1)
+    rows = rng.choice(prob.n, size=min(prob.n, train_rows), replace=False)
+    net = capi.Mlp(0, seed=1)
+    loss = learned.train(net, X[rows], T[rows], epochs=epochs, batch_rows=512, lr=2e-3, seed=1)
+    cands = {"plain": None, "diagonal": "diagonal", "network": net}
+    var, ms = {}, {}
+    for name, g in cands.items():
+        if g is None:
+            ctx.set_params({"Transition Sampler": FORMS["chosen"]["Transition Sampler"]})
+        else:
+            ctx.set_params({"Transition Sampler": "Learned"})
+            if g == "diagonal":
+                ctx.set_importance_diagonal()
+            else:
+                ctx.set_importance_model(net)
+        samples, t = [], 0.0
+        for k in range(reps):
+            samples.append(ctx.mc_solve_raw(rhs, (1 << 41) + k))
+            t += ctx.stats().ms_walk
+        var[name] = learned.relative_variance(np.stack(samples))
+        ms[name] = t / reps
+        if g is net:
+            ctx.set_importance_model(None)
+        del samples
+    eff = {k: var[k] * ms[k] for k in var}
+    choice = min(eff, key=eff.get)
+    if choice == "plain":
+        ctx.set_params({"Transition Sampler": FORMS["chosen"]["Transition Sampler"]})
+    else:
+        ctx.set_params({"Transition Sampler": "Learned"})
+        if choice == "diagonal":
+            ctx.set_importance_diagonal()
+        else:
+            ctx.set_importance_model(net)
+    ctx.set_params({"Number of Histories": nh})
+    return {"choice": choice, "relative_variance": var, "walk_ms_per_solve": ms,
+            "variance_x_ms": eff, "pilot_histories": min(pilot_histories, nh), "pilot_solves": reps,
+            "train_rows": int(len(rows)), "train_epochs": epochs, "train_loss": float(loss),
+            "seconds": time.perf_counter() - t0}, net
 
 
 def measured_traffic(workload, form, steps_per_launch):
@@ -155,6 +218,8 @@ def main():
     ap.add_argument("--form", default="chosen", choices=sorted(FORMS))
     ap.add_argument("--no-tune", action="store_true", help="skip the walk autotuner (default launch shape)")
     ap.add_argument("--no-forward", action="store_true", help="skip the forward-method solve")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-proposal selection of the chosen form (plain sampling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -191,6 +256,13 @@ def main():
     st = ctx.stats()
     log(f"domain built: nnz={st.domain_nnz} in {time.perf_counter() - t0:.2f}s (kernel {st.ms_domain:.1f} ms)")
     z = st.domain_nnz / prob.n
+    learned_rec, net = None, None
+    if args.form == "chosen" and not args.no_learned:
+        learned_rec, net = learned_stage(ctx, prob, w)
+        log(f"learned proposal: {learned_rec['choice']} (relative variance "
+            f"{ {k: round(v, 5) for k, v in learned_rec['relative_variance'].items()} }, walk ms "
+            f"{ {k: round(v, 2) for k, v in learned_rec['walk_ms_per_solve'].items()} })")
+    sampler = "Learned" if learned_rec and learned_rec["choice"] != "plain" else FORMS[args.form]["Transition Sampler"]
     tuning = None
     if not args.no_tune:
         tuning = ctx.autotune(prob.b)
@@ -214,8 +286,7 @@ def main():
     total = sum(times)
     hps = hist / total
     prec = FORMS[args.form]["Walk Precision"]
-    bstep = bytes_per_step(prec, z, FORMS[args.form]["Transition Sampler"],
-                           FORMS[args.form].get("Monte Carlo Type") == "Forward")
+    bstep = bytes_per_step(prec, z, sampler, FORMS[args.form].get("Monte Carlo Type") == "Forward")
     avg_launch_s = walk_ms / 1e3 / max(launches, 1)
     steps_per_launch = steps / max(launches, 1)
     achieved = bstep * steps_per_launch / avg_launch_s / 1e9
@@ -247,6 +318,7 @@ def main():
                      "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},
         "tuning": tuning,
+        "learned_proposal": learned_rec,
     }
     if not args.no_plain and args.form == "chosen":
         ctx.set_params(FORMS["plain"])
