@@ -99,11 +99,16 @@ def bytes_per_step(prec, z, sampler, forward=False):
     p = 4 if prec == "FP32" else 8
     if forward:  # the tally is one register add of w b[state]: read 8 bytes of b instead
         return (32 if (sampler == "Alias" and p == 4) else 16 + (16 if sampler == "Alias" else p * z + 4)) + 8
-    if sampler == "Alias":
+    if sampler in ("Alias", "Learned Alias"):  # the learned records carry S_i/g_j in place of W_j
         return 32 + 2 * p if p == 4 else 16 + 16 + 2 * p
     if sampler == "Learned":  # reshaped CDF row plus the entry's weight factor
         return 16 + p * z + 4 + p + 2 * p
     return 16 + p * z + 4 + 2 * p
+
+
+# the learned proposal drawn by alias records: the same per-step cost as the
+# chosen form's plain alias walk
+LEARNED_SAMPLER = "Learned Alias"
 
 
 def learned_stage(ctx, prob, w, pilot_histories=1 << 24, reps=4, train_rows=1 << 18, epochs=20):
@@ -132,7 +137,7 @@ def learned_stage(ctx, prob, w, pilot_histories=1 << 24, reps=4, train_rows=1 <<
         if g is None:
             ctx.set_params({"Transition Sampler": FORMS["chosen"]["Transition Sampler"]})
         else:
-            ctx.set_params({"Transition Sampler": "Learned"})
+            ctx.set_params({"Transition Sampler": LEARNED_SAMPLER})
             if g == "diagonal":
                 ctx.set_importance_diagonal()
             else:
@@ -151,7 +156,7 @@ def learned_stage(ctx, prob, w, pilot_histories=1 << 24, reps=4, train_rows=1 <<
     if choice == "plain":
         ctx.set_params({"Transition Sampler": FORMS["chosen"]["Transition Sampler"]})
     else:
-        ctx.set_params({"Transition Sampler": "Learned"})
+        ctx.set_params({"Transition Sampler": LEARNED_SAMPLER})
         if choice == "diagonal":
             ctx.set_importance_diagonal()
         else:
@@ -262,7 +267,7 @@ def main():
         log(f"learned proposal: {learned_rec['choice']} (relative variance "
             f"{ {k: round(v, 5) for k, v in learned_rec['relative_variance'].items()} }, walk ms "
             f"{ {k: round(v, 2) for k, v in learned_rec['walk_ms_per_solve'].items()} })")
-    sampler = "Learned" if learned_rec and learned_rec["choice"] != "plain" else FORMS[args.form]["Transition Sampler"]
+    sampler = LEARNED_SAMPLER if learned_rec and learned_rec["choice"] != "plain" else FORMS[args.form]["Transition Sampler"]
     tuning = None
     if not args.no_tune:
         tuning = ctx.autotune(prob.b)

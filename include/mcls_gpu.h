@@ -5,12 +5,13 @@
  * last failure on the calling thread is returned by mcls_gpu_last_error().
  *
  * One context = one GPU stream holding one linear problem (matrix, optional
- * point-Jacobi left preconditioner, the adjoint domain built from them, and
- * the work vectors).  A context is not thread-safe; separate contexts are
+ * left preconditioner (diagonal or sparse) and right preconditioner
+ * (sparse), the Monte Carlo domain built from them, and the work vectors).  A context is not thread-safe; separate contexts are
  * independent.  Vectors are host or device pointers, chosen by `ptr_kind`.
  *
  * Reference interfaces replaced (paths relative to the MCLS source root):
- *   LinearProblem(A, x, b), setLeftPrec       src/MCLS_LinearProblem.hpp:74-91
+ *   LinearProblem(A, x, b), setLeftPrec,
+ *   setRightPrec                              src/MCLS_LinearProblem.hpp:74-91
  *   TpetraPointJacobiPreconditioner           tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
  *   AlmostOptimalDomain(A^T composite, x, pl) src/MCLS_AlmostOptimalDomain_impl.hpp:71-191,258-366
  *   MonteCarloSolverManager<..,AdjointTag>    src/MCLS_MonteCarloSolverManager.hpp:103-220
@@ -134,9 +135,11 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  * Build-only keys:
  *   i64 "Number of Histories"  (0 = N * Sample Ratio)
  *   str "Walk Precision"       ("FP64" | "FP32")
- *   str "Transition Sampler"   ("CDF" | "Alias" | "Learned"): row CDF
- *        inversion (the reference's lower_bound), Vose alias tables (rows
- *        <= 64 entries) or the importance-reshaped rows (learned proposal)
+ *   str "Transition Sampler"   ("CDF" | "Alias" | "Learned" | "Learned Alias"):
+ *        row CDF inversion (the reference's lower_bound), Vose alias tables
+ *        (rows <= 64 entries), the importance-reshaped rows of the learned
+ *        proposal by CDF inversion, or the same proposal by alias records
+ *        (FP32 walks, rows <= 64 entries)
  *   i64 "Random Seed"          (0)
  *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
  *   i64 "Verbosity"            (0)

@@ -150,6 +150,8 @@ struct mcls_gpu_context {
   bool has_importance = false, proposal_valid = false;
   DBuf<double> d_lcdf64, d_lfac64;
   DBuf<float> d_lcdf32, d_lfac32;
+  DBuf<mcls::Rec32> d_lrec32;  // the proposal as alias records (FP32 walks)
+  int proposal_kind = -1;      // the sampler the proposal tables were built for
   mcls_mlp* model = nullptr;       // recomputes g on each MC source when set
   DBuf<float> feat, mlp_out;       // n x 8 features, n outputs
   DBuf<signed char> imp_mask;      // 0 = importance switched off at that state
@@ -202,6 +204,11 @@ uint64_t solve_key(uint64_t seed, uint64_t solve_index) {
 }
 
 bool general_prec(const mcls_gpu_context* c) { return c->pl_csr.on || c->pr_csr.on; }
+
+// the learned proposal, sampled by CDF inversion or by alias records
+bool learned_sampler(const mcls_gpu_context* c) {
+  return c->prm.sampler == mcls::SAMPLER_LEARNED || c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS;
+}
 
 int check_ctx(mcls_gpu_context* c) {
   if (!c) return fail(MCLS_ERR_INVALID_ARG, "null context");
@@ -602,11 +609,29 @@ int ensure_alias(mcls_gpu_context* c) {
 
 int ensure_proposal_buffers(mcls_gpu_context* c) {
   const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
-  MCLS_TRY(c->d_lcdf64.ensure(m, "proposal cdf64"));
-  MCLS_TRY(c->d_lcdf32.ensure(m, "proposal cdf32"));
-  MCLS_TRY(c->d_lfac64.ensure(m, "proposal factors64"));
-  MCLS_TRY(c->d_lfac32.ensure(m, "proposal factors32"));
+  if (c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) {
+    if (c->prm.precision != 32)
+      return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Learned Alias' needs FP32 walks (Walk Precision 'FP32')");
+    MCLS_TRY(c->d_lrec32.ensure(m, "proposal alias records"));
+  } else {
+    MCLS_TRY(c->d_lcdf64.ensure(m, "proposal cdf64"));
+    MCLS_TRY(c->d_lcdf32.ensure(m, "proposal cdf32"));
+    MCLS_TRY(c->d_lfac64.ensure(m, "proposal factors64"));
+    MCLS_TRY(c->d_lfac32.ensure(m, "proposal factors32"));
+  }
   MCLS_TRY(c->g_imp.ensure((size_t)c->n, "importance"));
+  return MCLS_OK;
+}
+
+// The proposal tables of the sampler in use from c->g_imp (on the stream).
+int build_proposal_tables(mcls_gpu_context* c) {
+  if (c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) {
+    HIP_TRY(mcls::launch_pack_proposal_alias(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lrec32.p,
+                                             c->flags.p, c->stream));
+  } else {
+    HIP_TRY(mcls::launch_build_proposal(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lcdf64.p,
+                                        c->d_lcdf32.p, c->d_lfac64.p, c->d_lfac32.p, c->flags.p, c->stream));
+  }
   return MCLS_OK;
 }
 
@@ -629,36 +654,47 @@ int model_importance(mcls_gpu_context* c, const double* v) {
   HIP_TRY(mcls::launch_mlp_forward(mcls::mlp_params_device(c->model), c->feat.p, c->n, c->mlp_out.p, c->stream));
   HIP_TRY(mcls::launch_importance(c->n, c->mlp_out.p, c->has_mask ? c->imp_mask.p : nullptr, c->g_imp.p,
                                   c->stream));
-  HIP_TRY(mcls::launch_build_proposal(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lcdf64.p,
-                                      c->d_lcdf32.p, c->d_lfac64.p, c->d_lfac32.p, c->flags.p, c->stream));
-  return MCLS_OK;
+  return build_proposal_tables(c);
 }
 
 // Learned-proposal tables from the stored importance (rebuilt after a
 // domain rebuild or a new g).  With a network attached the tables are
 // rebuilt for every MC source instead.
 int ensure_proposal(mcls_gpu_context* c) {
-  if (c->prm.sampler != mcls::SAMPLER_LEARNED || c->proposal_valid) return MCLS_OK;
+  if (!learned_sampler(c) || (c->proposal_valid && c->proposal_kind == c->prm.sampler)) return MCLS_OK;
+  if (c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) {
+    // alias records hold rows of at most 64 entries
+    MCLS_TRY(reset_flags(c));
+    HIP_TRY(mcls::launch_flag_long_rows(c->n, c->d_rp.p, 64, c->flags.p, c->stream));
+    int lf[mcls::FLAG_COUNT];
+    MCLS_TRY(read_flags(c, lf));
+    if (lf[mcls::FLAG_LONG_ROW] != INT_MAX)
+      return fail(MCLS_ERR_UNSUPPORTED, "domain row " + std::to_string(lf[mcls::FLAG_LONG_ROW] - 1) +
+                                            " has more than 64 entries (use the 'Learned' sampler)");
+  }
   if (c->model) return ensure_proposal_buffers(c);
   if (!c->has_importance)
     return fail(MCLS_ERR_STATE, "Transition Sampler 'Learned' needs an importance (mcls_gpu_set_importance)");
   MCLS_TRY(ensure_proposal_buffers(c));
   MCLS_TRY(reset_flags(c));
-  HIP_TRY(mcls::launch_build_proposal(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->g_imp.p, c->d_lcdf64.p,
-                                      c->d_lcdf32.p, c->d_lfac64.p, c->d_lfac32.p, c->flags.p, c->stream));
+  MCLS_TRY(build_proposal_tables(c));
   int flags[mcls::FLAG_COUNT];
   MCLS_TRY(read_flags(c, flags));
   if (flags[mcls::FLAG_BAD_IMPORTANCE] != INT_MAX)
     return fail(MCLS_ERR_NUMERIC, "importance g must be finite and > 0 (domain row " +
                                       std::to_string(flags[mcls::FLAG_BAD_IMPORTANCE] - 1) + ")");
+  if (flags[mcls::FLAG_LONG_ROW] != INT_MAX)
+    return fail(MCLS_ERR_UNSUPPORTED, "domain row " + std::to_string(flags[mcls::FLAG_LONG_ROW] - 1) +
+                                          " has more than 64 entries (use the 'Learned' sampler)");
   c->proposal_valid = true;
+  c->proposal_kind = c->prm.sampler;
   return MCLS_OK;
 }
 
 // Target records of the walk precision in use, for the CDF and learned
 // samplers (alias walks read their own records).
 int ensure_targets(mcls_gpu_context* c) {
-  if (c->prm.sampler == mcls::SAMPLER_ALIAS) return MCLS_OK;
+  if (c->prm.sampler == mcls::SAMPLER_ALIAS || c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) return MCLS_OK;
   const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
   if (c->prm.precision == 32 && !c->tgt32_valid) {
     MCLS_TRY(c->d_tgt32.ensure(m, "target records 32"));
@@ -749,6 +785,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.lcdf32 = c->d_lcdf32.p;
   a.d.lfac64 = c->d_lfac64.p;
   a.d.lfac32 = c->d_lfac32.p;
+  a.d.lrec32 = c->d_lrec32.p;
   a.queue = c->queue.p;
   a.rhs = rhs;
   a.q = c->q.p;
@@ -796,7 +833,9 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
-  if (c->prm.sampler == mcls::SAMPLER_LEARNED && c->model) MCLS_TRY(model_importance(c, rhs));
+  if (c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS && c->prm.precision != 32)
+    return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Learned Alias' needs FP32 walks (Walk Precision 'FP32')");
+  if (learned_sampler(c) && c->model) MCLS_TRY(model_importance(c, rhs));
   const bool forward = c->prm.forward != 0;
   if (forward && c->prm.estimator != mcls::EST_COLLISION)
     return fail(MCLS_ERR_UNSUPPORTED, "the forward method tallies collisions only (Estimator Type 'Collision')");
@@ -1387,6 +1426,7 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     if (key_is(v, "CDF")) p.sampler = mcls::SAMPLER_CDF;
     else if (key_is(v, "Alias")) p.sampler = mcls::SAMPLER_ALIAS;
     else if (key_is(v, "Learned")) p.sampler = mcls::SAMPLER_LEARNED;
+    else if (key_is(v, "Learned Alias")) p.sampler = mcls::SAMPLER_LEARNED_ALIAS;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Transition Sampler '") + v + "'");
   } else if (key_is(key, "Walk Precision")) {
     if (key_is(v, "FP64")) p.precision = 64;
@@ -1716,7 +1756,7 @@ int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
   c->has_importance = true;
   c->proposal_valid = false;
   invalidate_graph(c);
-  if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
+  if (c->domain_valid && learned_sampler(c)) return ensure_proposal(c);
   return MCLS_OK;
 }
 
@@ -1837,7 +1877,7 @@ int mcls_gpu_set_importance_diagonal(mcls_gpu_context* c) {
   c->has_importance = true;
   c->proposal_valid = false;
   invalidate_graph(c);
-  if (c->domain_valid && c->prm.sampler == mcls::SAMPLER_LEARNED) return ensure_proposal(c);
+  if (c->domain_valid && learned_sampler(c)) return ensure_proposal(c);
   return MCLS_OK;
 }
 
@@ -1897,7 +1937,7 @@ int mcls_gpu_importance_targets(mcls_gpu_context* c, const double* v, uint64_t s
   MCLS_TRY(ensure_vectors(c));
   // one plain adjoint MC solve of v (almost-optimal transitions)
   const int sampler = c->prm.sampler;
-  if (sampler == mcls::SAMPLER_LEARNED) c->prm.sampler = mcls::SAMPLER_CDF;
+  if (sampler == mcls::SAMPLER_LEARNED || sampler == mcls::SAMPLER_LEARNED_ALIAS) c->prm.sampler = mcls::SAMPLER_CDF;
   int rc = ensure_domain(c);
   if (rc == MCLS_OK) {
     HIP_TRY(hipMemcpyAsync(c->rhs.p, v, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), c->stream));

@@ -433,6 +433,18 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         beg = keep ? sr.b.x : sr.b.y;
         len = keep ? (sr.a.w & 0xFFFFu) : (sr.a.w >> 16);
         Wrow = (Real)__uint_as_float(keep ? sr.b.z : sr.b.w);
+      } else if constexpr (SAMP == SAMPLER_LEARNED_ALIAS) {
+        // the learned proposal by alias: the record holds both candidates'
+        // row headers and weight factors S_i / g_j
+        const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
+        const Rec32 sr = d.lrec32[beg + slot];
+        tally_step();
+        const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
+        const bool keep = u2 < __uint_as_float(sr.a.x);
+        cp = keep ? sr.a.y : sr.a.z;
+        w = w * (Real)__uint_as_float(keep ? sr.b.z : sr.b.w);
+        beg = keep ? sr.b.x : sr.b.y;
+        len = keep ? (sr.a.w & 0xFFFFu) : (sr.a.w >> 16);
       } else if constexpr (SAMP == SAMPLER_LEARNED) {
         // importance-reshaped row: P = |h| g / S, weight factor S / g_j
         const uint32_t k = beg + (uint32_t)lower_bound_guess<Real>(Tables<Real>::lcdf(d) + beg, (int)len,
@@ -492,6 +504,10 @@ hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
 template <typename Real, int EST, int SRC>
 hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
   if (c.sampler == SAMPLER_LEARNED) return launch_walk_s<Real, EST, SRC, SAMPLER_LEARNED>(a, c, s);
+  if (c.sampler == SAMPLER_LEARNED_ALIAS) {
+    if constexpr (sizeof(Real) == 4) return launch_walk_s<Real, EST, SRC, SAMPLER_LEARNED_ALIAS>(a, c, s);
+    return hipErrorInvalidValue;  // FP32 records only; the context refuses FP64 first
+  }
   return c.sampler == SAMPLER_ALIAS ? launch_walk_s<Real, EST, SRC, SAMPLER_ALIAS>(a, c, s)
                                     : launch_walk_s<Real, EST, SRC, SAMPLER_CDF>(a, c, s);
 }
@@ -912,6 +928,90 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
       r.b = make_uint4((uint32_t)bs, (uint32_t)bo, __float_as_uint(W32[js]), __float_as_uint(W32[jo]));
       rec32[beg + k] = r;
       slot64[beg + k] = make_uint4((uint32_t)__double2loint(thr[k]), (uint32_t)__double2hiint(thr[k]), self, other);
+    }
+  }
+}
+
+// FLAG_LONG_ROW for every domain row longer than `limit` entries.
+__global__ void k_flag_long_rows(int n, const int* __restrict__ d_rp, int limit, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (d_rp[i + 1] - d_rp[i] > limit) set_flag(flags, FLAG_LONG_ROW, i);
+}
+
+// Alias records of the learned proposal: Vose's method over the reshaped
+// row P_ik = |h_ik| g_k / S_i (S_i summed in row order, as k_build_proposal
+// does), and for each candidate its weight factor S_i / g_j rounded to FP32,
+// the value the CDF form of the proposal multiplies by.
+__global__ void k_pack_proposal_alias(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                                      const double* __restrict__ h64, const float* __restrict__ g,
+                                      Rec32* __restrict__ lrec32, int* flags) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = d_rp[i];
+    const int z = d_rp[i + 1] - beg;
+    double wts[64];
+    double S = 0.0;
+    bool bad = false;
+    for (int k = 0; k < z && k < 64; ++k) {
+      const double gj = (double)g[col[beg + k] & 0x7FFFFFFFu];
+      if (!(gj > 0.0) || !(gj <= DBL_MAX)) bad = true;
+      wts[k] = fabs(h64[beg + k]) * gj;
+      S = wts[k] + S;
+    }
+    if (z > 64 || bad) {
+      set_flag(flags, z > 64 ? FLAG_LONG_ROW : FLAG_BAD_IMPORTANCE, i);
+      // the caller refuses the tables; the records are still written, as
+      // uniform draws with weight factor z |h|, so that every record a
+      // walk could read holds in-range row headers
+      for (int k = 0; k < z; ++k) {
+        const uint32_t self = col[beg + k];
+        const int js = (int)(self & 0x7FFFFFFFu);
+        const int bs = d_rp[js];
+        const uint32_t ls = (uint32_t)(d_rp[js + 1] - bs);
+        const float f = (float)(fabs(h64[beg + k]) * (double)z);
+        Rec32 r;
+        r.a = make_uint4(__float_as_uint(1.0f), self, self, (ls & 0xFFFFu) | (ls << 16));
+        r.b = make_uint4((uint32_t)bs, (uint32_t)bs, __float_as_uint(f), __float_as_uint(f));
+        lrec32[beg + k] = r;
+      }
+      continue;
+    }
+    double s[64], thr[64];
+    int small[64], large[64], al[64];
+    int ns = 0, nl = 0;
+    for (int k = 0; k < z; ++k) {
+      s[k] = (wts[k] / S) * (double)z;
+      if (s[k] < 1.0) small[ns++] = k;
+      else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+      const int l = small[--ns];
+      const int gg = large[--nl];
+      thr[l] = s[l];
+      al[l] = gg;
+      s[gg] = (s[gg] + s[l]) - 1.0;
+      if (s[gg] < 1.0) small[ns++] = gg;
+      else large[nl++] = gg;
+    }
+    while (nl > 0) {
+      const int gg = large[--nl];
+      thr[gg] = 1.0;
+      al[gg] = gg;
+    }
+    while (ns > 0) {
+      const int l = small[--ns];
+      thr[l] = 1.0;
+      al[l] = l;
+    }
+    for (int k = 0; k < z; ++k) {
+      const uint32_t self = col[beg + k], other = col[beg + al[k]];
+      const int js = (int)(self & 0x7FFFFFFFu), jo = (int)(other & 0x7FFFFFFFu);
+      const int bs = d_rp[js], bo = d_rp[jo];
+      const uint32_t ls = (uint32_t)(d_rp[js + 1] - bs), lo = (uint32_t)(d_rp[jo + 1] - bo);
+      const float fs = (float)(S / (double)g[js]), fo = (float)(S / (double)g[jo]);
+      Rec32 r;
+      r.a = make_uint4(__float_as_uint((float)thr[k]), self, other, (ls & 0xFFFFu) | (lo << 16));
+      r.b = make_uint4((uint32_t)bs, (uint32_t)bo, __float_as_uint(fs), __float_as_uint(fo));
+      lrec32[beg + k] = r;
     }
   }
 }
@@ -1552,6 +1652,17 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
                             bool alias, int* flags, hipStream_t s) {
   k_pack_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, hdr32, hdr64, rec32, slot64,
                                                 alias ? 1 : 0, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_flag_long_rows(int n, const int* d_rp, int limit, int* flags, hipStream_t s) {
+  k_flag_long_rows<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, limit, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_proposal_alias(int n, const int* d_rp, const uint32_t* col, const double* h64, const float* g,
+                                      Rec32* lrec32, int* flags, hipStream_t s) {
+  k_pack_proposal_alias<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, g, lrec32, flags);
   return hipGetLastError();
 }
 

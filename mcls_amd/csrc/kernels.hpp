@@ -62,6 +62,9 @@ struct DomainView {
   const float* lcdf32;    // nnz
   const double* lfac64;   // nnz
   const float* lfac32;    // nnz
+  // learned proposal sampled by alias (FP32 walks): per entry a 32-byte
+  // record {thr, col, alias col, lens | start, alias start, S_i/g_j of both}
+  const Rec32* lrec32;    // nnz
 };
 
 // EST_FORWARD: the forward method on the forward domain (rows of H): history
@@ -69,7 +72,7 @@ struct DomainView {
 // own sum at every collision; the sum goes to its start state at the end.
 enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1, EST_FORWARD = 2 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
-enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1, SAMPLER_LEARNED = 2 };
+enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1, SAMPLER_LEARNED = 2, SAMPLER_LEARNED_ALIAS = 3 };
 
 // Work queue of the walk kernel: one counter per XCD slot, 128 bytes apart.
 constexpr int kQueueSlots = 8;
@@ -134,6 +137,13 @@ hipError_t launch_targets(int n, const double* x, const double* sums, float* T, 
 
 // Learned-proposal tables from the importance g (n, FP32, > 0); a row that
 // reaches g <= 0 or a non-finite g sets FLAG_BAD_IMPORTANCE.
+// The learned proposal as alias records (Vose over |h_ij| g_j / S_i per row,
+// sequential, rows of <= 64 entries; longer rows set FLAG_LONG_ROW, a bad g
+// sets FLAG_BAD_IMPORTANCE).
+hipError_t launch_pack_proposal_alias(int n, const int* d_rp, const uint32_t* col, const double* h64, const float* g,
+                                      Rec32* lrec32, int* flags, hipStream_t s);
+// FLAG_LONG_ROW for every domain row longer than limit
+hipError_t launch_flag_long_rows(int n, const int* d_rp, int limit, int* flags, hipStream_t s);
 hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, const double* h64, const float* g,
                                  double* lcdf64, float* lcdf32, double* lfac64, float* lfac32, int* flags,
                                  hipStream_t s);

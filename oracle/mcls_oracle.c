@@ -262,29 +262,58 @@ int64_t orc_build_domain_forward(int n, const int64_t* a_rp, const int* a_col,
  * 1991), run sequentially per row in a fixed order so that the CPU and the
  * GPU tables are bit-identical.  Build extension: the reference samples
  * rows only by CDF inversion. */
-int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
-                        const double* d_W, double* thr, int* alias) {
+/* One row of Vose's method over weights wts[0..z) with total S (z <= 64):
+ * s_k = (wts_k / S) z, small / large stacks popped from the top. */
+static void vose_row(const double* wts, double S, int z, double* thr, int* alias) {
   double s[64];
   int small[64], large[64];
+  int ns = 0, nl = 0;
+  for (int k = 0; k < z; ++k) {
+    s[k] = (wts[k] / S) * (double)z;
+    if (s[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+  }
+  while (ns > 0 && nl > 0) {
+    int l = small[--ns];
+    int g = large[--nl];
+    thr[l] = s[l];
+    alias[l] = g;
+    s[g] = (s[g] + s[l]) - 1.0;
+    if (s[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+  }
+  while (nl > 0) { int g = large[--nl]; thr[g] = 1.0; alias[g] = g; }
+  while (ns > 0) { int l = small[--ns]; thr[l] = 1.0; alias[l] = l; }
+}
+
+int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
+                        const double* d_W, double* thr, int* alias) {
+  double wts[64];
   for (int i = 0; i < n; ++i) {
     int64_t beg = d_rp[i];
     int z = (int)(d_rp[i + 1] - beg);
     if (z > 64) return -(int64_t)(i + 1);
-    int ns = 0, nl = 0;
+    for (int k = 0; k < z; ++k) wts[k] = fabs(d_h[beg + k]);
+    vose_row(wts, d_W[i], z, thr + beg, alias + beg);
+  }
+  return 0;
+}
+
+int64_t orc_build_proposal_alias(int n, const int64_t* d_rp, const int* d_col,
+                                 const double* d_h, const float* g, double* thr,
+                                 int* alias, double* lfac) {
+  double wts[64];
+  for (int i = 0; i < n; ++i) {
+    int64_t beg = d_rp[i];
+    int z = (int)(d_rp[i + 1] - beg);
+    if (z > 64) return -(int64_t)(i + 1);
+    double acc = 0.0;
     for (int k = 0; k < z; ++k) {
-      s[k] = (fabs(d_h[beg + k]) / d_W[i]) * (double)z;
-      if (s[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+      double gj = (double)g[d_col[beg + k]];
+      if (!(gj > 0.0) || !(gj <= DBL_MAX)) return -(int64_t)(i + 1);
+      wts[k] = fabs(d_h[beg + k]) * gj;
+      acc = wts[k] + acc;
     }
-    while (ns > 0 && nl > 0) {
-      int l = small[--ns];
-      int g = large[--nl];
-      thr[beg + l] = s[l];
-      alias[beg + l] = g;
-      s[g] = (s[g] + s[l]) - 1.0;
-      if (s[g] < 1.0) small[ns++] = g; else large[nl++] = g;
-    }
-    while (nl > 0) { int g = large[--nl]; thr[beg + g] = 1.0; alias[beg + g] = g; }
-    while (ns > 0) { int l = small[--ns]; thr[beg + l] = 1.0; alias[beg + l] = l; }
+    vose_row(wts, acc, z, thr + beg, alias + beg);
+    for (int k = 0; k < z; ++k) lfac[beg + k] = acc / (double)g[d_col[beg + k]];
   }
   return 0;
 }
@@ -385,6 +414,7 @@ typedef struct {
   float* h32;
   float* cdf32;
   float* W32;
+  int learned_alias; /* thr / alias / lfac hold the learned proposal's tables */
   double* thr;   /* alias tables (sampler == ORC_SAMPLER_ALIAS) */
   float* thr32;
   int* alias;
@@ -479,7 +509,10 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
       int k;
-      if (d->lcdf32) {  /* importance-reshaped: P = |h| g / S, factor S / g */
+      if (d->learned_alias) {  /* the same proposal drawn by alias */
+        k = alias_pick32(r, len, d->thr32 + beg, d->alias + beg);
+        w = w * d->lfac32[beg + k];
+      } else if (d->lcdf32) {  /* importance-reshaped: P = |h| g / S, factor S / g */
         k = orc_lower_bound_f32(d->lcdf32 + beg, len, orc_u01_f32(r));
         w = w * d->lfac32[beg + k];
       } else {
@@ -511,7 +544,10 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
       int k;
-      if (d->lcdf) {
+      if (d->learned_alias) {
+        k = alias_pick64(r, len, d->thr + beg, d->alias + beg);
+        w = w * d->lfac[beg + k];
+      } else if (d->lcdf) {
         k = orc_lower_bound_f64(d->lcdf + beg, len, orc_u01_f64(r));
         w = w * d->lfac[beg + k];
       } else {
@@ -563,6 +599,24 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
   d->alias = NULL;
   d->lcdf = d->lfac = NULL;
   d->lcdf32 = d->lfac32 = NULL;
+  d->learned_alias = 0;
+  if (importance && need_alias) {
+    /* the learned proposal by alias (ORC_SAMPLER_LEARNED_ALIAS) */
+    int64_t nnz = rp[n];
+    d->thr = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    d->thr32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
+    d->alias = (int*)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    d->lfac = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    d->lfac32 = (float*)malloc((size_t)(nnz ? nnz : 1) * sizeof(float));
+    orc_build_proposal_alias(n, rp, col, h, importance, d->thr, d->alias, d->lfac);
+    for (int64_t k = 0; k < nnz; ++k) {
+      d->thr32[k] = (float)d->thr[k];
+      d->lfac32[k] = (float)d->lfac[k];
+    }
+    d->learned_alias = 1;
+    need_alias = 0;
+    importance = NULL;
+  }
   if (importance) {
     int64_t nnz = rp[n];
     d->lcdf = (double*)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
@@ -626,8 +680,9 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
                      int* paths, double* weights) {
   domain_view d;
   domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32,
-                   p->sampler == ORC_SAMPLER_ALIAS,
-                   p->sampler == ORC_SAMPLER_LEARNED ? p->importance : NULL);
+                   p->sampler == ORC_SAMPLER_ALIAS || p->sampler == ORC_SAMPLER_LEARNED_ALIAS,
+                   (p->sampler == ORC_SAMPLER_LEARNED || p->sampler == ORC_SAMPLER_LEARNED_ALIAS)
+                       ? p->importance : NULL);
   uint64_t* q = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* qcdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* ncdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));

@@ -92,6 +92,12 @@ int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
 int orc_build_proposal(int n, const int64_t* d_rp, const int* d_col,
                        const double* d_h, const float* g, double* lcdf,
                        double* lfac);
+/* The same proposal as per-row alias tables (Vose over |h_ij| g_j / S_i, S_i
+ * summed in row order) with the factors lfac = S_i / g_j; rows of at most 64
+ * entries.  Returns 0, or -(row+1) for a longer row or a bad g. */
+int64_t orc_build_proposal_alias(int n, const int64_t* d_rp, const int* d_col,
+                                 const double* d_h, const float* g, double* thr,
+                                 int* alias, double* lfac);
 
 /* ------------------------------------------------------------- MC solve -- */
 /* ORC_EST_FORWARD: the forward method (ForwardTag) on the forward domain:
@@ -101,7 +107,9 @@ int orc_build_proposal(int n, const int64_t* d_rp, const int* d_col,
  * ignored. */
 enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1, ORC_EST_FORWARD = 2 };
 enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
-enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2 };
+/* ORC_SAMPLER_LEARNED_ALIAS: the learned proposal drawn with alias tables
+ * (orc_build_proposal_alias), weight factor S_i / g_j as for LEARNED. */
+enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2, ORC_SAMPLER_LEARNED_ALIAS = 3 };
 
 typedef struct orc_mc_params {
   int64_t num_histories;   /* histories per MC solve                       */

@@ -20,6 +20,7 @@ SRC_STRATIFIED = 1
 SAMPLER_CDF = 0
 SAMPLER_ALIAS = 1
 SAMPLER_LEARNED = 2
+SAMPLER_LEARNED_ALIAS = 3
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -110,6 +111,8 @@ def lib():
     L.orc_build_proposal.restype = i32
     L.orc_build_alias.argtypes = [i32, _i64p, _f64p, _f64p, _f64p, _i32p]
     L.orc_build_alias.restype = i64
+    L.orc_build_proposal_alias.argtypes = [i32, _i64p, _i32p, _f64p, _f32p, _f64p, _i32p, _f64p]
+    L.orc_build_proposal_alias.restype = i64
     L.orc_jacobi_dinv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p]
     L.orc_jacobi_dinv.restype = i32
     L.orc_source_quantize.argtypes = [i32, _f64p, _u64p, _u64p]
@@ -237,7 +240,8 @@ def mc_params(num_histories, history_length=10, weight_cutoff=0.0, precision=64,
     imp = None
     if importance is not None:
         imp = np.ascontiguousarray(importance, np.float32)
-        sampler = SAMPLER_LEARNED
+        if sampler != SAMPLER_LEARNED_ALIAS:
+            sampler = SAMPLER_LEARNED
     p = MCParams(int(num_histories), int(history_length), float(weight_cutoff),
                  int(precision), int(estimator), int(source), int(seed) & (2**64 - 1),
                  int(num_threads), int(sampler), None if imp is None else imp.ctypes.data)
@@ -253,6 +257,16 @@ def build_proposal(dom, g):
     if rc != 0:
         raise ValueError(f"non-positive importance reached from row {-rc - 1}")
     return lcdf[: dom.nnz], lfac[: dom.nnz]
+
+
+def build_proposal_alias(dom, g):
+    g = np.ascontiguousarray(g, np.float32)
+    m = max(dom.nnz, 1)
+    thr, alias, lfac = np.zeros(m), np.zeros(m, np.int32), np.zeros(m)
+    rc = lib().orc_build_proposal_alias(dom.n, dom.rp, dom.col, dom.h, g, thr, alias, lfac)
+    if rc != 0:
+        raise ValueError(f"row {-rc - 1}: longer than 64 entries or a bad importance")
+    return thr[: dom.nnz], alias[: dom.nnz], lfac[: dom.nnz]
 
 
 def build_alias(dom):

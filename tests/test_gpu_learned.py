@@ -121,6 +121,76 @@ def test_learned_paths_bitexact(prec):
     assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("source", ["Stratified", "Random"])
+def test_learned_alias_paths_bitexact(source):
+    """The learned proposal drawn by alias records (FP32 walks): paths and
+    final weights bit-exact against the oracle's alias restatement, tallies
+    to FP32 accumulation noise."""
+    prob = _varcoef()
+    g = np.exp(np.random.default_rng(2).normal(size=prob.n)).astype(np.float32)
+    L = 120
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "Learned Alias", "Walk Precision": "FP32", "History Length": L,
+                        "Weight Cutoff": 1e-3, "Number of Histories": 4000, "Source Sampling Type": source,
+                        "Random Seed": 78})
+        ctx.set_importance(g)
+        A = po.Csr(prob.rp, prob.col, prob.values())
+        dinv = po.jacobi_dinv(A)
+        rhs = prob.b * dinv
+        gp, gw = ctx.record_paths(rhs, 6, 0, 2000, L)
+        xg = ctx.mc_solve_raw(rhs, 6)
+    dom = po.build_domain(A, dinv)
+    mp = po.mc_params(4000, L, 1e-3, 32, source=po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM,
+                      seed=78, importance=g, sampler=po.SAMPLER_LEARNED_ALIAS)
+    op, ow = po.record_paths(dom, rhs, mp, 0, 2000, L, solve_index=6)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    assert (gp[:, 1] >= 0).mean() > 0.9
+    xo, _ = po.mc_solve(dom, rhs, mp, solve_index=6)
+    assert np.linalg.norm(xg - xo) <= 2e-5 * np.linalg.norm(xo)
+
+
+def test_learned_alias_with_network_and_errors():
+    """With a network attached the alias records are rebuilt for each MC
+    source; FP64 walks and rows over 64 entries are refused."""
+    prob = _varcoef()
+    net = capi.Mlp(0, seed=12)
+    A = po.Csr(prob.rp, prob.col, prob.values())
+    rhs = prob.b * po.jacobi_dinv(A)
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "Learned Alias", "Walk Precision": "FP32", "History Length": 50,
+                        "Number of Histories": 4000, "Source Sampling Type": "Stratified", "Random Seed": 3})
+        ctx.set_importance_model(net)
+        x1 = ctx.mc_solve_raw(rhs, 0)
+        g = ctx.importance()
+        ctx.set_importance_model(None)
+        # the network's g, set explicitly, gives the same solve
+        ctx.set_importance(g)
+        x2 = ctx.mc_solve_raw(rhs, 0)
+        assert np.linalg.norm(x1 - x2) <= 2e-5 * np.linalg.norm(x2)
+        ctx.set_params({"Walk Precision": "FP64"})
+        with pytest.raises(capi.GpuError):
+            ctx.mc_solve_raw(rhs, 1)
+    # a dense row (a column of A with more than 64 entries) is refused
+    n = 100
+    S = sp.diags(np.full(n, 4.0)).tolil()
+    S[:, 0] = -0.01
+    S[0, 0] = 4.0
+    S = S.tocsr()
+    S.sort_indices()
+    dprob = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data, None, np.ones(n))
+    with capi.Context(0) as ctx:
+        ctx.set_problem(dprob)
+        ctx.set_point_jacobi()
+        ctx.set_params({"Transition Sampler": "Learned Alias", "Walk Precision": "FP32", "Number of Histories": 100})
+        ctx.set_importance(np.ones(n, np.float32))
+        with pytest.raises(capi.GpuError):
+            ctx.mc_solve_raw(np.ones(n) * 0.25, 0)
+
+
 def test_model_importance_and_mask():
     prob = _varcoef()
     net = capi.Mlp(0, seed=11)

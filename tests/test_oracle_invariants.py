@@ -93,6 +93,41 @@ def test_learned_proposal_is_unbiased(prec, est):
         assert np.allclose(p * lfac[s:e], np.abs(dom.h[s:e]), rtol=1e-12)
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_learned_alias_proposal_is_unbiased(prec):
+    """The learned proposal drawn by alias tables keeps E[x] too, and its
+    tables reproduce P_ij = |h_ij| g_j / S_i with factors S_i / g_j."""
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv)
+    H = np.eye(n) - dinv[:, None] * A
+    L = 6
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(L))
+    g = np.exp(np.random.default_rng(5).normal(size=n)).astype(np.float32)
+    mp = po.mc_params(3000, L, 0.0, prec, po.EST_COLLISION, po.SRC_STRATIFIED, seed=13, importance=g,
+                      sampler=po.SAMPLER_LEARNED_ALIAS)
+    mean, se = _replicas(dom, b, mp, 60)
+    _check_z(mean, se, exact)
+    thr, alias, lfac = po.build_proposal_alias(dom, g)
+    _, lfac_cdf = po.build_proposal(dom, g)
+    assert np.array_equal(lfac, lfac_cdf)  # the same factors as the CDF form
+    for i in range(n):
+        s, e = dom.rp[i], dom.rp[i + 1]
+        z = e - s
+        if z == 0:
+            continue
+        wts = np.abs(dom.h[s:e]) * g[dom.col[s:e]].astype(np.float64)
+        p = wts / wts.sum()
+        q = np.zeros(z)
+        for k in range(z):
+            q[k] += thr[s + k] / z
+            q[alias[s + k]] += (1.0 - thr[s + k]) / z
+        assert np.max(np.abs(q - p)) < 1e-14
+        assert np.allclose(p * lfac[s:e], np.abs(dom.h[s:e]), rtol=1e-12)
+
+
 def test_reference_semantics_unbiased():
     A, b = _system(10, seed=8)
     Ac = po.Csr.from_dense(A)
