@@ -696,12 +696,15 @@ int ensure_proposal(mcls_gpu_context* c) {
 int ensure_targets(mcls_gpu_context* c) {
   if (c->prm.sampler == mcls::SAMPLER_ALIAS || c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) return MCLS_OK;
   const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
-  if (c->prm.precision == 32 && !c->tgt32_valid) {
+  // CDF walks of both precisions read the FP64 records (which hold the CDF
+  // value); the FP32 learned walk reads the 16-byte FP32 records
+  const bool rec32 = c->prm.precision == 32 && c->prm.sampler == mcls::SAMPLER_LEARNED;
+  if (rec32 && !c->tgt32_valid) {
     MCLS_TRY(c->d_tgt32.ensure(m, "target records 32"));
     HIP_TRY(mcls::launch_pack_targets(c->d_nnz, c->d_rp.p, c->d_col.p, c->d_W64.p, c->d_W32.p, c->d_cdf64.p,
                                       c->d_tgt32.p, nullptr, c->stream));
     c->tgt32_valid = true;
-  } else if (c->prm.precision == 64 && !c->tgt64_valid) {
+  } else if (!rec32 && !c->tgt64_valid) {
     MCLS_TRY(c->d_tgt64.ensure(m, "target records 64"));
     HIP_TRY(mcls::launch_pack_targets(c->d_nnz, c->d_rp.p, c->d_col.p, c->d_W64.p, c->d_W32.p, c->d_cdf64.p,
                                       nullptr, c->d_tgt64.p, c->stream));

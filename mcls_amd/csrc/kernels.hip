@@ -148,44 +148,6 @@ template <> struct Row<double> {
     const double u2 = (double)(uint32_t)r * 0x1.0p-32;
     return u2 < __hiloint2double((int)s.y, (int)s.x) ? s.z : s.w;
   }
-  // CDF draw (lower_bound of u in the row's CDF) and transition from the
-  // target records, whose b.zw hold each entry's CDF value.  The search
-  // starts at the guess k = floor(u len): record k and the CDF value of
-  // k - 1 come in one round trip, and when they bracket u (cdf[k-1] < u <=
-  // cdf[k]) record k is also the transition.  Otherwise k moves one entry
-  // towards u per round trip (the lines are already cached), up to
-  // kCdfWalk moves, after which a bisection over cdf64 settles the rest.
-  template <typename F>
-  __device__ static uint32_t cdf_step(const DomainView& d, uint32_t& beg, uint32_t& len, double& W, double u,
-                                      F&& overlap) {
-    constexpr int kCdfWalk = 3;
-    const Rec32* __restrict__ t = d.tgt64 + beg;
-    const int n = (int)len;
-    int k = (int)(u * (double)n);
-    k = k < n - 1 ? k : n - 1;
-    Rec32 r = t[k];
-    uint2 l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
-    overlap();
-    for (int move = 0;; ++move) {
-      const double ck = __hiloint2double((int)r.b.w, (int)r.b.z);
-      const double cl = k > 0 ? __hiloint2double((int)l.y, (int)l.x) : -1.0;
-      const bool right = ck < u;  // then k < n - 1, as cdf[n - 1] = 1 > u
-      if (!right && !(cl >= u)) break;
-      if (move == kCdfWalk) {
-        const double* __restrict__ c = d.cdf64 + beg;
-        k = right ? k + 1 + lower_bound_cdf<double>(c + k + 1, n - k - 1, u) : lower_bound_cdf<double>(c, k - 1, u);
-        r = t[k];
-        break;
-      }
-      k += right ? 1 : -1;  // cl >= u implies k > 0
-      r = t[k];
-      l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
-    }
-    beg = r.a.y;
-    len = r.a.z;
-    W = __hiloint2double((int)r.b.y, (int)r.b.x);
-    return r.a.x;
-  }
 };
 template <> struct Row<float> {
   __device__ static uint4 hdr(const DomainView& d, int i) { return d.hdr32[i]; }
@@ -209,6 +171,47 @@ __device__ uint32_t Row<double>::target(const DomainView& d, uint32_t k, uint32_
 
 
 
+// CDF draw (lower_bound of u in the row's CDF) and transition from the FP64
+// target records, whose b.zw hold each entry's CDF value; an FP32 walk
+// rounds the record's values to float, which gives exactly its cdf32 / W32
+// entries.  The search starts at the guess k = floor(u len): record k and
+// the CDF value of k - 1 come in one round trip, and when they bracket u
+// (cdf[k-1] < u <= cdf[k]) record k is also the transition.  Otherwise k
+// moves one entry towards u per round trip (the lines are already cached),
+// up to kCdfWalk moves, after which a bisection over the walk precision's
+// CDF settles the rest.
+template <typename Real, typename F>
+__device__ __forceinline__ uint32_t cdf_step(const DomainView& d, uint32_t& beg, uint32_t& len, Real& W, Real u,
+                                             F&& overlap) {
+  constexpr int kCdfWalk = 3;
+  const Rec32* __restrict__ t = d.tgt64 + beg;
+  const int n = (int)len;
+  int k = (int)(u * (Real)n);
+  k = k < n - 1 ? k : n - 1;
+  Rec32 r = t[k];
+  uint2 l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
+  overlap();
+  for (int move = 0;; ++move) {
+    const Real ck = (Real)__hiloint2double((int)r.b.w, (int)r.b.z);
+    const Real cl = k > 0 ? (Real)__hiloint2double((int)l.y, (int)l.x) : (Real)-1;
+    const bool right = ck < u;  // then k < n - 1, as cdf[n - 1] = 1 > u
+    if (!right && !(cl >= u)) break;
+    if (move == kCdfWalk) {
+      const Real* __restrict__ c = Tables<Real>::cdf(d) + beg;
+      k = right ? k + 1 + lower_bound_cdf<Real>(c + k + 1, n - k - 1, u) : lower_bound_cdf<Real>(c, k - 1, u);
+      r = t[k];
+      break;
+    }
+    k += right ? 1 : -1;  // cl >= u implies k > 0
+    r = t[k];
+    l = *(const uint2*)&t[k > 0 ? k - 1 : 0].b.z;
+  }
+  beg = r.a.y;
+  len = r.a.z;
+  W = (Real)__hiloint2double((int)r.b.y, (int)r.b.x);
+  return r.a.x;
+}
+
 // --------------------------------------------------------------------------
 // Walk kernel.  Each lane owns one live history at a time.  Lanes whose
 // history ended are refilled together from a per-XCD work queue: the wave
@@ -228,7 +231,6 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   using Acc = typename std::conditional<TILE, double, Real>::type;
   const DomainView& d = a.d;
   const int n = d.n;
-  const Real* __restrict__ cdf = Tables<Real>::cdf(d);
   const Real* __restrict__ ht = Tables<Real>::h(d);
   Acc* __restrict__ tally = TILE ? (Acc*)a.tally64 : (Acc*)Tables<Real>::tally(a);
   const uint32_t* __restrict__ colp = d.col;
@@ -454,15 +456,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       } else if constexpr (SAMP == SAMPLER_ALIAS) {
         cp = Row<Real>::pick(d, beg, len, r, tally_step);
         w = w * Wrow;
-      } else if constexpr (sizeof(Real) == 8) {
-        const Real wn = w * Wrow;  // W of the current row; the tally still sees w
-        cp = Row<double>::cdf_step(d, beg, len, Wrow, Tables<Real>::u01(r), tally_step);
-        w = wn;
       } else {
-        const uint32_t k = beg + (uint32_t)lower_bound_guess<Real>(cdf + beg, (int)len, Tables<Real>::u01(r),
-                                                                   tally_step);
-        w = w * Wrow;
-        cp = Row<Real>::target(d, k, beg, len, Wrow);
+        const Real wn = w * Wrow;  // W of the current row; the tally still sees w
+        cp = cdf_step<Real>(d, beg, len, Wrow, Tables<Real>::u01(r), tally_step);
+        w = wn;
       }
       if (cp & 0x80000000u) w = -w;
       state = (int)(cp & 0x7FFFFFFFu);
