@@ -789,82 +789,183 @@ __global__ void k_copy_rows(int n, const int* __restrict__ rp, const int* __rest
 // Per domain row: order entries by source row (insertion sort; the scatter
 // order is arbitrary), check duplicates, count the entries that survive the
 // composite threshold and the DBL_EPSILON drop.
-__global__ void k_domain_rows(int n, const int* __restrict__ t_rp, int* __restrict__ t_src,
-                              double* __restrict__ t_val, double omega, double threshold, int has_prec,
-                              int* __restrict__ keep, int* flags) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int beg = t_rp[i], end = t_rp[i + 1];
-    for (int p = beg + 1; p < end; ++p) {
-      int s = t_src[p];
-      double v = t_val[p];
-      int q = p - 1;
-      while (q >= beg && t_src[q] > s) {
-        t_src[q + 1] = t_src[q];
-        t_val[q + 1] = t_val[q];
-        --q;
-      }
-      t_src[q + 1] = s;
-      t_val[q + 1] = v;
+// One domain row by one lane (rows longer than a wave): sort the
+// transposed row by source index, then count the entries that survive the
+// multiply's filter (with a preconditioner) and the DBL_EPSILON drop.
+__device__ void domain_row_serial(int i, const int* __restrict__ t_rp, int* __restrict__ t_src,
+                                  double* __restrict__ t_val, double omega, double threshold, int has_prec,
+                                  int* __restrict__ keep, int* flags) {
+  const int beg = t_rp[i], end = t_rp[i + 1];
+  for (int p = beg + 1; p < end; ++p) {
+    int s = t_src[p];
+    double v = t_val[p];
+    int q = p - 1;
+    while (q >= beg && t_src[q] > s) {
+      t_src[q + 1] = t_src[q];
+      t_val[q + 1] = t_val[q];
+      --q;
     }
-    double thr = 0.0;
-    const bool filter = has_prec && end > beg;
-    if (filter) {
-      double mx = t_val[beg];
-      for (int p = beg + 1; p < end; ++p)
-        if (t_val[p] > mx) mx = t_val[p];
-      thr = mx * threshold;
-    }
-    int kept = 0;
-    for (int p = beg; p < end; ++p) {
-      if (p > beg && t_src[p] == t_src[p - 1]) set_flag(flags, FLAG_DUPLICATE, i);
-      double m = t_val[p];
-      if (filter && fabs(m) <= thr) continue;
-      double h = (t_src[p] == i) ? 1.0 - omega * m : -omega * m;
-      if (fabs(h) < DBL_EPSILON) continue;
-      ++kept;
-    }
-    keep[i] = kept;
+    t_src[q + 1] = s;
+    t_val[q + 1] = v;
+  }
+  double thr = 0.0;
+  const bool filter = has_prec && end > beg;
+  if (filter) {
+    double mx = t_val[beg];
+    for (int p = beg + 1; p < end; ++p)
+      if (t_val[p] > mx) mx = t_val[p];
+    thr = mx * threshold;
+  }
+  int kept = 0;
+  for (int p = beg; p < end; ++p) {
+    if (p > beg && t_src[p] == t_src[p - 1]) set_flag(flags, FLAG_DUPLICATE, i);
+    double m = t_val[p];
+    if (filter && fabs(m) <= thr) continue;
+    double h = (t_src[p] == i) ? 1.0 - omega * m : -omega * m;
+    if (fabs(h) < DBL_EPSILON) continue;
+    ++kept;
+  }
+  keep[i] = kept;
+}
+
+// The sorted row's entries h = delta - omega m that survive, their packed
+// columns, the sequential |h| prefix sums normalised to the CDF, and W.
+__device__ void domain_fill_serial(int i, const int* __restrict__ t_rp, const int* __restrict__ t_src,
+                                   const double* __restrict__ t_val, double omega, double threshold, int has_prec,
+                                   const int* __restrict__ d_rp, uint32_t* __restrict__ col,
+                                   double* __restrict__ h64, float* __restrict__ h32, double* __restrict__ cdf64,
+                                   float* __restrict__ cdf32, double* __restrict__ W64, float* __restrict__ W32) {
+  const int beg = t_rp[i], end = t_rp[i + 1];
+  double thr = 0.0;
+  const bool filter = has_prec && end > beg;
+  if (filter) {
+    double mx = t_val[beg];
+    for (int p = beg + 1; p < end; ++p)
+      if (t_val[p] > mx) mx = t_val[p];
+    thr = mx * threshold;
+  }
+  int out = d_rp[i];
+  const int obeg = out;
+  double acc = 0.0;
+  for (int p = beg; p < end; ++p) {
+    double m = t_val[p];
+    if (filter && fabs(m) <= thr) continue;
+    int j = t_src[p];
+    double h = (j == i) ? 1.0 - omega * m : -omega * m;
+    if (fabs(h) < DBL_EPSILON) continue;
+    col[out] = (uint32_t)j | (h < 0.0 ? 0x80000000u : 0u);
+    h64[out] = h;
+    h32[out] = (float)h;
+    acc = fabs(h) + acc;
+    cdf64[out] = acc;
+    ++out;
+  }
+  W64[i] = acc;
+  W32[i] = (float)acc;
+  for (int p = obeg; p < out; ++p) {
+    double c = cdf64[p] / acc;
+    cdf64[p] = c;
+    cdf32[p] = (float)c;
   }
 }
 
-__global__ void k_domain_fill(int n, const int* __restrict__ t_rp, const int* __restrict__ t_src,
-                              const double* __restrict__ t_val, double omega, double threshold,
-                              int has_prec, const int* __restrict__ d_rp, uint32_t* __restrict__ col,
-                              double* __restrict__ h64, float* __restrict__ h32,
-                              double* __restrict__ cdf64, float* __restrict__ cdf32,
-                              double* __restrict__ W64, float* __restrict__ W32) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int beg = t_rp[i], end = t_rp[i + 1];
-    double thr = 0.0;
-    const bool filter = has_prec && end > beg;
-    if (filter) {
-      double mx = t_val[beg];
-      for (int p = beg + 1; p < end; ++p)
-        if (t_val[p] > mx) mx = t_val[p];
-      thr = mx * threshold;
+// Wave-wide maximum over the active lanes' values (-inf elsewhere).
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_xor(v, o);
+    v = t > v ? t : v;
+  }
+  return v;
+}
+
+// One domain row per wave (rows of <= 64 entries; longer rows go to lane 0
+// and the serial code).  Lane k holds entry k; its sorted position is the
+// number of smaller source indices in the row.
+__global__ __launch_bounds__(256) void k_domain_rows(int n, const int* __restrict__ t_rp, int* __restrict__ t_src,
+                                                     double* __restrict__ t_val, double omega, double threshold,
+                                                     int has_prec, int* __restrict__ keep, int* flags) {
+  const int lane = __lane_id();
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
+    const int beg = t_rp[i], z = t_rp[i + 1] - beg;
+    if (z > 64) {
+      if (lane == 0) domain_row_serial(i, t_rp, t_src, t_val, omega, threshold, has_prec, keep, flags);
+      continue;
     }
-    int out = d_rp[i];
-    const int obeg = out;
-    double acc = 0.0;
-    for (int p = beg; p < end; ++p) {
-      double m = t_val[p];
-      if (filter && fabs(m) <= thr) continue;
-      int j = t_src[p];
-      double h = (j == i) ? 1.0 - omega * m : -omega * m;
-      if (fabs(h) < DBL_EPSILON) continue;
+    const bool on = lane < z;
+    const int key = on ? t_src[beg + lane] : INT_MAX;
+    const double v = on ? t_val[beg + lane] : 0.0;
+    int rank = 0;
+    bool dup = false;
+    for (int k = 0; k < z; ++k) {
+      const int kk = __shfl(key, k);
+      rank += kk < key ? 1 : 0;
+      dup = dup || (kk == key && k != lane);
+    }
+    if (on && dup) set_flag(flags, FLAG_DUPLICATE, i);
+    if (on) {
+      t_src[beg + rank] = key;
+      t_val[beg + rank] = v;
+    }
+    const bool filter = has_prec && z > 0;
+    const double thr = filter ? wave_max(on ? v : -INFINITY) * threshold : 0.0;
+    const double h = key == i ? 1.0 - omega * v : -omega * v;
+    const bool kept = on && !(filter && fabs(v) <= thr) && !(fabs(h) < DBL_EPSILON);
+    const int cnt = __popcll(__ballot(kept));
+    if (lane == 0) keep[i] = cnt;
+  }
+}
+
+// One sorted row per wave: kept entries are compacted by ballot rank; each
+// lane forms its CDF prefix by adding the kept |h| of lanes 0..itself in
+// row order, the same additions in the same order as the serial loop.
+__global__ __launch_bounds__(256) void k_domain_fill(int n, const int* __restrict__ t_rp,
+                                                     const int* __restrict__ t_src,
+                                                     const double* __restrict__ t_val, double omega,
+                                                     double threshold, int has_prec, const int* __restrict__ d_rp,
+                                                     uint32_t* __restrict__ col, double* __restrict__ h64,
+                                                     float* __restrict__ h32, double* __restrict__ cdf64,
+                                                     float* __restrict__ cdf32, double* __restrict__ W64,
+                                                     float* __restrict__ W32) {
+  const int lane = __lane_id();
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n; i += nw) {
+    const int beg = t_rp[i], z = t_rp[i + 1] - beg;
+    if (z > 64) {
+      if (lane == 0)
+        domain_fill_serial(i, t_rp, t_src, t_val, omega, threshold, has_prec, d_rp, col, h64, h32, cdf64, cdf32,
+                           W64, W32);
+      continue;
+    }
+    const bool on = lane < z;
+    const int j = on ? t_src[beg + lane] : 0;
+    const double m = on ? t_val[beg + lane] : 0.0;
+    const bool filter = has_prec && z > 0;
+    const double thr = filter ? wave_max(on ? m : -INFINITY) * threshold : 0.0;
+    const double h = j == i ? 1.0 - omega * m : -omega * m;
+    const bool kept = on && !(filter && fabs(m) <= thr) && !(fabs(h) < DBL_EPSILON);
+    const uint64_t km = __ballot(kept);
+    const double ah = kept ? fabs(h) : 0.0;
+    double pre = 0.0, tot = 0.0;
+    for (int k = 0; k < z; ++k) {
+      const double hk = __shfl(ah, k);
+      if ((km >> k) & 1ull) {
+        tot = hk + tot;
+        if (k <= lane) pre = hk + pre;
+      }
+    }
+    if (kept) {
+      const int out = d_rp[i] + __popcll(km & __lanemask_lt());
       col[out] = (uint32_t)j | (h < 0.0 ? 0x80000000u : 0u);
       h64[out] = h;
       h32[out] = (float)h;
-      acc = fabs(h) + acc;
-      cdf64[out] = acc;
-      ++out;
+      const double c = pre / tot;
+      cdf64[out] = c;
+      cdf32[out] = (float)c;
     }
-    W64[i] = acc;
-    W32[i] = (float)acc;
-    for (int p = obeg; p < out; ++p) {
-      double c = cdf64[p] / acc;
-      cdf64[p] = c;
-      cdf32[p] = (float)c;
+    if (lane == 0) {
+      W64[i] = tot;
+      W32[i] = (float)tot;
     }
   }
 }
@@ -1768,16 +1869,17 @@ hipError_t launch_copy_rows(int n, const int* rp, const int* col, const void* va
 
 hipError_t launch_domain_rows(int n, const int* t_rp, int* t_src, double* t_val, double omega, double threshold,
                               bool has_prec, int* keep, int* flags, hipStream_t s) {
-  k_domain_rows<<<grid_for(n, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold, has_prec ? 1 : 0,
-                                                  keep, flags);
+  k_domain_rows<<<grid_for((int64_t)n * 64, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold,
+                                                                 has_prec ? 1 : 0, keep, flags);
   return hipGetLastError();
 }
 
 hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const double* t_val, double omega,
                               double threshold, bool has_prec, const int* d_rp, uint32_t* col, double* h64,
                               float* h32, double* cdf64, float* cdf32, double* W64, float* W32, hipStream_t s) {
-  k_domain_fill<<<grid_for(n, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold, has_prec ? 1 : 0, d_rp,
-                                                  col, h64, h32, cdf64, cdf32, W64, W32);
+  k_domain_fill<<<grid_for((int64_t)n * 64, 256), 256, 0, s>>>(n, t_rp, t_src, t_val, omega, threshold,
+                                                                 has_prec ? 1 : 0, d_rp, col, h64, h32, cdf64, cdf32,
+                                                                 W64, W32);
   return hipGetLastError();
 }
 

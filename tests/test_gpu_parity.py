@@ -84,6 +84,43 @@ def test_domain_no_prec_and_random_matrix(ctx):
         assert np.array_equal(h, dom.h) and np.array_equal(cdf, dom.cdf) and np.array_equal(W, dom.W)
 
 
+@pytest.mark.parametrize("forward", [False, True])
+@pytest.mark.parametrize("thr", [0.0, 0.05])
+def test_domain_bitexact_long_rows(ctx, forward, thr):
+    """Domain rows longer than a wave (dense columns and rows of A) take the
+    one-lane path of the row kernels; both paths match the oracle."""
+    rng = np.random.default_rng(12)
+    n = 400
+    S = sp.random(n, n, density=0.03, random_state=rng, format="lil")
+    for j in (3, 150, 399):
+        # magnitudes within 2x of each other, so that the rows stay long
+        # under the 5% filter
+        S[:, j] = 0.5 + 0.5 * rng.random((n, 1))  # columns of A: long adjoint rows
+        S[j, :] = 0.5 + 0.5 * rng.random((1, n))  # rows of A: long forward rows
+    S = sp.csr_matrix(S)
+    S.data = S.data - 0.004
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    S = (S + sp.diags(np.abs(S).sum(axis=1).A1 * 1.3 + 0.5)).tocsr()
+    S.sort_indices()
+    prob = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(np.float64),
+                        None, rng.random(n))
+    _setup(ctx, prob, False, params={"Composite Operator Threshold": thr, "Neumann Relaxation": 0.8,
+                                     "Monte Carlo Type": "Forward" if forward else "Adjoint"})
+    ctx.build_domain()
+    rp, col, h, cdf, W = ctx.domain()
+    ctx.set_params({"Monte Carlo Type": "Adjoint"})
+    A = _csr(prob)
+    dom = po.build_domain(A, po.jacobi_dinv(A), 0.8, thr, forward)
+    # (a forward row holds m_jj = 1 under Jacobi, so a 5% filter leaves the
+    # dense forward rows short; the serial path is then exercised by the
+    # unfiltered rows of the source)
+    if not (forward and thr > 0):
+        assert np.diff(dom.rp).max() > 64
+    assert np.array_equal(rp, dom.rp) and np.array_equal(col, dom.col)
+    assert np.array_equal(h, dom.h) and np.array_equal(cdf, dom.cdf) and np.array_equal(W, dom.W)
+
+
 @pytest.mark.parametrize("sampler", ["CDF", "Alias"])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
