@@ -1303,7 +1303,18 @@ __global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int*
   for (int i = blockIdx.x * RPB + g; i < n; i += gridDim.x * RPB) {
     double s = 0.0;
     const int end = rp[i + 1];
-    for (int k = rp[i] + j; k < end; k += V) s = s + (double)v[k] * x[col[k]];
+    int k = rp[i] + j;
+    // the lane's first U entries: their loads are issued together
+    constexpr int U = 4;
+    double prod[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int kk = k + u * V;
+      prod[u] = kk < end ? (double)v[kk] * x[col[kk]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s = s + prod[u];
+    for (k += U * V; k < end; k += V) s = s + (double)v[k] * x[col[k]];
     if (V > 1)
       for (int off = V / 2; off > 0; off >>= 1) s = s + __shfl_xor(s, off, V);
     if (j == 0) {
@@ -1885,10 +1896,11 @@ hipError_t launch_domain_fill(int n, const int* t_rp, const int* t_src, const do
 
 int residual_lanes(int64_t n, int64_t nnz) {
   const double z = n > 0 ? (double)nnz / (double)n : 0.0;
+  // each lane takes up to 4 entries of a row in one round of loads
   if (z < 3.0) return 1;
-  if (z <= 6.0) return 4;
-  if (z <= 12.0) return 8;
-  if (z <= 24.0) return 16;
+  if (z <= 12.0) return 4;
+  if (z <= 32.0) return 8;
+  if (z <= 64.0) return 16;
   return 32;
 }
 
