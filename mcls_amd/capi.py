@@ -223,6 +223,67 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime the solver library is linked against (loaded with it),
+    for plain device buffers; nothing else in this package allocates device
+    memory from Python."""
+    global _hip
+    if _hip is None:
+        gpu_lib()  # loads libamdhip64.so.7 as the library's dependency
+        L = C.CDLL("libamdhip64.so.7")
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        L.hipDeviceSynchronize.argtypes = []
+        _hip = L
+    return _hip
+
+
+class DeviceArray:
+    """n doubles in device memory (hipMalloc), for solves whose vectors stay
+    resident on the GPU."""
+
+    _H2D, _D2H = 1, 2
+
+    def __init__(self, n):
+        self.L = hip_runtime()
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        if self.L.hipMalloc(C.byref(self.ptr), 8 * max(self.n, 1)) != 0:
+            raise GpuError(MCLS_ERR_HIP, "hipMalloc failed")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.size == self.n
+        if self.L.hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), 8 * self.n, self._H2D) != 0:
+            raise GpuError(MCLS_ERR_HIP, "hipMemcpy to device failed")
+
+    def download(self):
+        a = np.empty(self.n)
+        if self.L.hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, 8 * self.n, self._D2H) != 0:
+            raise GpuError(MCLS_ERR_HIP, "hipMemcpy to host failed")
+        return a
+
+    def zero(self):
+        if self.L.hipMemset(self.ptr, 0, 8 * self.n) != 0 or self.L.hipDeviceSynchronize() != 0:
+            raise GpuError(MCLS_ERR_HIP, "hipMemset failed")
+
+    def close(self):
+        if self.ptr:
+            self.L.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Problem:
     """A generated system: CSR (int64 rp, int32 col, f64 and/or f32 values), b."""
 
@@ -397,6 +458,13 @@ class Context:
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
         self._check(self.L.mcls_gpu_mcsa_solve(self.h, _p(b), _p(x), MCLS_HOST))
         return x
+
+    def mcsa_solve_device(self, b_ptr, x_ptr):
+        """MCSA with b and x (initial guess in, solution out) already in
+        device memory: raw pointers to n doubles each (e.g. a torch tensor's
+        data_ptr(), written before this call)."""
+        self._check(self.L.mcls_gpu_mcsa_solve(self.h, C.c_void_p(int(b_ptr)), C.c_void_p(int(x_ptr)),
+                                               MCLS_DEVICE))
 
     def prec_residual(self, b, x):
         b = np.ascontiguousarray(b, np.float64)

@@ -366,3 +366,28 @@ def test_forward_rejects_expected_value(ctx):
     with pytest.raises(capi.GpuError):
         ctx.set_params({"Monte Carlo Type": "Sideways"})
     ctx.set_params({"Monte Carlo Type": "Adjoint", "Estimator Type": "Collision"})
+
+
+def test_device_pointer_solve_matches_host():
+    """b and x in device memory (ptr_kind MCLS_DEVICE) give the host-pointer
+    solve's result: same streams, only the tally summation order differs."""
+    prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 24, 24, seed=0))
+    params = {"History Length": 5000, "Weight Cutoff": 1e-4, "Number of Histories": 20000,
+              "Convergence Tolerance": 1e-8, "Maximum Iterations": 200, "Random Seed": 3}
+    out = []
+    for device in (False, True):
+        with capi.Context(0) as c:
+            _setup(c, prob, False, params=params)
+            if device:
+                b, x = capi.DeviceArray(prob.n), capi.DeviceArray(prob.n)
+                b.upload(prob.b)
+                x.zero()
+                c.mcsa_solve_device(b.ptr.value, x.ptr.value)
+                out.append((x.download(), c.stats().iterations))
+                b.close()
+                x.close()
+            else:
+                out.append((c.mcsa_solve(prob.b), c.stats().iterations))
+    (xh, ih), (xd, idv) = out
+    assert ih == idv
+    assert np.linalg.norm(xh - xd) <= 1e-10 * np.linalg.norm(xh)
