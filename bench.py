@@ -394,6 +394,8 @@ def main():
                               "bytes_per_step": fbytes, "steps_per_launch": fsteps_per_launch,
                               "avg_launch_ms": fst.ms_walk / max(fst.walk_launches, 1),
                               "speedup_vs_plain": dt / fdt, "tuning": ftune}
+    dev.b.close()
+    dev.x.close()
     ctx.close()
     if not args.no_cpu:
         log("cpu baseline")
