@@ -850,7 +850,7 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, last);
   if (wc.tile) {
-    const size_t elem = sizeof(int) + (c->prm.precision == 32 ? sizeof(float) : sizeof(double));
+    const size_t elem = sizeof(int) + sizeof(double);  // key + FP64 partial sum
     if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile x waves per block exceeds 64 KiB of LDS per workgroup");
   }

@@ -258,12 +258,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const int lane = __lane_id();
 
   int* tkeys = nullptr;  // TILE: this wave's slots
-  Real* tvals = nullptr;
+  double* tvals = nullptr;  // partial sums in FP64, whatever the walk precision
   if constexpr (TILE) {
     extern __shared__ __align__(16) unsigned char walk_lds[];
     const int wv = threadIdx.x >> 6;
     tkeys = (int*)walk_lds + wv * a.tile_slots;
-    tvals = (Real*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
+    tvals = (double*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
     for (int k = lane; k < a.tile_slots; k += 64) {
       tkeys[k] = -1;
       tvals[k] = 0;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       for (int k = lane; k < a.tile_slots; k += 64) {
         const int key = tkeys[k];
         if (key >= 0) {
-          tally_add(&tally[key], (double)tvals[k]);
+          tally_add(&tally[key], tvals[k]);
           tkeys[k] = -1;
           tvals[k] = 0;
         }
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           if (k < 0) k = state;
         }
         if (k == state)
-          atomicAdd(&tvals[slot], w);
+          __hip_atomic_fetch_add(&tvals[slot], (double)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
           tally_add(&tally[state], (Acc)w);
       } else {
@@ -487,7 +487,7 @@ hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
     k_walk<Real, EST, SRC, SAMP, true, false><<<c.grid, c.block, 0, s>>>(a);
   } else if constexpr (EST == EST_COLLISION) {
     if (c.tile) {
-      const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * (sizeof(int) + sizeof(Real));
+      const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * (sizeof(int) + sizeof(double));
       k_walk<Real, EST, SRC, SAMP, false, true><<<c.grid, c.block, lds, s>>>(a);
     } else {
       k_walk<Real, EST, SRC, SAMP, false, false><<<c.grid, c.block, 0, s>>>(a);

@@ -56,8 +56,8 @@ def test_walk_batch_validation(ctx):
 @pytest.mark.parametrize("sampler,prec", [("Alias", 32), ("CDF", 64), ("CDF", 32)])
 def test_tally_tile_matches_direct_tally(ctx, sampler, prec):
     """The LDS tile changes only where the additions happen: same paths, same
-    tallies to the rounding of the summation order (FP32 walks accumulate in
-    FP64 through the tile, so closer to the FP64 answer)."""
+    tallies to the rounding of the summation order (the tile's partial sums
+    are FP64 for every walk, so FP32 walks accumulate in FP64 through it)."""
     prob, rhs = _poisson(ctx, sampler, prec)
     ctx.set_params({"Tally Tile": 0})
     ref = ctx.mc_solve_raw(rhs, 3)
@@ -71,10 +71,10 @@ def test_tally_tile_matches_direct_tally(ctx, sampler, prec):
     ctx.set_params({"Tally Tile": 0, "Walk Block Size": 256})
     with pytest.raises(capi.GpuError):
         ctx.set_params({"Tally Tile": 100})
-    if prec == 64:
-        ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
-        with pytest.raises(capi.GpuError):
-            ctx.mc_solve_raw(rhs, 4)  # 4 waves x 2048 x 12 bytes > 64 KiB
+    # every walk keeps FP64 partial sums in the tile: 4 waves x 2048 x 12 bytes > 64 KiB
+    ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
+    with pytest.raises(capi.GpuError):
+        ctx.mc_solve_raw(rhs, 4)
     ctx.set_params({"Tally Tile": 0})
 
 
