@@ -77,6 +77,21 @@ const char* mcls_gpu_last_error(void);
 int mcls_gpu_synchronize(mcls_gpu_context* ctx);
 
 /* ------------------------------------------------------------ problem -- */
+/* State order (no counterpart in the reference; MCLS walks the operator in
+ * the caller's numbering): the context numbers the unknowns its own way,
+ * internal state k being the caller's unknown order[k] (a permutation of
+ * 0..n-1, host or device array), and works on P A P^T throughout.  An order
+ * that keeps the states a walk visits together close in memory (bricks of a
+ * structured grid, say) raises the walks' L2 hit rate.  Every vector passed
+ * through this API stays in the caller's numbering (gathered on the way in,
+ * scattered on the way out, on the context's stream), and so do the states
+ * of mcls_gpu_record_paths; mcls_gpu_get_domain and the features' neighbour
+ * classes (mcls_gpu_importance_features) are in the internal numbering.
+ * Call it before mcls_gpu_set_matrix_csr: it drops the operator and what
+ * was set for it (preconditioners, importance, mask, systems).  NULL returns
+ * to the caller's numbering.  Batched systems are refused while an order is
+ * set (MCLS_ERR_UNSUPPORTED). */
+int mcls_gpu_set_state_order(mcls_gpu_context* ctx, int64_t n, const int32_t* order, int ptr_kind);
 /* LinearProblem operator A (src/MCLS_LinearProblem.hpp:74-91).  n x n CSR,
  * int64 row pointers, int32 columns (each row's columns unique), values
  * FP64 or FP32 (`val_dtype`).  The library copies the arrays (host or

@@ -41,6 +41,7 @@ GPU_SYMBOLS = [
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
     "mcls_gpu_set_left_prec_matrix", "mcls_gpu_set_right_prec_matrix", "mcls_gpu_clear_right_prec",
+    "mcls_gpu_set_state_order",
 ]
 MLP_SYMBOLS = [
     "mcls_mlp_create", "mcls_mlp_destroy", "mcls_mlp_get_params", "mcls_mlp_set_params",
@@ -150,6 +151,7 @@ def gpu_lib():
     L.mcls_gpu_last_error.restype = C.c_char_p
     L.mcls_gpu_synchronize.argtypes = [vp]
     L.mcls_gpu_set_matrix_csr.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32]
+    L.mcls_gpu_set_state_order.argtypes = [vp, i64, vp, i32]
     L.mcls_gpu_set_left_prec_diag.argtypes = [vp, vp, i32]
     L.mcls_gpu_set_left_prec_point_jacobi.argtypes = [vp]
     L.mcls_gpu_clear_left_prec.argtypes = [vp]
@@ -306,6 +308,24 @@ def make_spec(kind, nx, ny=0, nz=0, seed=0, p0=0.0, p1=0.0, p2=0.0, p3=0.0):
     return ProblemSpec(kind, nx, ny, nz, seed & (2**64 - 1), p0, p1, p2, p3)
 
 
+def brick_order(nx, ny, nz, bx, by, bz):
+    """State order of an nx x ny x nz grid (unknown x + nx (y + ny z))
+    visited brick by brick: bricks of bx x by x bz in x-fastest order, the
+    points of a brick x-fastest inside it.  Bricks at the high faces are cut
+    to fit.  Returns order[k] = the unknown stored k-th (int32)."""
+    if min(nx, ny, nz, bx, by, bz) < 1:
+        raise ValueError("grid and brick sizes must be >= 1")
+    x = np.arange(nx, dtype=np.int64)
+    y = np.arange(ny, dtype=np.int64)
+    z = np.arange(nz, dtype=np.int64)
+    # sort key: (brick z, brick y, brick x, z, y, x) of every unknown
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    nbx, nby = -(-nx // bx), -(-ny // by)
+    brick = (Z // bz * nby + Y // by) * nbx + X // bx
+    key = np.lexsort(((X % bx).ravel(), (Y % by).ravel(), (Z % bz).ravel(), brick.ravel()))
+    return key.astype(np.int32)
+
+
 def build_problem(spec, want64=True, want32=False, nthreads=None):
     L = problems_lib()
     n = C.c_int64(0)
@@ -376,6 +396,17 @@ class Context:
         self._check(self.L.mcls_gpu_set_matrix_csr(self.h, n, int(rp[-1]), _p(rp), _p(col), _p(val),
                                                    dt, MCLS_HOST))
         self.n = n
+
+    def set_state_order(self, order):
+        """Number the unknowns in `order` (internal state k = unknown
+        order[k]); None returns to the caller's numbering.  Drops the
+        operator: call before set_matrix / set_problem."""
+        if order is None:
+            self._check(self.L.mcls_gpu_set_state_order(self.h, 0, None, MCLS_HOST))
+            return
+        order = np.ascontiguousarray(order, np.int32)
+        self._order_keep = order
+        self._check(self.L.mcls_gpu_set_state_order(self.h, len(order), _p(order), MCLS_HOST))
 
     def set_problem(self, prob, dtype="f64"):
         val = prob.val32 if dtype == "f32" else prob.values()

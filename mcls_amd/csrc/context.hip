@@ -7,6 +7,7 @@
 // src/MCLS_MonteCarloSolverManager_impl.hpp:200-235,347-380.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cfloat>
 #include <climits>
@@ -129,6 +130,12 @@ struct mcls_gpu_context {
     bool on = false;
   } pl_csr, pr_csr;
   DBuf<double> pt1, pt2;  // work vectors of PL (b - A PR x)
+  // state order (mcls_gpu_set_state_order): internal state k is the caller's
+  // unknown ord_h[k]; vectors cross the C ABI through a gather or a scatter
+  bool ordered = false;
+  std::vector<int> ord_h, inv_h;
+  DBuf<int> ord, ord_inv;
+  DBuf<char> ustage;  // host vectors on their way through the permutation
 
   // domain
   bool domain_valid = false;
@@ -232,6 +239,43 @@ hipMemcpyKind kind_from_dev(int ptr_kind) {
 int check_kind(int ptr_kind) {
   if (ptr_kind != MCLS_HOST && ptr_kind != MCLS_DEVICE)
     return fail(MCLS_ERR_INVALID_ARG, "ptr_kind must be MCLS_HOST or MCLS_DEVICE");
+  return MCLS_OK;
+}
+
+// A caller's vector (n rows of w elements of elem bytes) into the internal
+// numbering on the context's stream: dst[k] = src[ord[k]].
+int vec_in(mcls_gpu_context* c, const void* src, int ptr_kind, void* dst, int w, size_t elem) {
+  const size_t bytes = (size_t)c->n * (size_t)w * elem;
+  if (!c->ordered) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind_to_dev(ptr_kind), c->stream));
+    return MCLS_OK;
+  }
+  const void* from = src;
+  if (ptr_kind != MCLS_DEVICE) {
+    if (c->ustage.n < bytes) HIP_TRY(hipStreamSynchronize(c->stream));
+    MCLS_TRY(c->ustage.ensure(bytes, "order staging"));
+    HIP_TRY(hipMemcpyAsync(c->ustage.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    from = c->ustage.p;
+  }
+  HIP_TRY(mcls::launch_gather_rows(c->n, w, elem, c->ord.p, from, dst, c->stream));
+  return MCLS_OK;
+}
+
+// An internal vector out to the caller's numbering: dst[ord[k]] = src[k].
+int vec_out(mcls_gpu_context* c, const void* src, void* dst, int ptr_kind, int w, size_t elem) {
+  const size_t bytes = (size_t)c->n * (size_t)w * elem;
+  if (!c->ordered) {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind_from_dev(ptr_kind), c->stream));
+    return MCLS_OK;
+  }
+  if (ptr_kind == MCLS_DEVICE) {
+    HIP_TRY(mcls::launch_scatter_rows(c->n, w, elem, c->ord.p, src, dst, c->stream));
+    return MCLS_OK;
+  }
+  if (c->ustage.n < bytes) HIP_TRY(hipStreamSynchronize(c->stream));
+  MCLS_TRY(c->ustage.ensure(bytes, "order staging"));
+  HIP_TRY(mcls::launch_scatter_rows(c->n, w, elem, c->ord.p, src, c->ustage.p, c->stream));
+  HIP_TRY(hipMemcpyAsync(dst, c->ustage.p, bytes, hipMemcpyDeviceToHost, c->stream));
   return MCLS_OK;
 }
 
@@ -1147,6 +1191,47 @@ int mcls_gpu_synchronize(mcls_gpu_context* c) {
   return MCLS_OK;
 }
 
+int mcls_gpu_set_state_order(mcls_gpu_context* c, int64_t n, const int32_t* order, int ptr_kind) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_kind(ptr_kind));
+  if (order && (n <= 0 || n > INT32_MAX - 1)) return fail(MCLS_ERR_INVALID_ARG, "n must be in [1, 2^31-2]");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the operator and everything set for it were numbered the old way
+  c->has_matrix = false;
+  c->domain_valid = false;
+  c->has_prec = false;
+  c->pl_csr.on = false;
+  c->pr_csr.on = false;
+  c->has_importance = false;
+  c->proposal_valid = false;
+  c->has_mask = false;
+  c->nsys = 0;
+  invalidate_graph(c);
+  c->ordered = false;
+  c->ord_h.clear();
+  c->inv_h.clear();
+  if (!order) return MCLS_OK;
+  std::vector<int> o((size_t)n), inv((size_t)n, -1);
+  HIP_TRY(hipMemcpy(o.data(), order, sizeof(int) * (size_t)n,
+                    ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+  for (int64_t k = 0; k < n; ++k) {
+    const int v = o[(size_t)k];
+    if (v < 0 || v >= n) return fail(MCLS_ERR_INVALID_ARG, "state order entry " + std::to_string(k) + " out of range");
+    if (inv[(size_t)v] >= 0)
+      return fail(MCLS_ERR_INVALID_ARG, "state order names unknown " + std::to_string(v) + " twice");
+    inv[(size_t)v] = (int)k;
+  }
+  MCLS_TRY(c->ord.ensure((size_t)n, "state order"));
+  MCLS_TRY(c->ord_inv.ensure((size_t)n, "state order inverse"));
+  MCLS_TRY(c->ustage.ensure((size_t)n * sizeof(double), "order staging"));
+  HIP_TRY(hipMemcpy(c->ord.p, o.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->ord_inv.p, inv.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+  c->ord_h.swap(o);
+  c->inv_h.swap(inv);
+  c->ordered = true;
+  return MCLS_OK;
+}
+
 int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const int64_t* row_ptr,
                             const int32_t* col, const void* val, int val_dtype, int ptr_kind) {
   MCLS_TRY(check_ctx(c));
@@ -1155,6 +1240,8 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   if (nnz < 0 || nnz > INT32_MAX) return fail(MCLS_ERR_INVALID_ARG, "nnz must be in [0, 2^31-1]");
   if (!row_ptr || (nnz > 0 && (!col || !val))) return fail(MCLS_ERR_INVALID_ARG, "null CSR array");
   if (val_dtype != MCLS_F64 && val_dtype != MCLS_F32) return fail(MCLS_ERR_INVALID_ARG, "val_dtype");
+  if (c->ordered && n != (int64_t)c->ord_h.size())
+    return fail(MCLS_ERR_INVALID_ARG, "n differs from the length of the state order");
   // Row pointers are validated on the host (monotone, in range) and narrowed
   // to int32.
   std::vector<int64_t> rp64((size_t)n + 1);
@@ -1173,20 +1260,46 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   MCLS_TRY(c->a_rp.ensure((size_t)n + 1, "A rp"));
   MCLS_TRY(c->a_col.ensure((size_t)nnz, "A col"));
   MCLS_TRY(c->a_val.ensure((size_t)nnz * vsz, "A val"));
-  HIP_TRY(hipMemcpy(c->a_rp.p, rp32.data(), sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice));
+  // With a state order the caller's arrays are validated where they land,
+  // then permuted into the context's numbering (P A P^T).
+  DBuf<int> src_rp, src_col;
+  DBuf<char> src_val;
+  int* up_rp = c->a_rp.p;
+  int* up_col = c->a_col.p;
+  char* up_val = c->a_val.p;
+  if (c->ordered) {
+    MCLS_TRY(src_rp.ensure((size_t)n + 1, "A rp (caller order)"));
+    MCLS_TRY(src_col.ensure((size_t)nnz, "A col (caller order)"));
+    MCLS_TRY(src_val.ensure((size_t)nnz * vsz, "A val (caller order)"));
+    up_rp = src_rp.p;
+    up_col = src_col.p;
+    up_val = src_val.p;
+  }
+  HIP_TRY(hipMemcpy(up_rp, rp32.data(), sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice));
   if (nnz > 0) {
-    HIP_TRY(hipMemcpy(c->a_col.p, col, sizeof(int) * (size_t)nnz, kind_to_dev(ptr_kind)));
-    HIP_TRY(hipMemcpy(c->a_val.p, val, vsz * (size_t)nnz, kind_to_dev(ptr_kind)));
+    HIP_TRY(hipMemcpy(up_col, col, sizeof(int) * (size_t)nnz, kind_to_dev(ptr_kind)));
+    HIP_TRY(hipMemcpy(up_val, val, vsz * (size_t)nnz, kind_to_dev(ptr_kind)));
   }
   MCLS_TRY(reset_flags(c));
-  HIP_TRY(mcls::launch_validate_csr((int)n, c->a_rp.p, c->a_col.p, c->a_val.p, val_dtype == MCLS_F32, c->flags.p,
-                                    c->stream));
+  HIP_TRY(mcls::launch_validate_csr((int)n, up_rp, up_col, up_val, val_dtype == MCLS_F32, c->flags.p, c->stream));
   int flags[mcls::FLAG_COUNT];
   MCLS_TRY(read_flags(c, flags));
   if (flags[mcls::FLAG_BAD_COLUMN] != INT_MAX)
     return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(flags[mcls::FLAG_BAD_COLUMN] - 1));
   if (flags[mcls::FLAG_NONFINITE] != INT_MAX)
     return fail(MCLS_ERR_MATRIX, "non-finite matrix value in row " + std::to_string(flags[mcls::FLAG_NONFINITE] - 1));
+  if (c->ordered) {
+    std::vector<int> rpp((size_t)n + 1);
+    rpp[0] = 0;
+    for (int64_t k = 0; k < n; ++k) {
+      const int r = c->ord_h[(size_t)k];
+      rpp[(size_t)k + 1] = rpp[(size_t)k] + (rp32[(size_t)r + 1] - rp32[(size_t)r]);
+    }
+    HIP_TRY(hipMemcpy(c->a_rp.p, rpp.data(), sizeof(int) * (size_t)(n + 1), hipMemcpyHostToDevice));
+    HIP_TRY(mcls::launch_permute_csr((int)n, c->ord.p, c->ord_inv.p, src_rp.p, src_col.p, src_val.p,
+                                     val_dtype == MCLS_F32, c->a_rp.p, c->a_col.p, c->a_val.p, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   c->n = (int)n;
   c->nnz = nnz;
   c->f32 = val_dtype == MCLS_F32;
@@ -1209,7 +1322,8 @@ int mcls_gpu_set_left_prec_diag(mcls_gpu_context* c, const double* dinv, int ptr
   MCLS_TRY(check_matrix(c));
   if (!dinv) return fail(MCLS_ERR_INVALID_ARG, "null dinv");
   MCLS_TRY(c->dinv.ensure((size_t)c->n, "dinv"));
-  HIP_TRY(hipMemcpy(c->dinv.p, dinv, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind)));
+  MCLS_TRY(vec_in(c, dinv, ptr_kind, c->dinv.p, 1, sizeof(double)));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->has_prec = true;
   c->pl_csr.on = false;
   c->domain_valid = false;
@@ -1271,6 +1385,42 @@ int set_prec_csr(mcls_gpu_context* c, mcls_gpu_context::PrecCsr& P, const int64_
     for (int k = rp32[(size_t)i] + 1; k < rp32[(size_t)i + 1]; ++k)
       if (col_h[(size_t)k] <= col_h[(size_t)k - 1])
         return fail(MCLS_ERR_MATRIX, "preconditioner columns not strictly ascending in row " + std::to_string(i));
+  // With a state order: P M P^T on the host, each row re-sorted by column.
+  std::vector<double> val_h;
+  const double* val_src = val;
+  int val_kind = ptr_kind;
+  if (c->ordered && nnz > 0) {
+    for (int i = 0; i < n; ++i)
+      for (int k = rp32[(size_t)i]; k < rp32[(size_t)i + 1]; ++k)
+        if (col_h[(size_t)k] < 0 || col_h[(size_t)k] >= n)
+          return fail(MCLS_ERR_MATRIX, "column index out of range in row " + std::to_string(i));
+    std::vector<double> v0((size_t)nnz);
+    HIP_TRY(hipMemcpy(v0.data(), val, sizeof(double) * (size_t)nnz, to_host));
+    std::vector<int> rpp((size_t)n + 1), colp((size_t)nnz);
+    val_h.resize((size_t)nnz);
+    std::vector<std::pair<int, double>> row;
+    rpp[0] = 0;
+    for (int k = 0; k < n; ++k) {
+      const int r = c->ord_h[(size_t)k];
+      row.clear();
+      for (int q = rp32[(size_t)r]; q < rp32[(size_t)r + 1]; ++q)
+        row.emplace_back(c->inv_h[(size_t)col_h[(size_t)q]], v0[(size_t)q]);
+      std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+        return a.first < b.first;
+      });
+      int o = rpp[(size_t)k];
+      for (const auto& e : row) {
+        colp[(size_t)o] = e.first;
+        val_h[(size_t)o] = e.second;
+        ++o;
+      }
+      rpp[(size_t)k + 1] = o;
+    }
+    rp32.swap(rpp);
+    col_h.swap(colp);
+    val_src = val_h.data();
+    val_kind = MCLS_HOST;
+  }
   P.on = false;
   MCLS_TRY(P.rp.ensure((size_t)n + 1, "prec rp"));
   MCLS_TRY(P.col.ensure((size_t)(nnz > 0 ? nnz : 1), "prec col"));
@@ -1278,7 +1428,7 @@ int set_prec_csr(mcls_gpu_context* c, mcls_gpu_context::PrecCsr& P, const int64_
   HIP_TRY(hipMemcpy(P.rp.p, rp32.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
   if (nnz > 0) {
     HIP_TRY(hipMemcpy(P.col.p, col_h.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(P.val.p, val, sizeof(double) * (size_t)nnz, kind_to_dev(ptr_kind)));
+    HIP_TRY(hipMemcpy(P.val.p, val_src, sizeof(double) * (size_t)nnz, kind_to_dev(val_kind)));
   }
   MCLS_TRY(reset_flags(c));
   HIP_TRY(mcls::launch_validate_csr(n, P.rp.p, P.col.p, P.val.p, false, c->flags.p, c->stream));
@@ -1324,7 +1474,8 @@ int mcls_gpu_get_left_prec_diag(mcls_gpu_context* c, double* dinv, int ptr_kind)
   if (!c->has_prec) return fail(MCLS_ERR_STATE, c->pl_csr.on ? "the left preconditioner is not diagonal"
                                                            : "no left preconditioner");
   if (!dinv) return fail(MCLS_ERR_INVALID_ARG, "null dinv");
-  HIP_TRY(hipMemcpy(dinv, c->dinv.p, sizeof(double) * (size_t)c->n, kind_from_dev(ptr_kind)));
+  MCLS_TRY(vec_out(c, c->dinv.p, dinv, ptr_kind, 1, sizeof(double)));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;
 }
 
@@ -1487,11 +1638,10 @@ int mcls_gpu_prec_residual(mcls_gpu_context* c, const double* b, const double* x
   MCLS_TRY(check_kind(ptr_kind));
   MCLS_TRY(check_matrix(c));
   if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
-  const size_t bytes = sizeof(double) * (size_t)c->n;
-  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), c->stream));
-  HIP_TRY(hipMemcpyAsync(c->x.p, x, bytes, kind_to_dev(ptr_kind), c->stream));
+  MCLS_TRY(vec_in(c, b, ptr_kind, c->b.p, 1, sizeof(double)));
+  MCLS_TRY(vec_in(c, x, ptr_kind, c->x.p, 1, sizeof(double)));
   MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
-  if (r) HIP_TRY(hipMemcpyAsync(r, c->r.p, bytes, kind_from_dev(ptr_kind), c->stream));
+  if (r) MCLS_TRY(vec_out(c, c->r.p, r, ptr_kind, 1, sizeof(double)));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (norm2) *norm2 = std::sqrt(c->host_scal[0]);
   return MCLS_OK;
@@ -1503,13 +1653,12 @@ int mcls_gpu_mc_solve_raw(mcls_gpu_context* c, const double* rhs, uint64_t solve
   if (!rhs || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
-  const size_t bytes = sizeof(double) * (size_t)c->n;
   c->stats.walk_launches = 0;
   c->stats.ms_walk = c->stats.ms_source = 0.0;
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->rhs.p, rhs, bytes, hipMemcpyHostToDevice, c->stream));
+  MCLS_TRY(vec_in(c, rhs, MCLS_HOST, c->rhs.p, 1, sizeof(double)));
   MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index));
-  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  MCLS_TRY(vec_out(c, c->x.p, x, MCLS_HOST, 1, sizeof(double)));
   unsigned long long steps = 0;
   HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1530,7 +1679,6 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
   hipStream_t s = c->stream;
-  const size_t bytes = sizeof(double) * (size_t)c->n;
   DBuf<int> dpaths;
   DBuf<double> dw;
   MCLS_TRY(dpaths.ensure((size_t)count * (size_t)(max_len + 1), "paths"));
@@ -1538,7 +1686,7 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   HIP_TRY(mcls::launch_fill_i32(dpaths.p, count * (int64_t)(max_len + 1), -1, s));
   HIP_TRY(mcls::launch_fill_f64(dw.p, count, 0.0, s));
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
-  HIP_TRY(hipMemcpyAsync(c->rhs.p, rhs, bytes, hipMemcpyHostToDevice, s));
+  MCLS_TRY(vec_in(c, rhs, MCLS_HOST, c->rhs.p, 1, sizeof(double)));
   const int64_t nh = num_histories(c);
   if (!c->prm.forward) MCLS_TRY(build_source(c, c->rhs.p, nh));
   else if (first + count > nh) return fail(MCLS_ERR_INVALID_ARG, "history range beyond the forward source");
@@ -1556,6 +1704,11 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(weights, dw.p, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (c->ordered) {  // states in the caller's numbering
+    const size_t m = (size_t)count * (size_t)(max_len + 1);
+    for (size_t i = 0; i < m; ++i)
+      if (paths[i] >= 0) paths[i] = c->ord_h[(size_t)paths[i]];
+  }
   return MCLS_OK;
 }
 
@@ -1570,11 +1723,10 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
   hipStream_t s = c->stream;
-  const size_t bytes = sizeof(double) * (size_t)c->n;
   c->stats.walk_launches = 0;
   c->stats.ms_walk = c->stats.ms_source = c->stats.ms_residual = 0.0;
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
-  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), s));
+  MCLS_TRY(vec_in(c, b, ptr_kind, c->b.p, 1, sizeof(double)));
   // buildMonteCarloSource (src/MCLS_MonteCarloSolverManager_impl.hpp:347-380):
   // rhs = omega * PL b
   MCLS_TRY(apply_left_prec(c, c->b.p, c->rhs.p));
@@ -1583,7 +1735,7 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, sidx));
   // right preconditioning: x = PR x (src/MCLS_MonteCarloSolverManager_impl.hpp:220-229)
   MCLS_TRY(recover_right_prec(c, c->x.p));
-  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
+  MCLS_TRY(vec_out(c, c->x.p, x, ptr_kind, 1, sizeof(double)));
   unsigned long long steps = 0;
   HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1609,15 +1761,14 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   MCLS_TRY(ensure_domain(c));
   hipStream_t s = c->stream;
   const int n = c->n;
-  const size_t bytes = sizeof(double) * (size_t)n;
   const Params& p = c->prm;
   mcls_gpu_stats& st = c->stats;
   st.walk_launches = 0;
   st.ms_walk = st.ms_source = st.ms_residual = 0.0;
   st.histories = st.steps = 0;
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, s));
-  HIP_TRY(hipMemcpyAsync(c->b.p, b, bytes, kind_to_dev(ptr_kind), s));
-  HIP_TRY(hipMemcpyAsync(c->x.p, x, bytes, kind_to_dev(ptr_kind), s));
+  MCLS_TRY(vec_in(c, b, ptr_kind, c->b.p, 1, sizeof(double)));
+  MCLS_TRY(vec_in(c, x, ptr_kind, c->x.p, 1, sizeof(double)));
 
   // src/MCLS_MCSASolverManager_impl.hpp:300-318: ||PL b||_2, homogeneous case
   if (c->pl_csr.on) {
@@ -1719,7 +1870,7 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   }
   // src/MCLS_MCSASolverManager_impl.hpp:410-420: x = PR x when right preconditioned
   MCLS_TRY(recover_right_prec(c, c->x.p));
-  HIP_TRY(hipMemcpyAsync(x, c->x.p, bytes, kind_from_dev(ptr_kind), s));
+  MCLS_TRY(vec_out(c, c->x.p, x, ptr_kind, 1, sizeof(double)));
   unsigned long long steps = 0;
   HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1755,7 +1906,8 @@ int mcls_gpu_set_importance(mcls_gpu_context* c, const float* g, int ptr_kind) {
   MCLS_TRY(check_matrix(c));
   if (!g) return fail(MCLS_ERR_INVALID_ARG, "null importance");
   MCLS_TRY(c->g_imp.ensure((size_t)c->n, "importance"));
-  HIP_TRY(hipMemcpy(c->g_imp.p, g, sizeof(float) * (size_t)c->n, kind_to_dev(ptr_kind)));
+  MCLS_TRY(vec_in(c, g, ptr_kind, c->g_imp.p, 1, sizeof(float)));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->has_importance = true;
   c->proposal_valid = false;
   invalidate_graph(c);
@@ -1782,7 +1934,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   MCLS_TRY(ensure_vectors(c));
   MCLS_TRY(ensure_domain(c));
   hipStream_t s = c->stream;
-  HIP_TRY(hipMemcpyAsync(c->b.p, b, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), s));
+  MCLS_TRY(vec_in(c, b, ptr_kind, c->b.p, 1, sizeof(double)));
   HIP_TRY(mcls::launch_fill_f64(c->x.p, c->n, 0.0, s));
   MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));  // pilot source PL b
   const Params saved = c->prm;
@@ -1890,7 +2042,8 @@ int mcls_gpu_get_importance(mcls_gpu_context* c, float* g, int ptr_kind) {
   if (!g) return fail(MCLS_ERR_INVALID_ARG, "null output");
   if (!c->g_imp.p) return fail(MCLS_ERR_STATE, "no importance has been set or computed");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(g, c->g_imp.p, sizeof(float) * (size_t)c->n, kind_from_dev(ptr_kind)));
+  MCLS_TRY(vec_out(c, c->g_imp.p, g, ptr_kind, 1, sizeof(float)));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;
 }
 
@@ -1914,7 +2067,8 @@ int mcls_gpu_set_importance_mask(mcls_gpu_context* c, const signed char* mask, i
     return MCLS_OK;
   }
   MCLS_TRY(c->imp_mask.ensure((size_t)c->n, "importance mask"));
-  HIP_TRY(hipMemcpy(c->imp_mask.p, mask, (size_t)c->n, kind_to_dev(ptr_kind)));
+  MCLS_TRY(vec_in(c, mask, ptr_kind, c->imp_mask.p, 1, 1));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->has_mask = true;
   return MCLS_OK;
 }
@@ -1925,9 +2079,9 @@ int mcls_gpu_importance_features(mcls_gpu_context* c, const double* v, float* X,
   MCLS_TRY(check_matrix(c));
   if (!v || !X) return fail(MCLS_ERR_INVALID_ARG, "null vector");
   MCLS_TRY(ensure_vectors(c));
-  HIP_TRY(hipMemcpyAsync(c->rhs.p, v, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), c->stream));
+  MCLS_TRY(vec_in(c, v, ptr_kind, c->rhs.p, 1, sizeof(double)));
   MCLS_TRY(compute_features(c, c->rhs.p));
-  HIP_TRY(hipMemcpyAsync(X, c->feat.p, sizeof(float) * 8 * (size_t)c->n, kind_from_dev(ptr_kind), c->stream));
+  MCLS_TRY(vec_out(c, c->feat.p, X, ptr_kind, 8, sizeof(float)));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;
 }
@@ -1943,8 +2097,8 @@ int mcls_gpu_importance_targets(mcls_gpu_context* c, const double* v, uint64_t s
   if (sampler == mcls::SAMPLER_LEARNED || sampler == mcls::SAMPLER_LEARNED_ALIAS) c->prm.sampler = mcls::SAMPLER_CDF;
   int rc = ensure_domain(c);
   if (rc == MCLS_OK) {
-    HIP_TRY(hipMemcpyAsync(c->rhs.p, v, sizeof(double) * (size_t)c->n, kind_to_dev(ptr_kind), c->stream));
-    rc = mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index);
+    rc = vec_in(c, v, ptr_kind, c->rhs.p, 1, sizeof(double));
+    if (rc == MCLS_OK) rc = mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index);
   }
   c->prm.sampler = sampler;
   MCLS_TRY(rc);
@@ -1956,7 +2110,7 @@ int mcls_gpu_importance_targets(mcls_gpu_context* c, const double* v, uint64_t s
   HIP_TRY(mcls::launch_feature_sums(c->n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->r.p, c->part.p,
                                     c->scal.p + 4, c->stream));
   HIP_TRY(mcls::launch_targets(c->n, c->r.p, c->scal.p + 4, c->mlp_out.p, c->stream));
-  HIP_TRY(hipMemcpyAsync(T, c->mlp_out.p, sizeof(float) * (size_t)c->n, kind_from_dev(ptr_kind), c->stream));
+  MCLS_TRY(vec_out(c, c->mlp_out.p, T, ptr_kind, 1, sizeof(float)));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return MCLS_OK;
 }
@@ -1970,6 +2124,7 @@ int mcls_gpu_set_systems(mcls_gpu_context* c, int64_t nsys, const int64_t* offse
     return MCLS_OK;
   }
   if (nsys < 0 || nsys > c->n || !offsets) return fail(MCLS_ERR_INVALID_ARG, "bad system count or offsets");
+  if (c->ordered) return fail(MCLS_ERR_UNSUPPORTED, "batched systems with a state order");
   if (offsets[0] != 0 || offsets[nsys] != c->n) return fail(MCLS_ERR_INVALID_ARG, "offsets must span [0, n]");
   std::vector<int> sys_of((size_t)c->n);
   for (int64_t s = 0; s < nsys; ++s) {

@@ -700,6 +700,48 @@ inline int grid_for(int64_t n, int block, int cap = 65536) {
   return (int)g;
 }
 
+// --------------------------------------------------------------- state order
+// A context may number the unknowns in an order of its own: internal state k
+// is the caller's unknown ord[k].  Vectors (rows of w elements) cross the
+// C ABI by a gather into the internal numbering or a scatter back out.
+template <typename T>
+__global__ void k_gather_rows(int64_t n, int w, const int* __restrict__ ord, const T* __restrict__ src,
+                              T* __restrict__ dst) {
+  const int64_t total = n * w;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / w;
+    dst[i] = src[(int64_t)ord[k] * w + (i - k * w)];
+  }
+}
+
+template <typename T>
+__global__ void k_scatter_rows(int64_t n, int w, const int* __restrict__ ord, const T* __restrict__ src,
+                               T* __restrict__ dst) {
+  const int64_t total = n * w;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = i / w;
+    dst[(int64_t)ord[k] * w + (i - k * w)] = src[i];
+  }
+}
+
+// P A P^T: internal row k is the caller's row ord[k] with its columns
+// renumbered through inv (inv[ord[k]] = k); entries keep their order.
+template <typename V>
+__global__ void k_permute_csr(int n, const int* __restrict__ ord, const int* __restrict__ inv,
+                              const int* __restrict__ rp_src, const int* __restrict__ col_src,
+                              const V* __restrict__ val_src, const int* __restrict__ rp_dst,
+                              int* __restrict__ col_dst, V* __restrict__ val_dst) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int r = ord[k];
+    const int b = rp_src[r], e = rp_src[r + 1];
+    const int o = rp_dst[k] - b;
+    for (int p = b; p < e; ++p) {
+      col_dst[o + p] = inv[col_src[p]];
+      val_dst[o + p] = val_src[p];
+    }
+  }
+}
+
 // --------------------------------------------------------------- domain
 template <typename AV>
 __device__ __forceinline__ double aval(const void* v, int k) {
@@ -1819,6 +1861,43 @@ hipError_t launch_scan_i32_excl(const int* in, int* out, int64_t n, int* tmp, hi
   return scan_incl<int>(in, out + 1, n, tmp, s);
 }
 
+hipError_t launch_gather_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,
+                              hipStream_t s) {
+  const int g = grid_for(n * w, 256);
+  if (elem == 8)
+    k_gather_rows<uint64_t><<<g, 256, 0, s>>>(n, w, ord, (const uint64_t*)src, (uint64_t*)dst);
+  else if (elem == 4)
+    k_gather_rows<uint32_t><<<g, 256, 0, s>>>(n, w, ord, (const uint32_t*)src, (uint32_t*)dst);
+  else if (elem == 1)
+    k_gather_rows<uint8_t><<<g, 256, 0, s>>>(n, w, ord, (const uint8_t*)src, (uint8_t*)dst);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_scatter_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,
+                               hipStream_t s) {
+  const int g = grid_for(n * w, 256);
+  if (elem == 8)
+    k_scatter_rows<uint64_t><<<g, 256, 0, s>>>(n, w, ord, (const uint64_t*)src, (uint64_t*)dst);
+  else if (elem == 4)
+    k_scatter_rows<uint32_t><<<g, 256, 0, s>>>(n, w, ord, (const uint32_t*)src, (uint32_t*)dst);
+  else if (elem == 1)
+    k_scatter_rows<uint8_t><<<g, 256, 0, s>>>(n, w, ord, (const uint8_t*)src, (uint8_t*)dst);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+hipError_t launch_permute_csr(int n, const int* ord, const int* inv, const int* rp_src, const int* col_src,
+                              const void* val_src, bool f32, const int* rp_dst, int* col_dst, void* val_dst,
+                              hipStream_t s) {
+  if (f32)
+    k_permute_csr<float><<<grid_for(n, 256), 256, 0, s>>>(n, ord, inv, rp_src, col_src, (const float*)val_src,
+                                                          rp_dst, col_dst, (float*)val_dst);
+  else
+    k_permute_csr<double><<<grid_for(n, 256), 256, 0, s>>>(n, ord, inv, rp_src, col_src, (const double*)val_src,
+                                                           rp_dst, col_dst, (double*)val_dst);
+  return hipGetLastError();
+}
 hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
   k_fill<int><<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
   return hipGetLastError();

@@ -182,6 +182,17 @@ hipError_t launch_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_
 hipError_t launch_scan_i32_excl(const int* in, int* out, int64_t n, int* tmp, hipStream_t s);
 
 // ---------------------------------------------------------------- domain
+// state order (internal state k = the caller's unknown ord[k]), rows of w
+// elements of elem = 1, 4 or 8 bytes: dst[k] = src[ord[k]] (gather) or
+// dst[ord[k]] = src[k] (scatter)
+hipError_t launch_gather_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,
+                              hipStream_t s);
+hipError_t launch_scatter_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,
+                               hipStream_t s);
+// P A P^T of a validated CSR matrix; rp_dst holds the permuted row pointers
+hipError_t launch_permute_csr(int n, const int* ord, const int* inv, const int* rp_src, const int* col_src,
+                              const void* val_src, bool f32, const int* rp_dst, int* col_dst, void* val_dst,
+                              hipStream_t s);
 hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
