@@ -50,10 +50,10 @@ def log(msg):
 
 WORKLOADS = {
     "cfg4": dict(kind=capi.STENCIL3D_27PT, nx=256, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
-                 histories=1 << 28, label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
+                 histories=1 << 28, brick=8, label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
                  "FP32 CSR), point Jacobi, 2^28 histories/iteration, tol 1e-8"),
     "cfg4-64": dict(kind=capi.STENCIL3D_27PT, nx=64, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
-                    histories=16 * 64 ** 3, label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "
+                    histories=16 * 64 ** 3, brick=8, label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "
                     "16 histories/unknown, tol 1e-8"),
 }
 
@@ -191,11 +191,12 @@ def learned_stage(ctx, prob, w, pilot_histories=1 << 24, reps=4, train_rows=1 <<
             "seconds": time.perf_counter() - t0}, net
 
 
-def measured_traffic(workload, form, steps_per_launch):
+def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_order=None):
     """HBM bytes per launch of a form's walk kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload
     (calibrated against a copy kernel), when its transitions per launch match
-    this run's to 2%; None otherwise."""
+    this run's to 2% and it was taken with the same LDS tally tile and state
+    order; None otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"walk_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -205,6 +206,8 @@ def measured_traffic(workload, form, steps_per_launch):
         return None
     rec = rec["forms"][form]
     if abs(rec["steps_per_launch"] - steps_per_launch) > 0.02 * steps_per_launch:
+        return None
+    if rec.get("tally_tile") != tally_tile or rec.get("state_order") != state_order:
         return None
     return float(rec["traffic_bytes_per_launch"])
 
@@ -248,6 +251,11 @@ def main():
     ap.add_argument("--no-forward", action="store_true", help="skip the forward-method solve")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-proposal selection of the chosen form (plain sampling)")
+    ap.add_argument("--order", default="bricks", choices=["bricks", "natural"],
+                    help="state order of the context: the grid's unknowns in cubic bricks (the workload's "
+                    "brick edge) or the generator's x-fastest numbering")
+    ap.add_argument("--plain-order", default="natural", choices=["bricks", "natural"],
+                    help="state order of the plain form's context")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -272,6 +280,15 @@ def main():
     prob = make_problem(w)
     log(f"n={prob.n} nnz={prob.nnz}")
     ctx = capi.Context(0)
+    # The context numbers the unknowns brick by brick (P A P^T; b and x stay
+    # in the generator's numbering at the API): the walks born close together
+    # then share rows in L2.  Same operator, same histories, same tolerance.
+    order_label = "natural"
+    if args.order == "bricks" and w.get("brick"):
+        e = w["brick"]
+        ctx.set_state_order(capi.brick_order(w["nx"], w["nx"], w["nx"], e, e, e))
+        order_label = f"{e}^3 bricks"
+        log(f"state order: {order_label}")
     ctx.set_problem(prob, "f32" if w["f32"] else "f64")
     ctx.set_point_jacobi()
     base = {"Convergence Tolerance": 1e-8, "Maximum Iterations": 60, "Number of Histories": w["histories"],
@@ -295,6 +312,7 @@ def main():
     if not args.no_tune:
         tuning = ctx.autotune(prob.b)
         log(f"autotune: {tuning}")
+    ctx_tile = tuning["tally_tile"] if tuning else 0  # "Tally Tile" defaults to 0
 
     dev = DeviceVectors(prob.b)
     for i in range(args.warmup):
@@ -337,7 +355,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {w['label']}", "form": args.form, "n": prob.n,
                    "nnz": prob.nnz, "histories_per_iteration": w["histories"],
-                   "parallelism": "1 GPU"},
+                   "state_order": order_label, "parallelism": "1 GPU"},
         "time_to_tol_s": total / args.steps,
         "time_to_tol_host_io_s": host_io_s,
         "mcsa_iterations": iters,
@@ -346,22 +364,60 @@ def main():
         "walk_steps_per_s": steps / (walk_ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, args.form, steps_per_launch),
+                     "traffic": measured_traffic(args.workload, args.form, steps_per_launch,
+                                                 ctx_tile, order_label),
                      "kernel": f"k_walk<{'float' if prec == 'FP32' else 'double'}>",
                      "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},
         "tuning": tuning,
         "learned_proposal": learned_rec,
     }
+    if not args.no_forward and args.form == "chosen":
+        # the forward method with the chosen form's walk (reported beside
+        # the headline, which stays the adjoint solve)
+        ctx.set_params(FORMS["forward"])
+        ftune = None
+        if not args.no_tune:
+            ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
+            ftune = ctx.autotune(prob.b)
+        solve_once(ctx, dev)  # forward domain build and warm-up
+        fdt, fst = solve_once(ctx, dev)
+        log(f"forward FP32: {fdt:.3f}s iters={fst.iterations} tol={fst.achieved_tol:.2e}")
+        fbytes = bytes_per_step("FP32", z, "Alias", True)
+        fsteps_per_launch = fst.steps / max(fst.walk_launches, 1)
+        out["forward"] = {"time_to_tol_s": fdt, "mcsa_iterations": fst.iterations,
+                          "traffic": measured_traffic(args.workload, "forward", fsteps_per_launch,
+                                                      0, order_label),
+                          "achieved_tol": fst.achieved_tol, "histories_per_s": fst.histories / fdt,
+                          "walk_steps_per_s": fst.steps / (fst.ms_walk / 1e3),
+                          "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "bytes_per_step": fbytes, "steps_per_launch": fsteps_per_launch,
+                          "avg_launch_ms": fst.ms_walk / max(fst.walk_launches, 1),
+                          "state_order": order_label, "tuning": ftune}
+    dev.b.close()
+    dev.x.close()
+    ctx.close()
     if not args.no_plain and args.form == "chosen":
-        ctx.set_params(FORMS["plain"])
+        # The plain form runs in a context of its own, in the order it is
+        # fastest in: the generator's numbering (its random births gain
+        # nothing from bricks; its walk measured slower there).
+        pctx = capi.Context(0)
+        plain_order = "natural"
+        if args.plain_order == "bricks" and w.get("brick"):
+            e = w["brick"]
+            pctx.set_state_order(capi.brick_order(w["nx"], w["nx"], w["nx"], e, e, e))
+            plain_order = f"{e}^3 bricks"
+        pctx.set_problem(prob, "f32" if w["f32"] else "f64")
+        pctx.set_point_jacobi()
+        pctx.set_params(base)
+        pctx.set_params(FORMS["plain"])
+        pdev = DeviceVectors(prob.b)
         ptune = None
         if not args.no_tune:
-            ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64})
-            ptune = ctx.autotune(prob.b)
+            ptune = pctx.autotune(prob.b)
             log(f"autotune (plain): {ptune}")
-        dt, st = solve_once(ctx, dev)
-        log(f"plain FP64: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
+        dt, st = solve_once(pctx, pdev)
+        log(f"plain FP64 ({plain_order}): {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
         pbytes = bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
         psteps_per_launch = st.steps / max(st.walk_launches, 1)
         out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
@@ -370,33 +426,15 @@ def main():
                         "roofline_frac": pbytes * st.steps / (st.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
                         "bytes_per_step": pbytes, "steps_per_launch": psteps_per_launch,
                         "avg_launch_ms": st.ms_walk / max(st.walk_launches, 1),
-                        "traffic": measured_traffic(args.workload, "plain", psteps_per_launch),
-                        "tuning": ptune}
+                        "traffic": measured_traffic(args.workload, "plain", psteps_per_launch,
+                                                    (ptune or {}).get("tally_tile", 0), plain_order),
+                        "state_order": plain_order, "tuning": ptune}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
-        if not args.no_forward:
-            # the forward method with the chosen form's walk (reported beside
-            # the headline, which stays the adjoint solve)
-            ctx.set_params(FORMS["forward"])
-            ftune = None
-            if not args.no_tune:
-                ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
-                ftune = ctx.autotune(prob.b)
-            solve_once(ctx, dev)  # forward domain build and warm-up
-            fdt, fst = solve_once(ctx, dev)
-            log(f"forward FP32: {fdt:.3f}s iters={fst.iterations} tol={fst.achieved_tol:.2e}")
-            fbytes = bytes_per_step("FP32", z, "Alias", True)
-            fsteps_per_launch = fst.steps / max(fst.walk_launches, 1)
-            out["forward"] = {"time_to_tol_s": fdt, "mcsa_iterations": fst.iterations,
-                              "traffic": measured_traffic(args.workload, "forward", fsteps_per_launch),
-                              "achieved_tol": fst.achieved_tol, "histories_per_s": fst.histories / fdt,
-                              "walk_steps_per_s": fst.steps / (fst.ms_walk / 1e3),
-                              "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
-                              "bytes_per_step": fbytes, "steps_per_launch": fsteps_per_launch,
-                              "avg_launch_ms": fst.ms_walk / max(fst.walk_launches, 1),
-                              "speedup_vs_plain": dt / fdt, "tuning": ftune}
-    dev.b.close()
-    dev.x.close()
-    ctx.close()
+        if "forward" in out:
+            out["forward"]["speedup_vs_plain"] = dt / out["forward"]["time_to_tol_s"]
+        pdev.b.close()
+        pdev.x.close()
+        pctx.close()
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(prob, w, out.get("plain", {}).get("mcsa_iterations"))

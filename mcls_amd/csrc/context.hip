@@ -875,8 +875,11 @@ int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const doubl
 // accumulate: x += ..., else x = ...  `timed` brackets the source build and
 // the walk with events (not inside a graph capture); `key_dev` (nullable)
 // makes the walk read its stream key from device memory.
+// window < 1: walk only that fraction of the source's histories, a run of
+// consecutive history ids from the middle of the queue (the autotuner's
+// pilots: same births per state as the full solve, fewer states).
 int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx,
-                    bool timed = true, const uint64_t* key_dev = nullptr) {
+                    bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0) {
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
@@ -890,9 +893,15 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
   const bool strat = !forward && c->prm.source == mcls::SRC_STRATIFIED;
   const int64_t last = strat ? nh + c->n : nh;
-  WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), 0, last);
+  int64_t first = 0, end = last;
+  if (window < 1.0 && last > 0) {
+    const int64_t span = std::max<int64_t>(1, (int64_t)((double)last * window));
+    first = (last - span) / 2;
+    end = first + span;
+  }
+  WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), first, end);
   a.key_dev = key_dev;
-  WalkConfig wc = walk_config(c, false, last);
+  WalkConfig wc = walk_config(c, false, end - first);
   if (wc.tile) {
     const size_t elem = sizeof(int) + sizeof(double);  // key + FP64 partial sum
     if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
@@ -1942,7 +1951,9 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   const int64_t nh = num_histories(c);
   int64_t np = pilot > 0 ? pilot : std::max<int64_t>(nh / 8, (int64_t)1 << 20);
   if (nh > 0 && np > nh) np = nh;
-  c->prm.num_histories = np;
+  // a pilot walks np consecutive histories of the full source, so that the
+  // births per state (what the LDS tile and the caches see) are the solve's
+  const double window = nh > 0 ? (double)np / (double)nh : 1.0;
   c->nsys = 0;  // pilot births follow |PL b| over the whole operator
   auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double* ms) -> int {
     c->prm.walk_block = blk;
@@ -1950,7 +1961,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     c->prm.walk_batch = bat;
     c->prm.tally_tile = tile;
     MCLS_TRY(ensure_vectors(c));
-    MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true));
+    MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true, nullptr, window));
     HIP_TRY(hipStreamSynchronize(s));
     float t = 0.f;
     HIP_TRY(hipEventElapsedTime(&t, c->ev[3], c->ev[4]));
@@ -1996,12 +2007,14 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
       best.batch = batches[k];
     }
   }
-  // LDS tally tile (adjoint collision estimator): off, 128 or 512 entries per wave
+  // LDS tally tile (adjoint collision estimator): off, 128, 512 or 1024
+  // entries per wave, within 64 KiB of LDS per workgroup
   if (saved.estimator == mcls::EST_COLLISION && !saved.forward) {
-    const int64_t tiles[3] = {0, 128, 512};
+    const int64_t tiles[4] = {0, 128, 512, 1024};
     const int64_t shape_tile = best.tally_tile;
-    for (int k = 0; k < 3 && rc == MCLS_OK; ++k) {
+    for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
       if (tiles[k] == shape_tile) continue;
+      if ((best.block / 64) * tiles[k] * (int64_t)(sizeof(int) + sizeof(double)) > 65536) continue;
       rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, tiles[k], &ms);
       ++best.candidates;
       if (rc == MCLS_OK && ms < best.ms_best) {
