@@ -254,8 +254,6 @@ def main():
     ap.add_argument("--order", default="bricks", choices=["bricks", "natural"],
                     help="state order of the context: the grid's unknowns in cubic bricks (the workload's "
                     "brick edge) or the generator's x-fastest numbering")
-    ap.add_argument("--plain-order", default="natural", choices=["bricks", "natural"],
-                    help="state order of the plain form's context")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -393,31 +391,17 @@ def main():
                           "roofline_frac": fbytes * fst.steps / (fst.ms_walk / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "bytes_per_step": fbytes, "steps_per_launch": fsteps_per_launch,
                           "avg_launch_ms": fst.ms_walk / max(fst.walk_launches, 1),
-                          "state_order": order_label, "tuning": ftune}
-    dev.b.close()
-    dev.x.close()
-    ctx.close()
+                          "tuning": ftune}
     if not args.no_plain and args.form == "chosen":
-        # The plain form runs in a context of its own, in the order it is
-        # fastest in: the generator's numbering (its random births gain
-        # nothing from bricks; its walk measured slower there).
-        pctx = capi.Context(0)
-        plain_order = "natural"
-        if args.plain_order == "bricks" and w.get("brick"):
-            e = w["brick"]
-            pctx.set_state_order(capi.brick_order(w["nx"], w["nx"], w["nx"], e, e, e))
-            plain_order = f"{e}^3 bricks"
-        pctx.set_problem(prob, "f32" if w["f32"] else "f64")
-        pctx.set_point_jacobi()
-        pctx.set_params(base)
-        pctx.set_params(FORMS["plain"])
-        pdev = DeviceVectors(prob.b)
+        # the plain form in the same context (same numbering of the unknowns)
+        ctx.set_params(FORMS["plain"])
         ptune = None
         if not args.no_tune:
-            ptune = pctx.autotune(prob.b)
+            ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
+            ptune = ctx.autotune(prob.b)
             log(f"autotune (plain): {ptune}")
-        dt, st = solve_once(pctx, pdev)
-        log(f"plain FP64 ({plain_order}): {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
+        dt, st = solve_once(ctx, dev)
+        log(f"plain FP64: {dt:.3f}s iters={st.iterations} tol={st.achieved_tol:.2e}")
         pbytes = bytes_per_step("FP64", z, FORMS["plain"]["Transition Sampler"])
         psteps_per_launch = st.steps / max(st.walk_launches, 1)
         out["plain"] = {"time_to_tol_s": dt, "mcsa_iterations": st.iterations,
@@ -427,14 +411,14 @@ def main():
                         "bytes_per_step": pbytes, "steps_per_launch": psteps_per_launch,
                         "avg_launch_ms": st.ms_walk / max(st.walk_launches, 1),
                         "traffic": measured_traffic(args.workload, "plain", psteps_per_launch,
-                                                    (ptune or {}).get("tally_tile", 0), plain_order),
-                        "state_order": plain_order, "tuning": ptune}
+                                                    (ptune or {}).get("tally_tile", 0), order_label),
+                        "tuning": ptune}
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
         if "forward" in out:
             out["forward"]["speedup_vs_plain"] = dt / out["forward"]["time_to_tol_s"]
-        pdev.b.close()
-        pdev.x.close()
-        pctx.close()
+    dev.b.close()
+    dev.x.close()
+    ctx.close()
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(prob, w, out.get("plain", {}).get("mcsa_iterations"))
