@@ -157,6 +157,14 @@ class DeviceProblem {
   void sync(const Problem& p, const ParameterList& pl) {
     const Matrix& A = *p.getOperator();
     MCLS_INSIST(A.isFillComplete(), "operator must be fill-complete");
+    if (d_order_dirty) {
+      // a new numbering drops the device operator and what was set for it
+      MCLS_INSIST(d_order.empty() || (int64_t)d_order.size() == A.numRows(), "state order has the wrong length");
+      GpuContext::check(mcls_gpu_set_state_order(d_ctx->get(), (int64_t)d_order.size(),
+                                                 d_order.empty() ? nullptr : d_order.data(), MCLS_HOST));
+      d_op = nullptr;
+      d_order_dirty = false;
+    }
     if (d_op != p.getOperator().get()) {
       const void* val = A.values().data();
       int dt = std::is_same<typename Matrix::scalar_type, float>::value ? MCLS_F32 : MCLS_F64;
@@ -221,6 +229,13 @@ class DeviceProblem {
     d_imp_dirty = true;
   }
   void setImportanceModel(mcls_mlp* m) { GpuContext::check(mcls_gpu_set_importance_model(d_ctx->get(), m)); }
+  // The device's numbering of the unknowns (mcls_gpu_set_state_order),
+  // applied at the next solve; vectors keep the caller's numbering.  An
+  // empty order returns to the caller's numbering.
+  void setStateOrder(std::vector<int32_t> order) {
+    d_order = std::move(order);
+    d_order_dirty = true;
+  }
   mcls_gpu_context* ctx() const { return d_ctx->get(); }
   mcls_gpu_stats stats() const {
     mcls_gpu_stats s;
@@ -240,6 +255,8 @@ class DeviceProblem {
   ImportanceMode d_imp_mode = ImportanceMode::None;
   std::vector<float> d_imp;
   bool d_imp_dirty = false;
+  std::vector<int32_t> d_order;
+  bool d_order_dirty = false;
 };
 
 inline RCP<ParameterList> mcValidParameters() {
@@ -310,6 +327,7 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
   void setImportanceDiagonal() { d_dev.setImportanceDiagonal(); }
   void setImportance(std::vector<float> g) { d_dev.setImportance(std::move(g)); }
   void setImportanceModel(mcls_mlp* m) { d_dev.setImportanceModel(m); }
+  void setStateOrder(std::vector<int32_t> order) { d_dev.setStateOrder(std::move(order)); }
 
  private:
   RCP<Problem> d_problem;
@@ -380,6 +398,7 @@ class MCSASolverManager : public SolverManager<Vector, Matrix> {
   void setImportanceDiagonal() { d_dev.setImportanceDiagonal(); }
   void setImportance(std::vector<float> g) { d_dev.setImportance(std::move(g)); }
   void setImportanceModel(mcls_mlp* m) { d_dev.setImportanceModel(m); }
+  void setStateOrder(std::vector<int32_t> order) { d_dev.setStateOrder(std::move(order)); }
   // Walk launch shape by timing pilot MC solves of PL b (mcls_gpu_autotune);
   // kept by later solves unless the parameter list sets the walk keys.
   mcls_gpu_tuning autotune(int64_t pilot_histories = 0) {

@@ -7,6 +7,7 @@
 //   tpetra/test/tstTpetraMonteCarloSolverManager.cpp:73-197  (adjoint MC)
 //   src/MCLS_SolverFactory_impl.hpp:58-146                   (names)
 // Prints one line per check and exits non-zero on the first failure.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -404,6 +405,54 @@ static void test_sparse_prec() {
   std::printf("ok MCSASolverManager with sparse PL and PR iters=%d\n", solver.getNumIters());
 }
 
+// The device numbers the unknowns its own way (odd ones first, reversed);
+// the solution comes back in the caller's numbering and matches the
+// unordered solve, also with a left preconditioner and after the order is
+// dropped again.
+static void test_state_order() {
+  const int n = 40;
+  auto A = build_operator(n, 2.0, -0.5);
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  for (int i = 0; i < n; ++i) (*b)[i] = 1.0 + 0.05 * i;
+  std::vector<int32_t> order;
+  for (int i = n - 1; i >= 0; --i)
+    if (i % 2) order.push_back(i);
+  for (int i = n - 1; i >= 0; --i)
+    if (i % 2 == 0) order.push_back(i);
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-10);
+  plist->set<int>("Maximum Iterations", 40);
+  plist->set<double>("Sample Ratio", 100.0);
+  plist->set<int>("Random Seed", 5);
+  auto solve = [&](bool ordered, bool unorder_after) {
+    auto x = MT::cloneVectorFromMatrixRows(*A);
+    VT::putScalar(*x, 0.0);
+    auto problem = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, x, b);
+    PointJacobiPreconditioner<double> pj;
+    pj.setOperator(A);
+    pj.buildPreconditioner();
+    problem->setLeftPrec(pj.getLeftPreconditioner());
+    MCSASolverManager<VectorType, MatrixType, AdjointTag> solver(problem, plist);
+    if (ordered) solver.setStateOrder(order);
+    CHECK(solver.solve() && solver.achievedTol() <= 1e-10);
+    if (unorder_after) {
+      solver.setStateOrder({});
+      VT::putScalar(*x, 0.0);
+      CHECK(solver.solve() && solver.achievedTol() <= 1e-10);
+    }
+    return x;
+  };
+  auto x0 = solve(false, false), x1 = solve(true, false), x2 = solve(true, true);
+  double d1 = 0.0, d2 = 0.0, nx = 0.0;
+  for (int i = 0; i < n; ++i) {
+    d1 = std::max(d1, std::fabs((*x1)[i] - (*x0)[i]));
+    d2 = std::max(d2, std::fabs((*x2)[i] - (*x0)[i]));
+    nx = std::max(nx, std::fabs((*x0)[i]));
+  }
+  CHECK(d1 <= 1e-8 * nx && d2 <= 1e-8 * nx);
+  std::printf("ok state order: max diff %.2e / %.2e of %.2e\n", d1, d2, nx);
+}
+
 int main() {
   test_mcsa_adjoint(false);
   test_mcsa_adjoint(true);
@@ -415,6 +464,7 @@ int main() {
   test_identity_prec<AdjointTag>("MCSASolverManager.prec_adjoint");
   test_identity_prec<ForwardTag>("MCSASolverManager.prec_forward");
   test_sparse_prec();
+  test_state_order();
   std::printf("ALL %d checks passed\n", g_checks);
   return 0;
 }
