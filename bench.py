@@ -62,10 +62,13 @@ FORMS = {
     # iteration (8 iterations of 8.5 steps per history; 1e-2 takes 6 of 16.5)
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
-               "Transition Sampler": "Alias", "Monte Carlo Type": "Adjoint"},
+               "Transition Sampler": "Alias", "Monte Carlo Type": "Adjoint",
+               # LDS tally tile partial sums in FP32 (flushed into the FP64
+               # tally): 8 bytes per entry, more resident waves than FP64's 12
+               "Tally Tile Precision": "Walk"},
     "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random",
               "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000,
-              "Transition Sampler": "CDF", "Monte Carlo Type": "Adjoint"},
+              "Transition Sampler": "CDF", "Monte Carlo Type": "Adjoint", "Tally Tile Precision": "FP64"},
     # the forward method with the chosen form's walk (no per-step tally atomics)
     "forward": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                 "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
@@ -195,8 +198,8 @@ def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_or
     """HBM bytes per launch of a form's walk kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload
     (calibrated against a copy kernel), when its transitions per launch match
-    this run's to 2% and it was taken with the same LDS tally tile and state
-    order; None otherwise."""
+    this run's to 2% and it was taken with the same LDS tally tile (size and
+    partial-sum precision) and state order; None otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"walk_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -208,6 +211,8 @@ def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_or
     if abs(rec["steps_per_launch"] - steps_per_launch) > 0.02 * steps_per_launch:
         return None
     if rec.get("tally_tile") != tally_tile or rec.get("state_order") != state_order:
+        return None
+    if tally_tile and rec.get("tally_tile_precision") != FORMS[form].get("Tally Tile Precision", "FP64"):
         return None
     return float(rec["traffic_bytes_per_launch"])
 

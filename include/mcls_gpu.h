@@ -166,6 +166,10 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        else a power of two in [64, 2048]; collision estimator): tallies
  *        merge in LDS and are flushed into an FP64 tally, also for FP32 walks
  *        (these four are what mcls_gpu_autotune sets)
+ *   str "Tally Tile Precision" ("FP64" | "Walk"): the tile's partial sums
+ *        in FP64 (12 bytes of LDS per entry) or in the walk's precision
+ *        (FP32 walks: 8 bytes, more resident waves per CU); either way
+ *        they are flushed into the FP64 tally
  * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
 int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
 int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);

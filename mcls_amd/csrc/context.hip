@@ -93,6 +93,7 @@ struct Params {
   int64_t walk_blocks_per_cu = 0;
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
+  int tile_prec = 64;      // tile partial sums: 64, or 32 = the walk's precision
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
   int sampler = mcls::SAMPLER_CDF;
@@ -799,7 +800,7 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   wc.source = c->prm.source;
   wc.sampler = c->prm.sampler;
   wc.record = record;
-  wc.tile = tile_on(c, record);
+  wc.tile = !tile_on(c, record) ? mcls::kTileOff : c->prm.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64;
   wc.block = (int)c->prm.walk_block;
   int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
   int64_t cap = (int64_t)c->num_cus * per_cu;
@@ -902,8 +903,8 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), first, end);
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, end - first);
-  if (wc.tile) {
-    const size_t elem = sizeof(int) + sizeof(double);  // key + FP64 partial sum
+  if (wc.tile != mcls::kTileOff) {
+    const size_t elem = mcls::tile_entry_bytes(wc.tile, c->prm.precision == 32 ? 4 : 8);  // key + partial sum
     if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile x waves per block exceeds 64 KiB of LDS per workgroup");
   }
@@ -1596,6 +1597,10 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     else if (key_is(v, "FP32")) p.precision = 32;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Walk Precision '") + v + "'");
     if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
+  } else if (key_is(key, "Tally Tile Precision")) {
+    if (key_is(v, "FP64")) p.tile_prec = 64;
+    else if (key_is(v, "Walk")) p.tile_prec = 32;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Tally Tile Precision '") + v + "'");
   } else if (key_is(key, "Fixed Point Type")) {
     if (!key_is(v, "Richardson"))
       return fail(MCLS_ERR_UNSUPPORTED, std::string("Fixed Point Type '") + v + "' (only Richardson)");
@@ -2014,7 +2019,9 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     const int64_t shape_tile = best.tally_tile;
     for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
       if (tiles[k] == shape_tile) continue;
-      if ((best.block / 64) * tiles[k] * (int64_t)(sizeof(int) + sizeof(double)) > 65536) continue;
+      const size_t elem = mcls::tile_entry_bytes(saved.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64,
+                                                 saved.precision == 32 ? 4 : 8);
+      if ((size_t)(best.block / 64) * (size_t)tiles[k] * elem > 65536) continue;
       rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, tiles[k], &ms);
       ++best.candidates;
       if (rc == MCLS_OK && ms < best.ms_best) {

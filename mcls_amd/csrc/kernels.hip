@@ -226,13 +226,16 @@ __device__ __forceinline__ uint32_t cdf_step(const DomainView& d, uint32_t& beg,
 // states in LDS (a.tile_slots entries of {state, partial sum}); a state that
 // finds its slot taken by another goes straight to HBM.  A wave flushes its
 // tile into the FP64 HBM tally at every work-queue grab and when it exits.
-template <typename Real, int EST, int SRC, int SAMP, bool REC, bool TILE>
+// TILE = kTileF64: FP64 partial sums; kTileWalk: partial sums in the walk's
+// precision (FP32 walks: 8 bytes of LDS per entry instead of 12).
+template <typename Real, int EST, int SRC, int SAMP, bool REC, int TILE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
-  using Acc = typename std::conditional<TILE, double, Real>::type;
+  using Acc = typename std::conditional<TILE != kTileOff, double, Real>::type;
+  using TAcc = typename std::conditional<TILE == kTileWalk, Real, double>::type;
   const DomainView& d = a.d;
   const int n = d.n;
   const Real* __restrict__ ht = Tables<Real>::h(d);
-  Acc* __restrict__ tally = TILE ? (Acc*)a.tally64 : (Acc*)Tables<Real>::tally(a);
+  Acc* __restrict__ tally = TILE != kTileOff ? (Acc*)a.tally64 : (Acc*)Tables<Real>::tally(a);
   const uint32_t* __restrict__ colp = d.col;
 
   int64_t hi_id = a.last;
@@ -258,12 +261,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   const int lane = __lane_id();
 
   int* tkeys = nullptr;  // TILE: this wave's slots
-  double* tvals = nullptr;  // partial sums in FP64, whatever the walk precision
-  if constexpr (TILE) {
+  TAcc* tvals = nullptr;  // partial sums
+  if constexpr (TILE != kTileOff) {
     extern __shared__ __align__(16) unsigned char walk_lds[];
     const int wv = threadIdx.x >> 6;
     tkeys = (int*)walk_lds + wv * a.tile_slots;
-    tvals = (double*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
+    tvals = (TAcc*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
     for (int k = lane; k < a.tile_slots; k += 64) {
       tkeys[k] = -1;
       tvals[k] = 0;
@@ -273,13 +276,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   }
   // wave-uniform: every lane of the wave runs it together
   auto flush_tile = [&]() {
-    if constexpr (TILE) {
+    if constexpr (TILE != kTileOff) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       for (int k = lane; k < a.tile_slots; k += 64) {
         const int key = tkeys[k];
         if (key >= 0) {
-          tally_add(&tally[key], tvals[k]);
+          tally_add(&tally[key], (Acc)tvals[k]);
           tkeys[k] = -1;
           tvals[k] = 0;
         }
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         fsum = fsum + (double)w * a.rhs[state];
       } else if (EST == EST_EXPECTED_VALUE) {
         for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
-      } else if constexpr (TILE) {
+      } else if constexpr (TILE != kTileOff) {
         const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
         int k = tkeys[slot];
         if (k < 0) {
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           if (k < 0) k = state;
         }
         if (k == state)
-          __hip_atomic_fetch_add(&tvals[slot], (double)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(&tvals[slot], (TAcc)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
           tally_add(&tally[state], (Acc)w);
       } else {
@@ -484,16 +487,18 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
 template <typename Real, int EST, int SRC, int SAMP>
 hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
   if (c.record) {
-    k_walk<Real, EST, SRC, SAMP, true, false><<<c.grid, c.block, 0, s>>>(a);
+    k_walk<Real, EST, SRC, SAMP, true, kTileOff><<<c.grid, c.block, 0, s>>>(a);
   } else if constexpr (EST == EST_COLLISION) {
-    if (c.tile) {
-      const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * (sizeof(int) + sizeof(double));
-      k_walk<Real, EST, SRC, SAMP, false, true><<<c.grid, c.block, lds, s>>>(a);
+    const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * tile_entry_bytes(c.tile, sizeof(Real));
+    if (c.tile == kTileWalk && sizeof(Real) == 4) {
+      k_walk<Real, EST, SRC, SAMP, false, kTileWalk><<<c.grid, c.block, lds, s>>>(a);
+    } else if (c.tile != kTileOff) {
+      k_walk<Real, EST, SRC, SAMP, false, kTileF64><<<c.grid, c.block, lds, s>>>(a);
     } else {
-      k_walk<Real, EST, SRC, SAMP, false, false><<<c.grid, c.block, 0, s>>>(a);
+      k_walk<Real, EST, SRC, SAMP, false, kTileOff><<<c.grid, c.block, 0, s>>>(a);
     }
   } else {
-    k_walk<Real, EST, SRC, SAMP, false, false><<<c.grid, c.block, 0, s>>>(a);
+    k_walk<Real, EST, SRC, SAMP, false, kTileOff><<<c.grid, c.block, 0, s>>>(a);
   }
   return hipGetLastError();
 }

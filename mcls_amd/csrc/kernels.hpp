@@ -104,13 +104,21 @@ struct WalkArgs {
   int tile_shift;          // 32 - log2(tile_slots)
 };
 
+// LDS tally tile: off, FP64 partial sums, or partial sums in the walk's
+// precision (the same as FP64 for FP64 walks)
+enum TileKind { kTileOff = 0, kTileF64 = 1, kTileWalk = 2 };
+// LDS bytes per tile entry: an int key and the partial sum
+inline size_t tile_entry_bytes(int tile, size_t real_bytes) {
+  return sizeof(int) + (tile == kTileWalk ? real_bytes : sizeof(double));
+}
+
 struct WalkConfig {
   int precision;   // 64 or 32
   int estimator;   // Estimator
   int source;      // SourceMode
   int sampler;     // Sampler
   bool record;
-  bool tile;       // LDS tally tile (collision estimator; tallies go to tally64)
+  int tile;        // TileKind (collision estimator; tallies go to tally64)
   int block;       // threads per block
   int grid;        // blocks (rounded up to a multiple of kQueueSlots)
 };

@@ -53,32 +53,40 @@ def test_walk_batch_validation(ctx):
             ctx.set_params({"Walk Batch": bad})
 
 
+@pytest.mark.parametrize("tile_prec", ["FP64", "Walk"])
 @pytest.mark.parametrize("sampler,prec", [("Alias", 32), ("CDF", 64), ("CDF", 32)])
-def test_tally_tile_matches_direct_tally(ctx, sampler, prec):
+def test_tally_tile_matches_direct_tally(ctx, sampler, prec, tile_prec):
     """The LDS tile changes only where the additions happen: same paths, same
-    tallies to the rounding of the summation order (the tile's partial sums
-    are FP64 for every walk, so FP32 walks accumulate in FP64 through it)."""
+    tallies to the rounding of the summation order.  Its partial sums are
+    FP64, or in the walk's precision ("Walk"), and are flushed into an FP64
+    tally either way."""
     prob, rhs = _poisson(ctx, sampler, prec)
-    ctx.set_params({"Tally Tile": 0})
+    ctx.set_params({"Tally Tile": 0, "Tally Tile Precision": tile_prec})
     ref = ctx.mc_solve_raw(rhs, 3)
     for tile, blk in ((64, 256), (512, 128), (1024, 64)):
         ctx.set_params({"Tally Tile": tile, "Walk Block Size": blk, "Walk Blocks Per CU": 0})
         x = ctx.mc_solve_raw(rhs, 3)
         tol = 1e-12 if prec == 64 else 1e-4
         assert np.linalg.norm(x - ref) <= tol * np.linalg.norm(ref), tile
-    # FP32 walks with the tile flush into an FP64 tally: the result equals the
-    # FP64 accumulation of the same FP32 paths to ~FP32 weight rounding
     ctx.set_params({"Tally Tile": 0, "Walk Block Size": 256})
     with pytest.raises(capi.GpuError):
         ctx.set_params({"Tally Tile": 100})
-    # every walk keeps FP64 partial sums in the tile: 4 waves x 2048 x 12 bytes > 64 KiB
-    ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
     with pytest.raises(capi.GpuError):
-        ctx.mc_solve_raw(rhs, 4)
-    ctx.set_params({"Tally Tile": 0})
+        ctx.set_params({"Tally Tile Precision": "FP16"})
+    # 4 waves x 2048 entries x 12 bytes (FP64 partial sums) > 64 KiB; with
+    # FP32 partial sums 8 bytes an entry fit exactly
+    ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
+    if tile_prec == "FP64" or prec == 64:
+        with pytest.raises(capi.GpuError):
+            ctx.mc_solve_raw(rhs, 4)
+    else:
+        x = ctx.mc_solve_raw(rhs, 3)
+        assert np.linalg.norm(x - ref) <= 1e-4 * np.linalg.norm(ref)
+    ctx.set_params({"Tally Tile": 0, "Tally Tile Precision": "FP64"})
 
 
-def test_mcsa_with_tally_tile_converges(ctx):
+@pytest.mark.parametrize("tile_prec", ["FP64", "Walk"])
+def test_mcsa_with_tally_tile_converges(ctx, tile_prec):
     prob = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 16, seed=3, p0=0.6, p1=0.9, p2=0.8, p3=1.2),
                               want64=False, want32=True)
     ctx.set_problem(prob, "f32")
@@ -86,8 +94,9 @@ def test_mcsa_with_tally_tile_converges(ctx):
     ctx.set_params({"Walk Precision": "FP32", "Transition Sampler": "Alias", "Source Sampling Type": "Stratified",
                     "History Length": 1000, "Weight Cutoff": 1e-2, "Number of Histories": 16 * prob.n,
                     "Random Seed": 3, "Convergence Tolerance": 1e-8, "Maximum Iterations": 50,
-                    "Tally Tile": 256, "Walk Block Size": 256, "Walk Blocks Per CU": 0})
+                    "Tally Tile": 256, "Walk Block Size": 256, "Walk Blocks Per CU": 0,
+                    "Tally Tile Precision": tile_prec})
     x = ctx.mcsa_solve(prob.b)
     st = ctx.stats()
-    ctx.set_params({"Tally Tile": 0})
+    ctx.set_params({"Tally Tile": 0, "Tally Tile Precision": "FP64"})
     assert st.converged and st.achieved_tol <= 1e-8
