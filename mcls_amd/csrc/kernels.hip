@@ -222,12 +222,17 @@ __device__ __forceinline__ uint32_t cdf_step(const DomainView& d, uint32_t& beg,
 // transition (row CDF inversion or alias slot), multiply the weight by
 // W_i sign(h_ij), stop when steps >= L or |w| < cutoff |w0|.
 // --------------------------------------------------------------------------
-// With TILE, collision tallies first go to a per-wave direct-mapped tile of
+// With TILE, collision tallies first go to a per-wave set-associative tile of
 // states in LDS (a.tile_slots entries of {state, partial sum}); a state that
-// finds its slot taken by another goes straight to HBM.  A wave flushes its
+// finds every slot of its set taken by others goes straight to HBM.  A wave flushes its
 // tile into the FP64 HBM tally at every work-queue grab and when it exits.
 // TILE = kTileF64: FP64 partial sums; kTileWalk: partial sums in the walk's
 // precision (FP32 walks: 8 bytes of LDS per entry instead of 12).
+// Slots per set of the tile (4: measured 2% faster than 2 and 7% faster
+// than direct mapping on the 256^3 27-point chosen walk).
+constexpr int kTileWaysLog2 = 2;
+constexpr int kTileWays = 1 << kTileWaysLog2;
+
 template <typename Real, int EST, int SRC, int SAMP, bool REC, int TILE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   using Acc = typename std::conditional<TILE != kTileOff, double, Real>::type;
@@ -405,13 +410,31 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       } else if (EST == EST_EXPECTED_VALUE) {
         for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
       } else if constexpr (TILE != kTileOff) {
-        const uint32_t slot = ((uint32_t)state * 2654435761u) >> a.tile_shift;
-        int k = tkeys[slot];
-        if (k < 0) {
-          k = atomicCAS(&tkeys[slot], -1, state);
-          if (k < 0) k = state;
+        // kTileWays-way sets: the state may sit in any slot of its set, whose
+        // keys are read together; empty ones are claimed by compare-and-swap
+        // in slot order (lanes of one wave with the same state agree on the
+        // slot, so a state never holds two).  Full sets send it to HBM.
+        const uint32_t s0 = (((uint32_t)state * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
+        int keys[kTileWays];
+        if constexpr (kTileWays == 4) {
+          const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
+          keys[0] = (int)kk.x; keys[1] = (int)kk.y; keys[2] = (int)kk.z; keys[3] = (int)kk.w;
+        } else {
+          const uint2 kk = *reinterpret_cast<const uint2*>(tkeys + s0);
+          keys[0] = (int)kk.x; keys[1] = (int)kk.y;
         }
-        if (k == state)
+        int slot = -1;
+#pragma unroll
+        for (int j = 0; j < kTileWays; ++j)
+          if (slot < 0 && keys[j] == state) slot = (int)s0 + j;
+#pragma unroll
+        for (int j = 0; j < kTileWays; ++j) {
+          if (slot < 0 && keys[j] < 0) {
+            const int k = atomicCAS(&tkeys[s0 + j], -1, state);
+            if (k < 0 || k == state) slot = (int)s0 + j;
+          }
+        }
+        if (slot >= 0)
           __hip_atomic_fetch_add(&tvals[slot], (TAcc)w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
           tally_add(&tally[state], (Acc)w);
@@ -490,9 +513,13 @@ hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
     k_walk<Real, EST, SRC, SAMP, true, kTileOff><<<c.grid, c.block, 0, s>>>(a);
   } else if constexpr (EST == EST_COLLISION) {
     const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * tile_entry_bytes(c.tile, sizeof(Real));
-    if (c.tile == kTileWalk && sizeof(Real) == 4) {
-      k_walk<Real, EST, SRC, SAMP, false, kTileWalk><<<c.grid, c.block, lds, s>>>(a);
-    } else if (c.tile != kTileOff) {
+    if constexpr (sizeof(Real) == 4) {
+      if (c.tile == kTileWalk) {
+        k_walk<Real, EST, SRC, SAMP, false, kTileWalk><<<c.grid, c.block, lds, s>>>(a);
+        return hipGetLastError();
+      }
+    }
+    if (c.tile != kTileOff) {
       k_walk<Real, EST, SRC, SAMP, false, kTileF64><<<c.grid, c.block, lds, s>>>(a);
     } else {
       k_walk<Real, EST, SRC, SAMP, false, kTileOff><<<c.grid, c.block, 0, s>>>(a);
