@@ -152,6 +152,10 @@ struct mcls_gpu_context {
   DBuf<uint4> d_tgt32;
   DBuf<mcls::Rec32> d_tgt64;
   bool alias_valid = false;
+  DBuf<uint4> d_srec;        // strided alias records, n * d_stride
+  int d_stride = 0;
+  bool strided_valid = false;
+  DBuf<int> maxlen;          // longest domain row (device scalar)
   bool tgt32_valid = false, tgt64_valid = false;
   // learned proposal: importance g and its transition tables
   DBuf<float> g_imp;
@@ -613,6 +617,7 @@ int build_domain(mcls_gpu_context* c) {
                                  c->d_hdr64.p, nullptr, nullptr, false, c->flags.p, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->alias_valid = false;
+  c->strided_valid = false;
   c->tgt32_valid = false;
   c->tgt64_valid = false;
   c->proposal_valid = false;
@@ -649,6 +654,34 @@ int ensure_alias(mcls_gpu_context* c) {
     return fail(MCLS_ERR_UNSUPPORTED, "alias sampler: domain row " + std::to_string(flags[mcls::FLAG_LONG_ROW] - 1) +
                                           " has more than 64 entries (use the CDF sampler)");
   c->alias_valid = true;
+  return MCLS_OK;
+}
+
+// Strided alias records (FP32 walks, collision or forward estimator), built
+// on first use: the stride is the smallest power of two >= the longest row.
+int ensure_strided(mcls_gpu_context* c) {
+  if (c->prm.sampler != mcls::SAMPLER_ALIAS_STRIDED || c->strided_valid) return MCLS_OK;
+  if (c->prm.precision != 32)
+    return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Alias Strided' needs FP32 walks (Walk Precision 'FP32')");
+  if (!c->prm.forward && c->prm.estimator == mcls::EST_EXPECTED_VALUE)
+    return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Alias Strided' tallies collisions only");
+  MCLS_TRY(c->maxlen.ensure(1, "row length"));
+  HIP_TRY(mcls::launch_fill_i32(c->maxlen.p, 1, 0, c->stream));
+  HIP_TRY(mcls::launch_max_row_len(c->n, c->d_rp.p, c->maxlen.p, c->stream));
+  int longest = 0;
+  HIP_TRY(hipMemcpyAsync(&longest, c->maxlen.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (longest > 64)
+    return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Alias Strided': a domain row has " + std::to_string(longest) +
+                                          " entries (at most 64)");
+  int stride = 1;
+  while (stride < longest) stride <<= 1;
+  MCLS_TRY(c->d_srec.ensure((size_t)c->n * (size_t)stride, "strided alias records"));
+  HIP_TRY(mcls::launch_pack_strided(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, stride,
+                                    c->d_srec.p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->d_stride = stride;
+  c->strided_valid = true;
   return MCLS_OK;
 }
 
@@ -739,7 +772,9 @@ int ensure_proposal(mcls_gpu_context* c) {
 // Target records of the walk precision in use, for the CDF and learned
 // samplers (alias walks read their own records).
 int ensure_targets(mcls_gpu_context* c) {
-  if (c->prm.sampler == mcls::SAMPLER_ALIAS || c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS) return MCLS_OK;
+  if (c->prm.sampler == mcls::SAMPLER_ALIAS || c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS ||
+      c->prm.sampler == mcls::SAMPLER_ALIAS_STRIDED)
+    return MCLS_OK;
   const size_t m = (size_t)(c->d_nnz > 0 ? c->d_nnz : 1);
   // CDF walks of both precisions read the FP64 records (which hold the CDF
   // value); the FP32 learned walk reads the 16-byte FP32 records
@@ -762,6 +797,7 @@ int ensure_domain_impl(mcls_gpu_context* c);
 int ensure_domain(mcls_gpu_context* c) {
   MCLS_TRY(ensure_domain_impl(c));
   MCLS_TRY(ensure_alias(c));
+  MCLS_TRY(ensure_strided(c));
   MCLS_TRY(ensure_targets(c));
   return ensure_proposal(c);
 }
@@ -826,6 +862,8 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.hdr32 = c->d_hdr32.p;
   a.d.hdr64 = c->d_hdr64.p;
   a.d.rec32 = c->d_rec32.p;
+  a.d.srec = c->d_srec.p;
+  a.d.stride = (uint32_t)c->d_stride;
   a.d.slot64 = c->d_slot64.p;
   a.d.tgt32 = c->d_tgt32.p;
   a.d.tgt64 = c->d_tgt64.p;
@@ -886,6 +924,8 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
   if (c->prm.sampler == mcls::SAMPLER_LEARNED_ALIAS && c->prm.precision != 32)
     return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Learned Alias' needs FP32 walks (Walk Precision 'FP32')");
+  if (c->prm.sampler == mcls::SAMPLER_ALIAS_STRIDED && !c->strided_valid)
+    return fail(MCLS_ERR_STATE, "strided alias records not built");
   if (learned_sampler(c) && c->model) MCLS_TRY(model_importance(c, rhs));
   const bool forward = c->prm.forward != 0;
   if (forward && c->prm.estimator != mcls::EST_COLLISION)
@@ -1591,6 +1631,7 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     else if (key_is(v, "Alias")) p.sampler = mcls::SAMPLER_ALIAS;
     else if (key_is(v, "Learned")) p.sampler = mcls::SAMPLER_LEARNED;
     else if (key_is(v, "Learned Alias")) p.sampler = mcls::SAMPLER_LEARNED_ALIAS;
+    else if (key_is(v, "Alias Strided")) p.sampler = mcls::SAMPLER_ALIAS_STRIDED;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Transition Sampler '") + v + "'");
   } else if (key_is(key, "Walk Precision")) {
     if (key_is(v, "FP64")) p.precision = 64;
@@ -1704,6 +1745,8 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   const int64_t nh = num_histories(c);
   if (!c->prm.forward) MCLS_TRY(build_source(c, c->rhs.p, nh));
   else if (first + count > nh) return fail(MCLS_ERR_INVALID_ARG, "history range beyond the forward source");
+  if (c->prm.sampler == mcls::SAMPLER_ALIAS_STRIDED && !c->strided_valid)
+    return fail(MCLS_ERR_STATE, "strided alias records not built");
   WalkArgs a = walk_args(c, c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
   a.paths = dpaths.p;
   a.wfinal = dw.p;

@@ -301,7 +301,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   // state (FP32 alias records, CDF / learned target records); FP64 alias
   // walks load it per step.
   constexpr bool FUSED = SAMP == SAMPLER_ALIAS && sizeof(Real) == 4;
-  constexpr bool CARRY = SAMP != SAMPLER_ALIAS || FUSED;
+  // strided alias records hold the current row's W: no header at all
+  constexpr bool STRIDED = SAMP == SAMPLER_ALIAS_STRIDED;
+  constexpr bool CARRY = (SAMP != SAMPLER_ALIAS || FUSED) && !STRIDED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
   int slo = 0, shi = 0;         // stratified: birth states of the current batch
   bool exhausted = false;       // wave-uniform
@@ -394,7 +396,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     }
     if (!alive) break;  // queue exhausted and no history left on this lane
 
-    if (!CARRY) {
+    if (!CARRY && !STRIDED) {
       const uint4 hd = Row<Real>::hdr(d, state);
       beg = hd.x;
       len = hd.y;
@@ -442,13 +444,22 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         tally_add(&tally[state], w);
       }
     };
-    bool end = len == 0;  // absorbing row (all of h dropped)
+    bool end = !STRIDED && len == 0;  // absorbing row (all of h dropped)
     if (end) {
       tally_step();
     } else {
       const uint64_t r = xorshift64(rng);
       uint32_t cp;
-      if constexpr (FUSED) {
+      bool absorbed = false;  // strided: the record says the row is empty
+      if constexpr (STRIDED) {
+        const uint32_t slot = __umulhi((uint32_t)(r >> 32), d.stride);
+        const uint4 sr = d.srec[(size_t)state * d.stride + slot];
+        tally_step();
+        const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
+        absorbed = sr.y == kStridedAbsorbing;
+        cp = u2 < __uint_as_float(sr.x) ? sr.y : sr.z;
+        if (!absorbed) w = w * (Real)__uint_as_float(sr.w);
+      } else if constexpr (FUSED) {
         // alias draw: slot from the high 32 bits, keep-or-alias from the low
         // 32; the record also holds both candidates' row headers
         const uint32_t slot = __umulhi((uint32_t)(r >> 32), len);
@@ -487,12 +498,16 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         cp = cdf_step<Real>(d, beg, len, Wrow, Tables<Real>::u01(r), tally_step);
         w = wn;
       }
-      if (cp & 0x80000000u) w = -w;
-      state = (int)(cp & 0x7FFFFFFFu);
-      ++steps;
-      ++my_steps;
-      if (REC && steps <= a.max_len) path[steps] = state;
-      end = steps >= a.history_length || (w < 0 ? -w : w) < wc;
+      if (absorbed) {
+        end = true;  // the weight is not read again: the walk stops here
+      } else {
+        if (cp & 0x80000000u) w = -w;
+        state = (int)(cp & 0x7FFFFFFFu);
+        ++steps;
+        ++my_steps;
+        if (REC && steps <= a.max_len) path[steps] = state;
+        end = steps >= a.history_length || (w < 0 ? -w : w) < wc;
+      }
     }
     if (end) {
       alive = false;
@@ -536,6 +551,12 @@ hipError_t launch_walk_t(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
   if (c.sampler == SAMPLER_LEARNED_ALIAS) {
     if constexpr (sizeof(Real) == 4) return launch_walk_s<Real, EST, SRC, SAMPLER_LEARNED_ALIAS>(a, c, s);
     return hipErrorInvalidValue;  // FP32 records only; the context refuses FP64 first
+  }
+  if (c.sampler == SAMPLER_ALIAS_STRIDED) {
+    // FP32 records, collision or forward estimator; the context refuses the rest first
+    if constexpr (sizeof(Real) == 4 && EST != EST_EXPECTED_VALUE)
+      return launch_walk_s<Real, EST, SRC, SAMPLER_ALIAS_STRIDED>(a, c, s);
+    return hipErrorInvalidValue;
   }
   return c.sampler == SAMPLER_ALIAS ? launch_walk_s<Real, EST, SRC, SAMPLER_ALIAS>(a, c, s)
                                     : launch_walk_s<Real, EST, SRC, SAMPLER_CDF>(a, c, s);
@@ -1100,6 +1121,69 @@ __global__ void k_pack_rows(int n, const int* __restrict__ d_rp, const uint32_t*
       r.b = make_uint4((uint32_t)bs, (uint32_t)bo, __float_as_uint(W32[js]), __float_as_uint(W32[jo]));
       rec32[beg + k] = r;
       slot64[beg + k] = make_uint4((uint32_t)__double2loint(thr[k]), (uint32_t)__double2hiint(thr[k]), self, other);
+    }
+  }
+}
+
+// Longest domain row.
+__global__ void k_max_row_len(int n, const int* __restrict__ d_rp, int* out) {
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = max(m, d_rp[i + 1] - d_rp[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_down(m, off));
+  if (__lane_id() == 0) atomicMax(out, m);
+}
+
+// Strided alias tables: Vose's method (as k_pack_rows) over the row padded
+// with zero-probability slots to `stride` entries.  A padding slot is never
+// kept (threshold 0); should rounding leave one over at the end, it aliases
+// the row's first entry.  Absorbing rows get kStridedAbsorbing records.
+__global__ void k_pack_strided(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
+                               const double* __restrict__ h64, const double* __restrict__ W64,
+                               const float* __restrict__ W32, int stride, uint4* __restrict__ srec) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int beg = d_rp[i];
+    const int z = d_rp[i + 1] - beg;
+    uint4* out = srec + (size_t)i * (size_t)stride;
+    if (z == 0) {
+      for (int k = 0; k < stride; ++k)
+        out[k] = make_uint4(0u, kStridedAbsorbing, kStridedAbsorbing, 0u);
+      continue;
+    }
+    const double W = W64[i];
+    double sc[64];
+    int small[64], large[64], al[64];
+    double thr[64];
+    int ns = 0, nl = 0;
+    for (int k = 0; k < stride; ++k) {
+      sc[k] = k < z ? (fabs(h64[beg + k]) / W) * (double)stride : 0.0;
+      if (sc[k] < 1.0) small[ns++] = k;
+      else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+      const int l = small[--ns];
+      const int g = large[--nl];
+      thr[l] = sc[l];
+      al[l] = g;
+      sc[g] = (sc[g] + sc[l]) - 1.0;
+      if (sc[g] < 1.0) small[ns++] = g;
+      else large[nl++] = g;
+    }
+    while (nl > 0) {
+      const int g = large[--nl];
+      thr[g] = 1.0;
+      al[g] = g;
+    }
+    while (ns > 0) {
+      const int l = small[--ns];
+      thr[l] = l < z ? 1.0 : 0.0;
+      al[l] = l < z ? l : 0;
+    }
+    const uint32_t wbits = __float_as_uint(W32[i]);
+    for (int k = 0; k < stride; ++k) {
+      const uint32_t other = col[beg + al[k]];
+      const uint32_t self = k < z ? col[beg + k] : other;
+      out[k] = make_uint4(__float_as_uint((float)thr[k]), self, other, wbits);
     }
   }
 }
@@ -1893,6 +1977,16 @@ hipError_t launch_scan_i32_excl(const int* in, int* out, int64_t n, int* tmp, hi
   return scan_incl<int>(in, out + 1, n, tmp, s);
 }
 
+hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s) {
+  k_max_row_len<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, d_rp, out);
+  return hipGetLastError();
+}
+hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
+                               const float* W32, int stride, uint4* srec, hipStream_t s) {
+  if (stride < 1 || stride > 64 || (stride & (stride - 1))) return hipErrorInvalidValue;
+  k_pack_strided<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, stride, srec);
+  return hipGetLastError();
+}
 hipError_t launch_gather_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,
                               hipStream_t s) {
   const int g = grid_for(n * w, 256);

@@ -65,6 +65,9 @@ struct DomainView {
   // learned proposal sampled by alias (FP32 walks): per entry a 32-byte
   // record {thr, col, alias col, lens | start, alias start, S_i/g_j of both}
   const Rec32* lrec32;    // nnz
+  // strided alias tables: n * stride records {thr, col, alias col, W_i}
+  const uint4* srec;
+  uint32_t stride;        // power of two >= the longest row
 };
 
 // EST_FORWARD: the forward method on the forward domain (rows of H): history
@@ -72,7 +75,18 @@ struct DomainView {
 // own sum at every collision; the sum goes to its start state at the end.
 enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1, EST_FORWARD = 2 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
-enum Sampler : int { SAMPLER_CDF = 0, SAMPLER_ALIAS = 1, SAMPLER_LEARNED = 2, SAMPLER_LEARNED_ALIAS = 3 };
+// SAMPLER_ALIAS_STRIDED: alias tables padded to a fixed power-of-two stride
+// per row (FP32 walks): the slot of row i is i * stride + floor(u stride) and
+// its 16-byte record carries the row's own W, so no row header is carried.
+enum Sampler : int {
+  SAMPLER_CDF = 0,
+  SAMPLER_ALIAS = 1,
+  SAMPLER_LEARNED = 2,
+  SAMPLER_LEARNED_ALIAS = 3,
+  SAMPLER_ALIAS_STRIDED = 4
+};
+// marks the records of an absorbing row (no entries): the walk ends there
+constexpr uint32_t kStridedAbsorbing = 0xFFFFFFFFu;
 
 // Work queue of the walk kernel: one counter per XCD slot, 128 bytes apart.
 constexpr int kQueueSlots = 8;
@@ -161,6 +175,12 @@ hipError_t launch_build_proposal(int n, const int* d_rp, const uint32_t* col, co
 hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const double* h64,
                             const double* W64, const float* W32, uint4* hdr32, uint4* hdr64,
                             Rec32* rec32, uint4* slot64, bool alias, int* flags, hipStream_t s);
+// *out = max(*out, longest domain row)
+hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s);
+// Strided alias tables: Vose over each row padded with zero-probability
+// slots to `stride` entries; records {thr (FP32), col, alias col, W32_i}.
+hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64,
+                               const double* W64, const float* W32, int stride, uint4* srec, hipStream_t s);
 
 // Target records (either output may be null).  FP64 records also carry the
 // entry's own CDF value, so one load serves the search and the transition.

@@ -297,6 +297,56 @@ int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
   return 0;
 }
 
+int orc_alias_stride(int n, const int64_t* d_rp) {
+  int64_t longest = 0;
+  for (int i = 0; i < n; ++i)
+    if (d_rp[i + 1] - d_rp[i] > longest) longest = d_rp[i + 1] - d_rp[i];
+  if (longest > 64) return 0;
+  int stride = 1;
+  while (stride < longest) stride <<= 1;
+  return stride;
+}
+
+/* k_pack_strided (mcls_amd/csrc/kernels.hip) restated: Vose over the row
+ * padded to `stride` slots. */
+int64_t orc_build_alias_strided(int n, const int64_t* d_rp, const double* d_h,
+                                const double* d_W, int stride, double* thr, int* alias) {
+  if (stride < 1 || stride > 64 || (stride & (stride - 1))) return -1;
+  double s[64];
+  int small[64], large[64];
+  for (int i = 0; i < n; ++i) {
+    int64_t beg = d_rp[i];
+    int z = (int)(d_rp[i + 1] - beg);
+    if (z > stride) return -(int64_t)(i + 1);
+    double* t = thr + (int64_t)i * stride;
+    int* al = alias + (int64_t)i * stride;
+    if (z == 0) {
+      for (int k = 0; k < stride; ++k) { t[k] = 0.0; al[k] = 0; }
+      continue;
+    }
+    int ns = 0, nl = 0;
+    for (int k = 0; k < stride; ++k) {
+      s[k] = k < z ? (fabs(d_h[beg + k]) / d_W[i]) * (double)stride : 0.0;
+      if (s[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+    }
+    while (ns > 0 && nl > 0) {
+      int l = small[--ns];
+      int g = large[--nl];
+      t[l] = s[l];
+      al[l] = g;
+      s[g] = (s[g] + s[l]) - 1.0;
+      if (s[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+    }
+    while (nl > 0) { int g = large[--nl]; t[g] = 1.0; al[g] = g; }
+    while (ns > 0) {
+      int l = small[--ns];
+      t[l] = l < z ? 1.0 : 0.0;
+      al[l] = l < z ? l : 0;
+    }
+  }
+  return 0;
+}
+
 int64_t orc_build_proposal_alias(int n, const int64_t* d_rp, const int* d_col,
                                  const double* d_h, const float* g, double* thr,
                                  int* alias, double* lfac) {
@@ -415,6 +465,7 @@ typedef struct {
   float* cdf32;
   float* W32;
   int learned_alias; /* thr / alias / lfac hold the learned proposal's tables */
+  int stride;    /* > 0: thr / thr32 / alias are strided tables (n * stride) */
   double* thr;   /* alias tables (sampler == ORC_SAMPLER_ALIAS) */
   float* thr32;
   int* alias;
@@ -509,7 +560,11 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
       int k;
-      if (d->learned_alias) {  /* the same proposal drawn by alias */
+      if (d->stride) {  /* strided alias tables: slot floor(u stride) of the row */
+        int64_t o = (int64_t)state * d->stride;
+        k = alias_pick32(r, d->stride, d->thr32 + o, d->alias + o);
+        w = w * d->W32[state];
+      } else if (d->learned_alias) {  /* the same proposal drawn by alias */
         k = alias_pick32(r, len, d->thr32 + beg, d->alias + beg);
         w = w * d->lfac32[beg + k];
       } else if (d->lcdf32) {  /* importance-reshaped: P = |h| g / S, factor S / g */
@@ -544,7 +599,11 @@ static int64_t walk_one(const walk_job* J, int64_t hid, double* x, int* path,
       if (len == 0) break;
       uint64_t r = orc_xorshift64(&s);
       int k;
-      if (d->learned_alias) {
+      if (d->stride) {
+        int64_t o = (int64_t)state * d->stride;
+        k = alias_pick64(r, d->stride, d->thr + o, d->alias + o);
+        w = w * d->W[state];
+      } else if (d->learned_alias) {
         k = alias_pick64(r, len, d->thr + beg, d->alias + beg);
         w = w * d->lfac[beg + k];
       } else if (d->lcdf) {
@@ -586,8 +645,9 @@ static void* walk_worker(void* arg) {
 static void domain_view_init(domain_view* d, int n, const int64_t* rp,
                              const int* col, const double* h,
                              const double* cdf, const double* W, int need32,
-                             int need_alias, const float* importance) {
+                             int need_alias, const float* importance, int strided) {
   d->n = n;
+  d->stride = 0;
   d->rp = rp;
   d->col = col;
   d->h = h;
@@ -628,6 +688,17 @@ static void domain_view_init(domain_view* d, int n, const int64_t* rp,
       d->lcdf32[k] = (float)d->lcdf[k];
       d->lfac32[k] = (float)d->lfac[k];
     }
+  }
+  if (strided) {
+    int stride = orc_alias_stride(n, rp);
+    int64_t m = (int64_t)n * (stride > 0 ? stride : 1);
+    d->thr = (double*)malloc((size_t)(m ? m : 1) * sizeof(double));
+    d->thr32 = (float*)malloc((size_t)(m ? m : 1) * sizeof(float));
+    d->alias = (int*)calloc((size_t)(m ? m : 1), sizeof(int));
+    if (stride > 0) orc_build_alias_strided(n, rp, h, W, stride, d->thr, d->alias);
+    for (int64_t k = 0; k < m; ++k) d->thr32[k] = stride > 0 ? (float)d->thr[k] : 0.0f;
+    d->stride = stride;
+    need_alias = 0;
   }
   if (need_alias) {
     int64_t nnz = rp[n];
@@ -682,7 +753,8 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
   domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32,
                    p->sampler == ORC_SAMPLER_ALIAS || p->sampler == ORC_SAMPLER_LEARNED_ALIAS,
                    (p->sampler == ORC_SAMPLER_LEARNED || p->sampler == ORC_SAMPLER_LEARNED_ALIAS)
-                       ? p->importance : NULL);
+                       ? p->importance : NULL,
+                   p->sampler == ORC_SAMPLER_ALIAS_STRIDED);
   uint64_t* q = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* qcdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* ncdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));

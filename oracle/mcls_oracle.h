@@ -83,6 +83,17 @@ int orc_source_birth(int n, const double* b, const uint64_t* q,
 int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
                         const double* d_W, double* thr, int* alias);
 
+/* Strided alias tables (the build's "Alias Strided" sampler): stride = the
+ * smallest power of two >= the longest row (>= 1; 0 when a row has more
+ * than 64 entries).  Each row is padded with zero-probability slots to
+ * `stride` entries and Vose's method runs over all of them, as
+ * orc_build_alias does; thr[n*stride], alias[n*stride] (row-local index).
+ * A padding slot keeps threshold 0; one that rounding leaves over at the end
+ * aliases the row's first entry. */
+int orc_alias_stride(int n, const int64_t* d_rp);
+int64_t orc_build_alias_strided(int n, const int64_t* d_rp, const double* d_h,
+                                const double* d_W, int stride, double* thr, int* alias);
+
 /* Importance-reshaped transition tables (build extension, the learned
  * proposal): for domain row i with entries (j, h_ij) and importance g_j > 0,
  * P_ij = |h_ij| g_j / S_i with S_i = sum_k |h_ik| g_k, sampled by CDF
@@ -108,8 +119,12 @@ int64_t orc_build_proposal_alias(int n, const int64_t* d_rp, const int* d_col,
 enum { ORC_EST_COLLISION = 0, ORC_EST_EXPECTED_VALUE = 1, ORC_EST_FORWARD = 2 };
 enum { ORC_SRC_RANDOM = 0, ORC_SRC_STRATIFIED = 1 };
 /* ORC_SAMPLER_LEARNED_ALIAS: the learned proposal drawn with alias tables
- * (orc_build_proposal_alias), weight factor S_i / g_j as for LEARNED. */
-enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2, ORC_SAMPLER_LEARNED_ALIAS = 3 };
+ * (orc_build_proposal_alias), weight factor S_i / g_j as for LEARNED.
+ * ORC_SAMPLER_ALIAS_STRIDED: orc_build_alias_strided tables, slot
+ * floor(u stride) of the row (same draw as ORC_SAMPLER_ALIAS with len =
+ * stride). */
+enum { ORC_SAMPLER_CDF = 0, ORC_SAMPLER_ALIAS = 1, ORC_SAMPLER_LEARNED = 2, ORC_SAMPLER_LEARNED_ALIAS = 3,
+       ORC_SAMPLER_ALIAS_STRIDED = 4 };
 
 typedef struct orc_mc_params {
   int64_t num_histories;   /* histories per MC solve                       */

@@ -21,6 +21,7 @@ SAMPLER_CDF = 0
 SAMPLER_ALIAS = 1
 SAMPLER_LEARNED = 2
 SAMPLER_LEARNED_ALIAS = 3
+SAMPLER_ALIAS_STRIDED = 4
 
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -111,6 +112,10 @@ def lib():
     L.orc_build_proposal.restype = i32
     L.orc_build_alias.argtypes = [i32, _i64p, _f64p, _f64p, _f64p, _i32p]
     L.orc_build_alias.restype = i64
+    L.orc_alias_stride.argtypes = [i32, _i64p]
+    L.orc_alias_stride.restype = i32
+    L.orc_build_alias_strided.argtypes = [i32, _i64p, _f64p, _f64p, i32, _f64p, _i32p]
+    L.orc_build_alias_strided.restype = i64
     L.orc_build_proposal_alias.argtypes = [i32, _i64p, _i32p, _f64p, _f32p, _f64p, _i32p, _f64p]
     L.orc_build_proposal_alias.restype = i64
     L.orc_jacobi_dinv.argtypes = [i32, _i64p, _i32p, _f64p, _f64p]
@@ -276,6 +281,24 @@ def build_alias(dom):
     if rc != 0:
         raise ValueError(f"row {-rc - 1} longer than 64 entries")
     return thr[: dom.nnz], alias[: dom.nnz]
+
+
+def alias_stride(dom):
+    """Smallest power of two >= the longest row (0 when a row exceeds 64)."""
+    return int(lib().orc_alias_stride(dom.n, dom.rp))
+
+
+def build_alias_strided(dom):
+    """Strided alias tables: (stride, thr[n, stride], alias[n, stride])."""
+    stride = alias_stride(dom)
+    if stride == 0:
+        raise ValueError("a row is longer than 64 entries")
+    thr = np.zeros(dom.n * stride)
+    alias = np.zeros(dom.n * stride, np.int32)
+    rc = lib().orc_build_alias_strided(dom.n, dom.rp, dom.h, dom.W, stride, thr, alias)
+    if rc != 0:
+        raise ValueError(f"strided alias tables failed ({rc})")
+    return stride, thr.reshape(dom.n, stride), alias.reshape(dom.n, stride)
 
 
 def mc_solve(dom, b, params, solve_index=0):

@@ -1,0 +1,146 @@
+"""Strided alias tables ("Transition Sampler" = "Alias Strided", FP32
+walks) on the GPU against the oracle: sampled paths and final weights
+bit-exact (adjoint and forward, random and stratified births, operators with
+absorbing rows and rows shorter than the stride), MC tallies to the FP32
+summation order, MCSA to a direct solve, the LDS tile and the state order on
+top, and the error paths."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+from mcls_amd import capi
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+BASE = {"Neumann Relaxation": 1.0, "Composite Operator Threshold": 0.0, "Walk Precision": "FP32",
+        "Estimator Type": "Collision", "Weight Cutoff": 1e-3, "History Length": 300,
+        "Transition Sampler": "Alias Strided", "Monte Carlo Type": "Adjoint"}
+
+
+def _problem(kind):
+    if kind == "s27":
+        p = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 10, seed=2, p0=0.6, p1=0.9, p2=0.8, p3=1.2),
+                               want64=False, want32=True)
+        return p, po.Csr(p.rp, p.col, p.val32.astype(np.float64)), "f32"
+    if kind == "poisson":
+        p = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 30, 30, seed=0))
+        return p, po.Csr(p.rp, p.col, p.values()), "f64"
+    # random nonsymmetric operator with absorbing states (columns of A with
+    # only their diagonal: empty rows of H^T) and rows of 0..~12 entries
+    rng = np.random.default_rng(4)
+    n = 150
+    S = sp.random(n, n, density=0.05, random_state=rng, format="lil")
+    for j in range(0, n, 7):
+        S[:, j] = 0.0
+    S = sp.csr_matrix(S)
+    S.data = -S.data * 0.4
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    S = (S + sp.diags(np.abs(S).sum(axis=1).A1 * 1.5 + 1.0)).tocsr()
+    S.sort_indices()
+    p = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(np.float64), None,
+                     rng.random(n) - 0.3)
+    return p, po.Csr(p.rp, p.col, p.values()), "f64"
+
+
+@pytest.mark.parametrize("forward", [False, True])
+@pytest.mark.parametrize("source", ["Random", "Stratified"])
+@pytest.mark.parametrize("kind", ["s27", "poisson", "absorbing"])
+def test_paths_bitexact(kind, source, forward):
+    prob, A, dt = _problem(kind)
+    L, nh = 300, 6000
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv, forward=forward)
+    rhs = prob.b * dinv
+    rhs[::5] *= -1.0
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Source Sampling Type": source, "Number of Histories": nh, "Random Seed": 41,
+                                     "Monte Carlo Type": "Forward" if forward else "Adjoint"}))
+        count = 2000
+        gp, gw = ctx.record_paths(rhs, 7, 50, count, L)
+    est = po.EST_FORWARD if forward else po.EST_COLLISION
+    mp = po.mc_params(nh, L, 1e-3, 32, est, po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM,
+                      seed=41, sampler=po.SAMPLER_ALIAS_STRIDED)
+    op, ow = po.record_paths(dom, rhs, mp, 50, count, L, solve_index=7)
+    assert np.array_equal(gp, op)
+    assert np.array_equal(gw, ow)
+    assert (gp[:, 1] >= 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("source", ["Random", "Stratified"])
+@pytest.mark.parametrize("tile", [0, 256])
+def test_tally_matches_oracle(source, tile):
+    prob, A, dt = _problem("s27")
+    nh, L = 40000, 100
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Source Sampling Type": source, "Number of Histories": nh, "History Length": L,
+                                     "Weight Cutoff": 1e-2, "Random Seed": 5, "Tally Tile": tile,
+                                     "Tally Tile Precision": "Walk"}))
+        xg = ctx.mc_solve_raw(rhs, 3)
+        st = ctx.stats()
+    mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION,
+                      po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=5,
+                      sampler=po.SAMPLER_ALIAS_STRIDED)
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=3)
+    assert st.steps == steps
+    assert np.linalg.norm(xg - xo) <= 2e-5 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+def test_mcsa_converges(ordered):
+    prob, A, dt = _problem("s27")
+    with capi.Context(0) as ctx:
+        if ordered:
+            ctx.set_state_order(capi.brick_order(10, 10, 10, 4, 4, 4))
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Source Sampling Type": "Stratified", "Number of Histories": 16 * prob.n,
+                                     "Weight Cutoff": 1e-1, "History Length": 1000, "Random Seed": 9,
+                                     "Convergence Tolerance": 1e-8, "Maximum Iterations": 60,
+                                     "Tally Tile": 512, "Tally Tile Precision": "Walk"}))
+        x = ctx.mcsa_solve(prob.b)
+        st = ctx.stats()
+    assert st.converged == 1 and st.achieved_tol <= 1e-8
+    S = sp.csr_matrix((prob.val32.astype(np.float64), prob.col, prob.rp), shape=(prob.n, prob.n))
+    xd = sla.spsolve(S.tocsc(), prob.b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+def test_errors():
+    prob, A, dt = _problem("s27")
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Number of Histories": 1000, "Walk Precision": "FP64"}))
+        with pytest.raises(capi.GpuError) as e:
+            ctx.mc_solve(prob.b)
+        assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
+        ctx.set_params({"Walk Precision": "FP32", "Estimator Type": "Expected Value"})
+        with pytest.raises(capi.GpuError) as e:
+            ctx.mc_solve(prob.b)
+        assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
+        ctx.set_params({"Estimator Type": "Collision"})
+        assert np.all(np.isfinite(ctx.mc_solve(prob.b)))
+    # a domain row of more than 64 entries
+    n = 80
+    S = sp.lil_matrix((n, n))
+    S.setdiag(100.0)
+    S[0, 1:] = -0.5
+    S = S.tocsr()
+    prob = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data, None, np.ones(n))
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Number of Histories": 1000, "Monte Carlo Type": "Forward"}))
+        with pytest.raises(capi.GpuError) as e:
+            ctx.mc_solve(prob.b)
+        assert e.value.code == capi.MCLS_ERR_UNSUPPORTED

@@ -257,3 +257,40 @@ def test_forward_source_and_one_step_tally():
     assert np.array_equal(paths[:, 0], np.repeat(np.arange(3), 4))
     # one transition: w = W_start sign(h) with W = 0.25, 0.5, 0.25 and h > 0
     assert np.array_equal(w, np.repeat([0.25, 0.5, 0.25], 4))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("est", [po.EST_COLLISION, po.EST_FORWARD])
+def test_strided_alias_is_unbiased(prec, est):
+    """Alias tables padded to a power-of-two stride (zero-probability slots)
+    keep E[x]; their slots reproduce |h_ij| / W_i, padding slots are never
+    kept, and the stride is the smallest power of two >= the longest row."""
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    H = np.eye(n) - dinv[:, None] * A
+    forward = est == po.EST_FORWARD
+    dom = po.build_domain(Ac, dinv, forward=forward)
+    L = 6
+    # adjoint collision: sum_k (H^T)^k-walks of the H^T domain give sum_k H^k b;
+    # forward: rows of H, the same series
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(L))
+    mp = po.mc_params(3000, L, 0.0, prec, est, po.SRC_STRATIFIED, seed=17, sampler=po.SAMPLER_ALIAS_STRIDED)
+    mean, se = _replicas(dom, b, mp, 60)
+    _check_z(mean, se, exact)
+    stride, thr, alias = po.build_alias_strided(dom)
+    lens = np.diff(dom.rp)
+    assert stride >= lens.max() and stride & (stride - 1) == 0 and (stride == 1 or stride // 2 < lens.max())
+    for i in range(n):
+        z = lens[i]
+        if z == 0:
+            continue
+        p = np.zeros(stride)
+        for k in range(stride):
+            p[k] += thr[i, k] / stride
+            p[alias[i, k]] += (1.0 - thr[i, k]) / stride
+        target = np.zeros(stride)
+        target[:z] = np.abs(dom.h[dom.rp[i]:dom.rp[i + 1]]) / dom.W[i]
+        assert np.max(np.abs(p - target)) < 1e-14
+        assert np.all(thr[i, z:] == 0.0) and np.all(alias[i] < z)
