@@ -62,7 +62,8 @@ FORMS = {
     # iteration (8 iterations of 8.5 steps per history; 1e-2 takes 6 of 16.5)
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
-               "Transition Sampler": "Alias", "Monte Carlo Type": "Adjoint",
+               # alias tables padded to the row stride: one 16-byte record per step
+               "Transition Sampler": "Alias Strided", "Monte Carlo Type": "Adjoint",
                # LDS tally tile partial sums in FP32 (flushed into the FP64
                # tally): 8 bytes per entry, more resident waves than FP64's 12
                "Tally Tile Precision": "Walk"},
@@ -72,6 +73,8 @@ FORMS = {
     # the forward method with the chosen form's walk (no per-step tally atomics)
     "forward": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                 "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
+                # the carried-header records: measured faster than the strided
+                # ones for this walk (0.227 s against 0.252 s on the 256^3 workload)
                 "Transition Sampler": "Alias", "Monte Carlo Type": "Forward"},
 }
 
@@ -114,22 +117,19 @@ def solve_once_host(ctx, b):
 
 
 def bytes_per_step(prec, z, sampler, forward=False):
-    """Algorithmic bytes of one walk step (one tally + one transition).
+    """Algorithmic bytes of one walk step (one tally + one transition), the
+    per-step figure of the measurement plan, whatever the record layout:
 
-    CDF:   16 (packed row header: start, len, W) + p z (CDF row) + 4 (packed
-           column) + 2p (tally read-modify-write)
-    Alias: FP32: 32 (slot record carrying both candidates' row headers) + 2p;
-           FP64: 16 (row header) + 16 (slot record) + 2p
+    CDF (and the learned CDF rows): 8 (row_ptr[i], row_ptr[i+1]) + p z (CDF
+        row) + 4 (col) + p (W_i, or the entry's weight factor) + 2p (tally
+        read-modify-write)
+    Alias (all alias forms): the CDF row replaced by 8 (one slot)
+    Forward: the tally read-modify-write replaced by 8 (b[state])
     p = 4 (FP32 walk) or 8 (FP64 walk); z = mean domain row length.
     """
     p = 4 if prec == "FP32" else 8
-    if forward:  # the tally is one register add of w b[state]: read 8 bytes of b instead
-        return (32 if (sampler == "Alias" and p == 4) else 16 + (16 if sampler == "Alias" else p * z + 4)) + 8
-    if sampler in ("Alias", "Learned Alias"):  # the learned records carry S_i/g_j in place of W_j
-        return 32 + 2 * p if p == 4 else 16 + 16 + 2 * p
-    if sampler == "Learned":  # reshaped CDF row plus the entry's weight factor
-        return 16 + p * z + 4 + p + 2 * p
-    return 16 + p * z + 4 + 2 * p
+    pick = 8 if "Alias" in sampler else p * z
+    return 8 + pick + 4 + p + (8 if forward else 2 * p)
 
 
 # the learned proposal drawn by alias records: the same per-step cost as the
@@ -386,7 +386,7 @@ def main():
         solve_once(ctx, dev)  # forward domain build and warm-up
         fdt, fst = solve_once(ctx, dev)
         log(f"forward FP32: {fdt:.3f}s iters={fst.iterations} tol={fst.achieved_tol:.2e}")
-        fbytes = bytes_per_step("FP32", z, "Alias", True)
+        fbytes = bytes_per_step("FP32", z, FORMS["forward"]["Transition Sampler"], True)
         fsteps_per_launch = fst.steps / max(fst.walk_launches, 1)
         out["forward"] = {"time_to_tol_s": fdt, "mcsa_iterations": fst.iterations,
                           "traffic": measured_traffic(args.workload, "forward", fsteps_per_launch,
