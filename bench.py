@@ -198,8 +198,8 @@ def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_or
     """HBM bytes per launch of a form's walk kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload
     (calibrated against a copy kernel), when its transitions per launch match
-    this run's to 2% and it was taken with the same LDS tally tile (size and
-    partial-sum precision) and state order; None otherwise."""
+    this run's to 2% and it was taken with the same sampler, LDS tally tile
+    (size and partial-sum precision) and state order; None otherwise."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"walk_{workload}_traffic.json")
     if not os.path.exists(path):
         return None
@@ -213,6 +213,8 @@ def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_or
     if rec.get("tally_tile") != tally_tile or rec.get("state_order") != state_order:
         return None
     if tally_tile and rec.get("tally_tile_precision") != FORMS[form].get("Tally Tile Precision", "FP64"):
+        return None
+    if rec.get("sampler") != FORMS[form]["Transition Sampler"]:
         return None
     return float(rec["traffic_bytes_per_launch"])
 
