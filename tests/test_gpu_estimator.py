@@ -3,7 +3,8 @@ mean of the reference estimator.  Over independent solves, the mean of the
 GPU estimate (build semantics: quantised source, per-history streams, the
 form's sampler and precision) and the mean of the reference-semantics
 restatement of MCLS's estimator (one stream, |b| CDF with lower_bound, LIFO
-source, weight sign(b) ||b||_1) agree within their standard errors, and both
+source, weight sign(b) ||b||_1) agree within their standard errors (also for
+the strided alias tables with the LDS tally tile), and both
 match the truncated Neumann series sum_{k<L} H^k b that the adjoint
 collision estimator targets."""
 import numpy as np
@@ -18,6 +19,8 @@ FORMS = {
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified", "Transition Sampler": "Alias"},
     "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random", "Transition Sampler": "CDF"},
     "fp32-cdf": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified", "Transition Sampler": "CDF"},
+    "strided": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified", "Transition Sampler": "Alias Strided",
+                "Tally Tile": 256, "Tally Tile Precision": "Walk"},
 }
 
 
