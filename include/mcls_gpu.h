@@ -155,10 +155,10 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        Vose alias tables (rows <= 64 entries), the importance-reshaped rows
  *        of the learned proposal by CDF inversion, the same proposal by
  *        alias records (FP32 walks, rows <= 64 entries), or alias tables
- *        padded to a power-of-two stride >= the longest row, one 16-byte
- *        record per slot holding the row's own W (FP32 walks, collision or
- *        forward estimator, rows <= 64 entries: half the bytes per row of
- *        "Alias" records on 27-point rows)
+ *        padded to the longest row's length, one 16-byte record per slot
+ *        holding the row's own W (FP32 walks, collision or forward
+ *        estimator, rows <= 64 entries: half the bytes per row of "Alias"
+ *        records on 27-point rows)
  *   i64 "Random Seed"          (0)
  *   i64 "Use Graph"            (0) capture each MCSA iteration in a hipGraph
  *   i64 "Verbosity"            (0)

@@ -658,7 +658,7 @@ int ensure_alias(mcls_gpu_context* c) {
 }
 
 // Strided alias records (FP32 walks, collision or forward estimator), built
-// on first use: the stride is the smallest power of two >= the longest row.
+// on first use: the stride is the longest row's length.
 int ensure_strided(mcls_gpu_context* c) {
   if (c->prm.sampler != mcls::SAMPLER_ALIAS_STRIDED || c->strided_valid) return MCLS_OK;
   if (c->prm.precision != 32)
@@ -674,8 +674,7 @@ int ensure_strided(mcls_gpu_context* c) {
   if (longest > 64)
     return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Alias Strided': a domain row has " + std::to_string(longest) +
                                           " entries (at most 64)");
-  int stride = 1;
-  while (stride < longest) stride <<= 1;
+  const int stride = longest > 0 ? longest : 1;
   MCLS_TRY(c->d_srec.ensure((size_t)c->n * (size_t)stride, "strided alias records"));
   HIP_TRY(mcls::launch_pack_strided(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, stride,
                                     c->d_srec.p, c->stream));

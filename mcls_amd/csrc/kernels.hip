@@ -1983,7 +1983,7 @@ hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s) {
 }
 hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
                                const float* W32, int stride, uint4* srec, hipStream_t s) {
-  if (stride < 1 || stride > 64 || (stride & (stride - 1))) return hipErrorInvalidValue;
+  if (stride < 1 || stride > 64) return hipErrorInvalidValue;
   k_pack_strided<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, stride, srec);
   return hipGetLastError();
 }

@@ -67,7 +67,7 @@ struct DomainView {
   const Rec32* lrec32;    // nnz
   // strided alias tables: n * stride records {thr, col, alias col, W_i}
   const uint4* srec;
-  uint32_t stride;        // power of two >= the longest row
+  uint32_t stride;        // the longest row's length (>= 1)
 };
 
 // EST_FORWARD: the forward method on the forward domain (rows of H): history
@@ -75,8 +75,8 @@ struct DomainView {
 // own sum at every collision; the sum goes to its start state at the end.
 enum Estimator : int { EST_COLLISION = 0, EST_EXPECTED_VALUE = 1, EST_FORWARD = 2 };
 enum SourceMode : int { SRC_RANDOM = 0, SRC_STRATIFIED = 1 };
-// SAMPLER_ALIAS_STRIDED: alias tables padded to a fixed power-of-two stride
-// per row (FP32 walks): the slot of row i is i * stride + floor(u stride) and
+// SAMPLER_ALIAS_STRIDED: alias tables padded to a fixed stride (the longest
+// row) per row (FP32 walks): the slot of row i is i * stride + floor(u stride) and
 // its 16-byte record carries the row's own W, so no row header is carried.
 enum Sampler : int {
   SAMPLER_CDF = 0,

@@ -302,16 +302,14 @@ int orc_alias_stride(int n, const int64_t* d_rp) {
   for (int i = 0; i < n; ++i)
     if (d_rp[i + 1] - d_rp[i] > longest) longest = d_rp[i + 1] - d_rp[i];
   if (longest > 64) return 0;
-  int stride = 1;
-  while (stride < longest) stride <<= 1;
-  return stride;
+  return longest > 0 ? (int)longest : 1;
 }
 
 /* k_pack_strided (mcls_amd/csrc/kernels.hip) restated: Vose over the row
  * padded to `stride` slots. */
 int64_t orc_build_alias_strided(int n, const int64_t* d_rp, const double* d_h,
                                 const double* d_W, int stride, double* thr, int* alias) {
-  if (stride < 1 || stride > 64 || (stride & (stride - 1))) return -1;
+  if (stride < 1 || stride > 64) return -1;
   double s[64];
   int small[64], large[64];
   for (int i = 0; i < n; ++i) {

@@ -84,8 +84,7 @@ int64_t orc_build_alias(int n, const int64_t* d_rp, const double* d_h,
                         const double* d_W, double* thr, int* alias);
 
 /* Strided alias tables (the build's "Alias Strided" sampler): stride = the
- * smallest power of two >= the longest row (>= 1; 0 when a row has more
- * than 64 entries).  Each row is padded with zero-probability slots to
+ * longest row's length (>= 1; 0 when a row has more than 64 entries).  Each row is padded with zero-probability slots to
  * `stride` entries and Vose's method runs over all of them, as
  * orc_build_alias does; thr[n*stride], alias[n*stride] (row-local index).
  * A padding slot keeps threshold 0; one that rounding leaves over at the end

@@ -284,7 +284,7 @@ def build_alias(dom):
 
 
 def alias_stride(dom):
-    """Smallest power of two >= the longest row (0 when a row exceeds 64)."""
+    """The longest row's length, at least 1 (0 when a row exceeds 64)."""
     return int(lib().orc_alias_stride(dom.n, dom.rp))
 
 
