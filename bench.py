@@ -50,10 +50,10 @@ def log(msg):
 
 WORKLOADS = {
     "cfg4": dict(kind=capi.STENCIL3D_27PT, nx=256, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
-                 histories=1 << 28, brick=8, label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
+                 histories=1 << 28, bricks=((32, 32, 32), (4, 4, 4)), label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
                  "FP32 CSR), point Jacobi, 2^28 histories/iteration, tol 1e-8"),
     "cfg4-64": dict(kind=capi.STENCIL3D_27PT, nx=64, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
-                    histories=16 * 64 ** 3, brick=8, label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "
+                    histories=16 * 64 ** 3, bricks=((32, 32, 32), (4, 4, 4)), label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "
                     "16 histories/unknown, tol 1e-8"),
 }
 
@@ -259,8 +259,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-proposal selection of the chosen form (plain sampling)")
     ap.add_argument("--order", default="bricks", choices=["bricks", "natural"],
-                    help="state order of the context: the grid's unknowns in cubic bricks (the workload's "
-                    "brick edge) or the generator's x-fastest numbering")
+                    help="state order of the context: the grid's unknowns in nested bricks (the workload's "
+                    "brick sizes) or the generator's x-fastest numbering")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -287,12 +287,13 @@ def main():
     ctx = capi.Context(0)
     # The context numbers the unknowns brick by brick (P A P^T; b and x stay
     # in the generator's numbering at the API): the walks born close together
-    # then share rows in L2.  Same operator, same histories, same tolerance.
+    # then share rows in L2 and LDS tile slots.  Same operator, same
+    # histories, same tolerance.
     order_label = "natural"
-    if args.order == "bricks" and w.get("brick"):
-        e = w["brick"]
-        ctx.set_state_order(capi.brick_order(w["nx"], w["nx"], w["nx"], e, e, e))
-        order_label = f"{e}^3 bricks"
+    if args.order == "bricks" and w.get("bricks"):
+        lv = w["bricks"]
+        ctx.set_state_order(capi.nested_brick_order(w["nx"], w["nx"], w["nx"], lv))
+        order_label = " in ".join(f"{b[0]}^3 bricks" for b in reversed(lv))
         log(f"state order: {order_label}")
     ctx.set_problem(prob, "f32" if w["f32"] else "f64")
     ctx.set_point_jacobi()

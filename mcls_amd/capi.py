@@ -326,6 +326,32 @@ def brick_order(nx, ny, nz, bx, by, bz):
     return key.astype(np.int32)
 
 
+def nested_brick_order(nx, ny, nz, levels):
+    """State order of an nx x ny x nz grid visited brick by brick on several
+    levels: `levels` lists brick sizes (bx, by, bz) from the coarsest to the
+    finest, each dividing the one before it; the grid is cut into bricks of
+    the first size in x-fastest order, each of those into bricks of the next
+    size in x-fastest order, and so on, the points of a finest brick
+    x-fastest.  One level is brick_order.  Returns order[k] = the unknown
+    stored k-th (int32)."""
+    levels = [tuple(int(v) for v in lv) for lv in levels]
+    if not levels or min(nx, ny, nz) < 1 or any(min(lv) < 1 for lv in levels):
+        raise ValueError("grid and brick sizes must be >= 1")
+    for outer, inner in zip(levels, levels[1:]):
+        if any(o % i for o, i in zip(outer, inner)):
+            raise ValueError("each brick size must divide the one before it")
+    Z, Y, X = np.meshgrid(np.arange(nz, dtype=np.int64), np.arange(ny, dtype=np.int64),
+                          np.arange(nx, dtype=np.int64), indexing="ij")
+    keys = []  # most significant first
+    nb = [-(-nz // levels[0][2]), -(-ny // levels[0][1]), -(-nx // levels[0][0])]
+    keys.append(((Z // levels[0][2]) * nb[1] + Y // levels[0][1]) * nb[2] + X // levels[0][0])
+    for outer, inner in zip(levels, levels[1:]):
+        keys += [(Z % outer[2]) // inner[2], (Y % outer[1]) // inner[1], (X % outer[0]) // inner[0]]
+    last = levels[-1]
+    keys += [Z % last[2], Y % last[1], X % last[0]]
+    return np.lexsort([k.ravel() for k in reversed(keys)]).astype(np.int32)
+
+
 def build_problem(spec, want64=True, want32=False, nthreads=None):
     L = problems_lib()
     n = C.c_int64(0)

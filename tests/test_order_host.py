@@ -50,3 +50,46 @@ def test_brick_order_rejects_bad_sizes():
         capi.brick_order(4, 4, 0, 2, 2, 2)
     with pytest.raises(ValueError):
         capi.brick_order(4, 4, 4, 2, 0, 2)
+
+
+def _nested_loop(nx, ny, nz, levels):
+    """Recursive definition: bricks of levels[0] x-fastest, each visited by
+    the remaining levels, the points of the last level x-fastest."""
+    out = []
+
+    def visit(x0, y0, z0, x1, y1, z1, lv):
+        if lv == len(levels):
+            for z in range(z0, z1):
+                for y in range(y0, y1):
+                    for x in range(x0, x1):
+                        out.append(x + nx * (y + ny * z))
+            return
+        bx, by, bz = levels[lv]
+        for gz in range(z0, z1, bz):
+            for gy in range(y0, y1, by):
+                for gx in range(x0, x1, bx):
+                    visit(gx, gy, gz, min(gx + bx, x1), min(gy + by, y1), min(gz + bz, z1), lv + 1)
+
+    visit(0, 0, 0, nx, ny, nz, 0)
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("dims,levels", [((16, 16, 16), [(8, 8, 8), (4, 4, 4)]),
+                                         ((12, 8, 10), [(6, 4, 5), (3, 2, 5), (1, 1, 1)]),
+                                         ((9, 7, 5), [(4, 4, 4), (2, 2, 2)]),
+                                         ((8, 8, 8), [(4, 2, 8)])])
+def test_nested_brick_order_matches_recursion(dims, levels):
+    o = capi.nested_brick_order(*dims, levels)
+    assert np.array_equal(o, _nested_loop(*dims, levels))
+    assert np.array_equal(np.sort(o), np.arange(dims[0] * dims[1] * dims[2]))
+
+
+def test_nested_brick_order_one_level_is_brick_order():
+    assert np.array_equal(capi.nested_brick_order(10, 9, 8, [(4, 3, 2)]), capi.brick_order(10, 9, 8, 4, 3, 2))
+
+
+def test_nested_brick_order_rejects_non_nesting_sizes():
+    with pytest.raises(ValueError):
+        capi.nested_brick_order(8, 8, 8, [(4, 4, 4), (3, 3, 3)])
+    with pytest.raises(ValueError):
+        capi.nested_brick_order(8, 8, 8, [])
