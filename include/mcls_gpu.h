@@ -236,11 +236,15 @@ int mcls_gpu_convergence_criteria(mcls_gpu_context* ctx, int64_t max_iters, doub
  * transport loop runs on CPU ranks).  Pilot MC solves of the source PL b
  * walk `pilot_histories` consecutive histories from the middle of the full
  * source (0: 1/8 of the configured count, at least 2^20), so that every
- * pilot sees the solve's births per state.  They time the block size (64,
+ * pilot sees the solve's births per state; each candidate keeps the faster
+ * of two pilots.  They time the block size (64,
  * 128, 256) at 8, 16 and 32 resident waves per CU, then the history ids a
  * wave takes per queue grab (64 .. 512) at the fastest shape, then
  * (collision estimator) the LDS tally tile: off, 128, 512 or 1024 entries
- * per wave (those that fit 64 KiB of LDS).  The winner is stored in "Walk
+ * per wave (those that fit 64 KiB of LDS); last, the best shape of each
+ * block size walks the whole source once, and the fastest of those sets
+ * the block size.  `ms_best` is the pilot time of the chosen batch and
+ * tile.  The winner is stored in "Walk
  * Block Size", "Walk Blocks Per CU", "Walk Batch" and "Tally Tile" and
  * reported in `out`.  Estimates do not
  * depend on the launch shape (each history's random stream depends only on

@@ -2002,18 +2002,27 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   // births per state (what the LDS tile and the caches see) are the solve's
   const double window = nh > 0 ? (double)np / (double)nh : 1.0;
   c->nsys = 0;  // pilot births follow |PL b| over the whole operator
-  auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double* ms) -> int {
+  auto pilot_win = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double win, double* ms) -> int {
     c->prm.walk_block = blk;
     c->prm.walk_blocks_per_cu = bpc;
     c->prm.walk_batch = bat;
     c->prm.tally_tile = tile;
     MCLS_TRY(ensure_vectors(c));
-    MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true, nullptr, window));
-    HIP_TRY(hipStreamSynchronize(s));
-    float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, c->ev[3], c->ev[4]));
-    *ms = t;
+    // the faster of two pilots: shapes within a few percent of each other
+    // are otherwise ranked by timing noise
+    double fastest = 0.0;
+    for (int rep = 0; rep < 2; ++rep) {
+      MCLS_TRY(mc_solve_device(c, c->r.p, c->x.p, false, 0, true, nullptr, win));
+      HIP_TRY(hipStreamSynchronize(s));
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, c->ev[3], c->ev[4]));
+      if (rep == 0 || t < fastest) fastest = t;
+    }
+    *ms = fastest;
     return MCLS_OK;
+  };
+  auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double* ms) -> int {
+    return pilot_win(blk, bpc, bat, tile, window, ms);
   };
   int rc = MCLS_OK;
   mcls_gpu_tuning best{};
@@ -2031,11 +2040,17 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   // launch shape: 8, 16 or 32 resident waves per CU at each block size
   const int64_t blocks[3] = {64, 128, 256};
   const int64_t waves[3] = {8, 16, 32};
+  double block_ms[3] = {0.0, 0.0, 0.0};
+  int64_t block_bpc[3] = {0, 0, 0};  // the best resident count of each block size
   for (int i = 0; i < 3 && rc == MCLS_OK; ++i)
     for (int j = 0; j < 3 && rc == MCLS_OK; ++j) {
       const int64_t bpc = waves[j] * 64 / blocks[i];
       rc = pilot_ms(blocks[i], bpc, best.batch, best.tally_tile, &ms);
       ++best.candidates;
+      if (rc == MCLS_OK && (block_bpc[i] == 0 || ms < block_ms[i])) {
+        block_ms[i] = ms;
+        block_bpc[i] = bpc;
+      }
       if (rc == MCLS_OK && ms < best.ms_best) {
         best.ms_best = ms;
         best.block = blocks[i];
@@ -2071,6 +2086,31 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
         best.tally_tile = tiles[k];
       }
     }
+  }
+  // The block size, confirmed over the whole source: pilots over a window
+  // of it have ranked 256-thread blocks ahead of smaller ones that walk the
+  // whole source faster.  The best shape of each block size, with the chosen
+  // batch and tile, walks the full source; the fastest is kept.
+  if (rc == MCLS_OK) {
+    double full_best = -1.0;
+    int64_t fb = best.block, fbpc = best.blocks_per_cu;
+    for (int i = 0; i < 3 && rc == MCLS_OK; ++i) {
+      if (block_bpc[i] == 0) continue;
+      if (best.tally_tile > 0 && saved.estimator == mcls::EST_COLLISION && !saved.forward) {
+        const size_t elem = mcls::tile_entry_bytes(saved.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64,
+                                                   saved.precision == 32 ? 4 : 8);
+        if ((size_t)(blocks[i] / 64) * (size_t)best.tally_tile * elem > 65536) continue;
+      }
+      rc = pilot_win(blocks[i], block_bpc[i], best.batch, best.tally_tile, 1.0, &ms);
+      ++best.candidates;
+      if (rc == MCLS_OK && (full_best < 0.0 || ms < full_best)) {
+        full_best = ms;
+        fb = blocks[i];
+        fbpc = block_bpc[i];
+      }
+    }
+    best.block = fb;
+    best.blocks_per_cu = fbpc;
   }
   c->prm = saved;
   c->nsys = saved_nsys;

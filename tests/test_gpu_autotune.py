@@ -28,7 +28,8 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
     t = ctx.autotune(prob.b, pilot_histories=1 << 16)
     print("tuning", t)
     assert ctx.stats().solve_index == before
-    assert t["candidates"] == (16 if method == "Adjoint" else 13)  # no tally tile for forward walks
+    # shapes, batches, tiles (not for forward walks), then one full walk per block size
+    assert t["candidates"] == (16 if method == "Adjoint" else 13) + 3
     assert t["tally_tile"] in (0, 128, 512, 1024)
     assert t["block"] in (64, 128, 256) and t["batch"] in (64, 128, 256, 512)
     assert t["blocks_per_cu"] * t["block"] // 64 in (8, 16, 32)
