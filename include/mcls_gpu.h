@@ -170,6 +170,10 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        else a power of two in [64, 2048]; collision estimator): tallies
  *        merge in LDS and are flushed into an FP64 tally, also for FP32 walks
  *        (these four are what mcls_gpu_autotune sets)
+ *   i64 "Walks Per Lane"       (1) 2: each lane carries two histories and
+ *        issues both of their record loads before waiting on either
+ *        ("Alias Strided" FP32 walks, collision or forward estimator; other
+ *        walks and path recording use one per lane)
  *   str "Tally Tile Precision" ("FP64" | "Walk"): the tile's partial sums
  *        in FP64 (12 bytes of LDS per entry) or in the walk's precision
  *        (FP32 walks: 8 bytes, more resident waves per CU); either way

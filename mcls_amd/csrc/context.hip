@@ -94,6 +94,7 @@ struct Params {
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
   int tile_prec = 64;      // tile partial sums: 64, or 32 = the walk's precision
+  int64_t walks_per_lane = 1;  // 2: two histories per lane (strided alias walks)
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
   int sampler = mcls::SAMPLER_CDF;
@@ -836,6 +837,7 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   wc.sampler = c->prm.sampler;
   wc.record = record;
   wc.tile = !tile_on(c, record) ? mcls::kTileOff : c->prm.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64;
+  wc.walks_per_lane = (int)c->prm.walks_per_lane;
   wc.block = (int)c->prm.walk_block;
   int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
   int64_t cap = (int64_t)c->num_cus * per_cu;
@@ -1567,6 +1569,9 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile must be 0 or a power of two in [64, 2048]");
     p.tally_tile = v;
     if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
+  } else if (key_is(key, "Walks Per Lane")) {
+    if (v != 1 && v != 2) return fail(MCLS_ERR_INVALID_ARG, "Walks Per Lane must be 1 or 2");
+    p.walks_per_lane = v;
   } else if (key_is(key, "Walk Batch")) {
     if (v < 64 || v > 4096 || v % 64 != 0)
       return fail(MCLS_ERR_INVALID_ARG, "Walk Batch must be a multiple of 64 in [64, 4096]");

@@ -522,8 +522,241 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   if (my_steps) atomicAdd(a.steps, (unsigned long long)my_steps);
 }
 
+// Two walks per lane on the strided alias tables (FP32, collision or
+// forward estimator, no path recording): each lane carries two histories
+// and issues both of their record loads before it waits on either, doubling
+// the loads in flight per resident wave.  Births, transitions, tallies and
+// the LDS tile are those of k_walk; only the order of the tally additions
+// differs.
+template <int EST, int SRC, int TILE>
+__global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
+  using Real = float;
+  using Acc = typename std::conditional<TILE != kTileOff, double, Real>::type;
+  using TAcc = typename std::conditional<TILE == kTileWalk, Real, double>::type;
+  constexpr int J = 2;
+  const DomainView& d = a.d;
+  const int n = d.n;
+  Acc* __restrict__ tally = TILE != kTileOff ? (Acc*)a.tally64 : (Acc*)Tables<Real>::tally(a);
+
+  int64_t hi_id = a.last;
+  uint64_t total = 0;
+  if (EST == EST_FORWARD) {
+  } else if (SRC == SRC_RANDOM) {
+    total = a.qcdf[n - 1];
+    if (total == 0) return;
+  } else {
+    total = a.ncdf[n - 1];
+    if ((int64_t)total < hi_id) hi_id = (int64_t)total;
+  }
+  const int64_t span = hi_id - a.first;
+  if (span <= 0) return;
+  const double totald = (double)total;
+  const uint64_t key = a.key_dev ? *a.key_dev : a.key;
+  const int qs = blockIdx.x % kQueueSlots;
+  const int64_t sbeg = a.first + span * qs / kQueueSlots;
+  const int64_t send = a.first + span * (qs + 1) / kQueueSlots;
+  unsigned long long* ctr = a.queue + qs * kQueueStride;
+  const int lane = __lane_id();
+  const uint32_t stride = d.stride;
+
+  int* tkeys = nullptr;
+  TAcc* tvals = nullptr;
+  if constexpr (TILE != kTileOff) {
+    extern __shared__ __align__(16) unsigned char walk_lds[];
+    const int wv = threadIdx.x >> 6;
+    tkeys = (int*)walk_lds + wv * a.tile_slots;
+    tvals = (TAcc*)(walk_lds + (size_t)(blockDim.x >> 6) * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
+    for (int k = lane; k < a.tile_slots; k += 64) {
+      tkeys[k] = -1;
+      tvals[k] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  auto flush_tile = [&]() {
+    if constexpr (TILE != kTileOff) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int k = lane; k < a.tile_slots; k += 64) {
+        const int tk = tkeys[k];
+        if (tk >= 0) {
+          tally_add(&tally[tk], (Acc)tvals[k]);
+          tkeys[k] = -1;
+          tvals[k] = 0;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  };
+  auto tally_at = [&](int st, Real wt) {
+    if constexpr (TILE != kTileOff) {
+      const uint32_t s0 = (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
+      int keys[kTileWays];
+      if constexpr (kTileWays == 4) {
+        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
+        keys[0] = (int)kk.x; keys[1] = (int)kk.y; keys[2] = (int)kk.z; keys[3] = (int)kk.w;
+      } else {
+        const uint2 kk = *reinterpret_cast<const uint2*>(tkeys + s0);
+        keys[0] = (int)kk.x; keys[1] = (int)kk.y;
+      }
+      int slot = -1;
+#pragma unroll
+      for (int q = 0; q < kTileWays; ++q)
+        if (slot < 0 && keys[q] == st) slot = (int)s0 + q;
+#pragma unroll
+      for (int q = 0; q < kTileWays; ++q) {
+        if (slot < 0 && keys[q] < 0) {
+          const int k = atomicCAS(&tkeys[s0 + q], -1, st);
+          if (k < 0 || k == st) slot = (int)s0 + q;
+        }
+      }
+      if (slot >= 0)
+        __hip_atomic_fetch_add(&tvals[slot], (TAcc)wt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else
+        tally_add(&tally[st], (Acc)wt);
+    } else {
+      tally_add(&tally[st], wt);
+    }
+  };
+
+  int64_t qnext = 0, qend = 0;
+  int slo = 0, shi = 0;
+  bool exhausted = false;
+  bool alive[J] = {false, false};
+  int state[J] = {0, 0};
+  Real w[J] = {0, 0}, wc[J] = {0, 0};
+  int start[J] = {0, 0};
+  double fsum[J] = {0.0, 0.0};
+  int64_t steps[J] = {0, 0};
+  uint64_t rng[J] = {0, 0};
+  uint64_t my_steps = 0;
+
+  for (;;) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (exhausted) break;
+      uint64_t need = __ballot(!alive[j]);
+      while (need != 0) {
+        if (qnext >= qend) {
+          flush_tile();
+          const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+          unsigned long long b = 0;
+          if (lane == leader) b = atomicAdd(ctr, (unsigned long long)a.batch);
+          b = __shfl(b, leader);
+          qnext = sbeg + (int64_t)b;
+          qend = qnext + a.batch < send ? qnext + a.batch : send;
+          if (qnext >= send) {
+            exhausted = true;
+            break;
+          }
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD) {
+            int lo = 0, hi = 0;
+            if (lane == leader) {
+              lo = upper_bound_u64(a.ncdf, n, (uint64_t)qnext);
+              hi = upper_bound_u64(a.ncdf, n, (uint64_t)(qend - 1));
+            }
+            slo = __shfl(lo, leader);
+            shi = __shfl(hi, leader);
+          }
+        }
+        const int64_t avail = qend - qnext;
+        const int cnt = __popcll(need);
+        const int take = (int64_t)cnt < avail ? cnt : (int)avail;
+        const int rank = __popcll(need & __lanemask_lt());
+        if (!alive[j] && rank < take) {
+          const int64_t hid = qnext + rank;
+          rng[j] = history_state(key, (uint64_t)hid);
+          double w0;
+          if (EST == EST_FORWARD) {
+            state[j] = (int)(hid / a.per_state);
+            start[j] = state[j];
+            fsum[j] = 0.0;
+            w0 = 1.0;
+          } else if (SRC == SRC_RANDOM) {
+            const uint64_t t = mul_u53(xorshift64(rng[j]), total);
+            state[j] = upper_bound_u64(a.qcdf, n, t);
+            w0 = a.rhs[state[j]] * (totald / (double)a.q[state[j]]);
+          } else {
+            state[j] = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
+            const uint64_t nk = a.ncdf[state[j]] - (state[j] ? a.ncdf[state[j] - 1] : 0);
+            w0 = a.rhs[state[j]] * (totald / (double)nk);
+          }
+          w[j] = (Real)w0;
+          wc[j] = (Real)a.cutoff * (w[j] < 0 ? -w[j] : w[j]);
+          steps[j] = 0;
+          alive[j] = true;
+        }
+        qnext += take;
+        need = __ballot(!alive[j]);
+      }
+    }
+    if (!alive[0] && !alive[1]) break;  // queue exhausted and both walks done
+
+    // both records in flight before either is used
+    uint64_t r[J] = {0, 0};
+    uint4 sr[J] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (alive[j]) {
+        r[j] = xorshift64(rng[j]);
+        const uint32_t slot = __umulhi((uint32_t)(r[j] >> 32), stride);
+        sr[j] = d.srec[(size_t)state[j] * stride + slot];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (!alive[j]) continue;
+      if (EST == EST_FORWARD)
+        fsum[j] = fsum[j] + (double)w[j] * a.rhs[state[j]];
+      else
+        tally_at(state[j], w[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      if (!alive[j]) continue;
+      bool end;
+      if (sr[j].y == kStridedAbsorbing) {
+        end = true;
+      } else {
+        const float u2 = (float)((uint32_t)r[j] >> 8) * 0x1.0p-24f;
+        const uint32_t cp = u2 < __uint_as_float(sr[j].x) ? sr[j].y : sr[j].z;
+        w[j] = w[j] * __uint_as_float(sr[j].w);
+        if (cp & 0x80000000u) w[j] = -w[j];
+        state[j] = (int)(cp & 0x7FFFFFFFu);
+        ++steps[j];
+        ++my_steps;
+        end = steps[j] >= a.history_length || (w[j] < 0 ? -w[j] : w[j]) < wc[j];
+      }
+      if (end) {
+        alive[j] = false;
+        if (EST == EST_FORWARD) tally_add(&tally[start[j]], (Acc)fsum[j]);
+      }
+    }
+  }
+  flush_tile();
+  if (my_steps) atomicAdd(a.steps, (unsigned long long)my_steps);
+}
+
 template <typename Real, int EST, int SRC, int SAMP>
 hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) {
+  if constexpr (SAMP == SAMPLER_ALIAS_STRIDED && sizeof(Real) == 4 && EST != EST_EXPECTED_VALUE) {
+    if (c.walks_per_lane == 2 && !c.record) {
+      const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * tile_entry_bytes(c.tile, sizeof(Real));
+      if constexpr (EST == EST_COLLISION) {
+        if (c.tile == kTileWalk) {
+          k_walk2<EST, SRC, kTileWalk><<<c.grid, c.block, lds, s>>>(a);
+          return hipGetLastError();
+        }
+        if (c.tile != kTileOff) {
+          k_walk2<EST, SRC, kTileF64><<<c.grid, c.block, lds, s>>>(a);
+          return hipGetLastError();
+        }
+      }
+      k_walk2<EST, SRC, kTileOff><<<c.grid, c.block, 0, s>>>(a);
+      return hipGetLastError();
+    }
+  }
   if (c.record) {
     k_walk<Real, EST, SRC, SAMP, true, kTileOff><<<c.grid, c.block, 0, s>>>(a);
   } else if constexpr (EST == EST_COLLISION) {

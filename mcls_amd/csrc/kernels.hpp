@@ -133,6 +133,7 @@ struct WalkConfig {
   int sampler;     // Sampler
   bool record;
   int tile;        // TileKind (collision estimator; tallies go to tally64)
+  int walks_per_lane;  // 2: two histories per lane (strided alias, FP32, no recording)
   int block;       // threads per block
   int grid;        // blocks (rounded up to a multiple of kQueueSlots)
 };

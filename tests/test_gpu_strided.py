@@ -71,9 +71,10 @@ def test_paths_bitexact(kind, source, forward):
     assert (gp[:, 1] >= 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("walks", [1, 2])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
 @pytest.mark.parametrize("tile", [0, 256])
-def test_tally_matches_oracle(source, tile):
+def test_tally_matches_oracle(source, tile, walks):
     prob, A, dt = _problem("s27")
     nh, L = 40000, 100
     dinv = po.jacobi_dinv(A)
@@ -84,7 +85,7 @@ def test_tally_matches_oracle(source, tile):
         ctx.set_point_jacobi()
         ctx.set_params(dict(BASE, **{"Source Sampling Type": source, "Number of Histories": nh, "History Length": L,
                                      "Weight Cutoff": 1e-2, "Random Seed": 5, "Tally Tile": tile,
-                                     "Tally Tile Precision": "Walk"}))
+                                     "Tally Tile Precision": "Walk", "Walks Per Lane": walks}))
         xg = ctx.mc_solve_raw(rhs, 3)
         st = ctx.stats()
     mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION,
@@ -95,8 +96,34 @@ def test_tally_matches_oracle(source, tile):
     assert np.linalg.norm(xg - xo) <= 2e-5 * np.linalg.norm(xo)
 
 
+def test_forward_two_walks_per_lane_matches_oracle():
+    """Two histories per lane on the forward method: the same per-state sums
+    as the oracle (and as one history per lane) to the FP32 summation order."""
+    prob, A, dt = _problem("s27")
+    nh, L = 30000, 100
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv, forward=True)
+    rhs = prob.b * dinv
+    xs = []
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        for walks in (1, 2):
+            ctx.set_params(dict(BASE, **{"Monte Carlo Type": "Forward", "Number of Histories": nh,
+                                         "History Length": L, "Weight Cutoff": 1e-2, "Random Seed": 6,
+                                         "Walks Per Lane": walks}))
+            xs.append(ctx.mc_solve_raw(rhs, 2))
+            st = ctx.stats()
+    mp = po.mc_params(nh, L, 1e-2, 32, po.EST_FORWARD, po.SRC_RANDOM, seed=6, sampler=po.SAMPLER_ALIAS_STRIDED)
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=2)
+    assert st.steps == steps
+    for x in xs:
+        assert np.linalg.norm(x - xo) <= 2e-5 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("walks", [1, 2])
 @pytest.mark.parametrize("ordered", [False, True])
-def test_mcsa_converges(ordered):
+def test_mcsa_converges(ordered, walks):
     prob, A, dt = _problem("s27")
     with capi.Context(0) as ctx:
         if ordered:
@@ -106,7 +133,8 @@ def test_mcsa_converges(ordered):
         ctx.set_params(dict(BASE, **{"Source Sampling Type": "Stratified", "Number of Histories": 16 * prob.n,
                                      "Weight Cutoff": 1e-1, "History Length": 1000, "Random Seed": 9,
                                      "Convergence Tolerance": 1e-8, "Maximum Iterations": 60,
-                                     "Tally Tile": 512, "Tally Tile Precision": "Walk"}))
+                                     "Tally Tile": 512, "Tally Tile Precision": "Walk",
+                                     "Walks Per Lane": walks}))
         x = ctx.mcsa_solve(prob.b)
         st = ctx.stats()
     assert st.converged == 1 and st.achieved_tol <= 1e-8
@@ -130,6 +158,8 @@ def test_errors():
         assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
         ctx.set_params({"Estimator Type": "Collision"})
         assert np.all(np.isfinite(ctx.mc_solve(prob.b)))
+        with pytest.raises(capi.GpuError):
+            ctx.set_params({"Walks Per Lane": 3})
     # a domain row of more than 64 entries
     n = 80
     S = sp.lil_matrix((n, n))
