@@ -284,12 +284,17 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if constexpr (TILE != kTileOff) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      for (int k = lane; k < a.tile_slots; k += 64) {
-        const int key = tkeys[k];
-        if (key >= 0) {
-          tally_add(&tally[key], (Acc)tvals[k]);
-          tkeys[k] = -1;
-          tvals[k] = 0;
+      // four slots (one set) per lane and pass: the keys in one 16-byte read
+      for (int k = lane * 4; k < a.tile_slots; k += 256) {
+        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
+        const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (keys[q] >= 0) {
+            tally_add(&tally[keys[q]], (Acc)tvals[k + q]);
+            tkeys[k + q] = -1;
+            tvals[k + q] = 0;
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -577,12 +582,17 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
     if constexpr (TILE != kTileOff) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      for (int k = lane; k < a.tile_slots; k += 64) {
-        const int tk = tkeys[k];
-        if (tk >= 0) {
-          tally_add(&tally[tk], (Acc)tvals[k]);
-          tkeys[k] = -1;
-          tvals[k] = 0;
+      // four slots (one set) per lane and pass: the keys in one 16-byte read
+      for (int k = lane * 4; k < a.tile_slots; k += 256) {
+        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
+        const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (keys[q] >= 0) {
+            tally_add(&tally[keys[q]], (Acc)tvals[k + q]);
+            tkeys[k + q] = -1;
+            tvals[k + q] = 0;
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
