@@ -233,6 +233,13 @@ __device__ __forceinline__ uint32_t cdf_step(const DomainView& d, uint32_t& beg,
 constexpr int kTileWaysLog2 = 2;
 constexpr int kTileWays = 1 << kTileWaysLog2;
 
+// The partial sums of one set, aligned so that a flush reads and clears them
+// with single LDS vector accesses (sets start at multiples of 4 entries).
+template <typename T>
+struct alignas(4 * sizeof(T)) TileSet {
+  T s[4];
+};
+
 template <typename Real, int EST, int SRC, int SAMP, bool REC, int TILE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   using Acc = typename std::conditional<TILE != kTileOff, double, Real>::type;
@@ -284,17 +291,19 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     if constexpr (TILE != kTileOff) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      // four slots (one set) per lane and pass: the keys in one 16-byte read
+      // four slots (one set) per lane and pass: keys and sums in vector
+      // reads, the set cleared by vector writes when any slot was held
       for (int k = lane * 4; k < a.tile_slots; k += 256) {
         const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
         const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
+        // the AND of the keys is negative only when all four slots are empty
+        if ((keys[0] & keys[1] & keys[2] & keys[3]) >= 0) {
+          const TileSet<TAcc> v = *reinterpret_cast<const TileSet<TAcc>*>(tvals + k);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (keys[q] >= 0) {
-            tally_add(&tally[keys[q]], (Acc)tvals[k + q]);
-            tkeys[k + q] = -1;
-            tvals[k + q] = 0;
-          }
+          for (int q = 0; q < 4; ++q)
+            if (keys[q] >= 0) tally_add(&tally[keys[q]], (Acc)v.s[q]);
+          *reinterpret_cast<uint4*>(tkeys + k) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+          *reinterpret_cast<TileSet<TAcc>*>(tvals + k) = TileSet<TAcc>{};
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -582,17 +591,19 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
     if constexpr (TILE != kTileOff) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      // four slots (one set) per lane and pass: the keys in one 16-byte read
+      // four slots (one set) per lane and pass: keys and sums in vector
+      // reads, the set cleared by vector writes when any slot was held
       for (int k = lane * 4; k < a.tile_slots; k += 256) {
         const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
         const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
+        // the AND of the keys is negative only when all four slots are empty
+        if ((keys[0] & keys[1] & keys[2] & keys[3]) >= 0) {
+          const TileSet<TAcc> v = *reinterpret_cast<const TileSet<TAcc>*>(tvals + k);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (keys[q] >= 0) {
-            tally_add(&tally[keys[q]], (Acc)tvals[k + q]);
-            tkeys[k + q] = -1;
-            tvals[k + q] = 0;
-          }
+          for (int q = 0; q < 4; ++q)
+            if (keys[q] >= 0) tally_add(&tally[keys[q]], (Acc)v.s[q]);
+          *reinterpret_cast<uint4*>(tkeys + k) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+          *reinterpret_cast<TileSet<TAcc>*>(tvals + k) = TileSet<TAcc>{};
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
