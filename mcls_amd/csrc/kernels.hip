@@ -430,7 +430,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         // keys are read together; empty ones are claimed by compare-and-swap
         // in slot order (lanes of one wave with the same state agree on the
         // slot, so a state never holds two).  Full sets send it to HBM.
-        const uint32_t s0 = (((uint32_t)state * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
+        const uint32_t s0 = a.tile_direct
+                                  ? ((uint32_t)state & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
+                                  : (((uint32_t)state * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
         int keys[kTileWays];
         if constexpr (kTileWays == 4) {
           const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
@@ -612,7 +614,9 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
   };
   auto tally_at = [&](int st, Real wt) {
     if constexpr (TILE != kTileOff) {
-      const uint32_t s0 = (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
+      const uint32_t s0 = a.tile_direct
+                                  ? ((uint32_t)st & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
+                                  : (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
       int keys[kTileWays];
       if constexpr (kTileWays == 4) {
         const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);

@@ -116,6 +116,8 @@ struct WalkArgs {
   int64_t per_state;       // forward method: histories per start state
   int tile_slots;          // LDS tally tile entries per wave (power of two), 0 = none
   int tile_shift;          // 32 - log2(tile_slots)
+  int tile_direct;         // 1: tile set = state mod sets (locality-ordered
+                           // states); 0: Fibonacci hash of the state
 };
 
 // LDS tally tile: off, FP64 partial sums, or partial sums in the walk's
