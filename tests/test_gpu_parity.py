@@ -370,7 +370,11 @@ def test_forward_rejects_expected_value(ctx):
 
 def test_device_pointer_solve_matches_host():
     """b and x in device memory (ptr_kind MCLS_DEVICE) give the host-pointer
-    solve's result: same streams, only the tally summation order differs."""
+    solve's result: same streams, only the tally summation order differs.
+    That order changes the residual in its last bits, and the next source's
+    integer quantisation of |r| can then move a birth, so the two solves'
+    MC noise parts ways after a few iterations: the iteration counts agree
+    to one and both solutions meet the tolerance."""
     prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 24, 24, seed=0))
     params = {"History Length": 5000, "Weight Cutoff": 1e-4, "Number of Histories": 20000,
               "Convergence Tolerance": 1e-8, "Maximum Iterations": 200, "Random Seed": 3}
@@ -383,11 +387,12 @@ def test_device_pointer_solve_matches_host():
                 b.upload(prob.b)
                 x.zero()
                 c.mcsa_solve_device(b.ptr.value, x.ptr.value)
-                out.append((x.download(), c.stats().iterations))
+                out.append((x.download(), c.stats().iterations, c.stats().converged))
                 b.close()
                 x.close()
             else:
-                out.append((c.mcsa_solve(prob.b), c.stats().iterations))
-    (xh, ih), (xd, idv) = out
-    assert ih == idv
-    assert np.linalg.norm(xh - xd) <= 1e-10 * np.linalg.norm(xh)
+                out.append((c.mcsa_solve(prob.b), c.stats().iterations, c.stats().converged))
+    (xh, ih, ch), (xd, idv, cd) = out
+    assert ch == cd == 1
+    assert abs(ih - idv) <= 1
+    assert np.linalg.norm(xh - xd) <= 1e-5 * np.linalg.norm(xh)
