@@ -207,6 +207,25 @@ int mcls_gpu_mc_solve(mcls_gpu_context* ctx, const double* b, double* x,
  * tolerance was met; read converged from mcls_gpu_get_stats. */
 int mcls_gpu_mcsa_solve(mcls_gpu_context* ctx, const double* b, double* x,
                         int ptr_kind);
+/* mcls_gpu_mc_solve that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  b and x are device pointers (n doubles, caller's numbering).
+ * Every per-call buffer (source tables, tally, work-queue and step
+ * counters, scan partials; with a state order b and x in the internal
+ * numbering) lives in a static device array of 65 MiB that each GPU the
+ * library runs on holds until the process ends, so n is at most
+ * MCLS_NOALLOC_MAX_N (MCLS_ERR_INVALID_ARG beyond).  The domain and the
+ * Transition Sampler's tables must exist for the current operator and
+ * parameters (mcls_gpu_build_domain builds both), else MCLS_ERR_STATE;
+ * batched systems, sparse preconditioners and an attached importance
+ * network are refused (MCLS_ERR_UNSUPPORTED).  The result equals
+ * mcls_gpu_mc_solve's for the same solve number.  All calls on one GPU
+ * share its array: calls that overlap on one GPU (two contexts' streams,
+ * two threads) race; calls that follow each other do not, each clears
+ * what it reads as zero. */
+#define MCLS_NOALLOC_MAX_N 1048576
+int mcls_gpu_mc_solve_noalloc(mcls_gpu_context* ctx, const double* b, double* x);
 /* LinearProblem::updatePrecResidual: r = PL (b - A x) and ||r||_2. */
 int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            const double* x, double* r, double* norm2,

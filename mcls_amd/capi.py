@@ -41,8 +41,9 @@ GPU_SYMBOLS = [
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
     "mcls_gpu_set_left_prec_matrix", "mcls_gpu_set_right_prec_matrix", "mcls_gpu_clear_right_prec",
-    "mcls_gpu_set_state_order",
+    "mcls_gpu_set_state_order", "mcls_gpu_mc_solve_noalloc",
 ]
+MCLS_NOALLOC_MAX_N = 1048576
 MLP_SYMBOLS = [
     "mcls_mlp_create", "mcls_mlp_destroy", "mcls_mlp_get_params", "mcls_mlp_set_params",
     "mcls_mlp_forward", "mcls_mlp_loss_grad", "mcls_mlp_dataset_clear", "mcls_mlp_dataset_append",
@@ -167,6 +168,7 @@ def gpu_lib():
     L.mcls_gpu_get_domain.argtypes = [vp, vp, vp, vp, vp, vp]
     L.mcls_gpu_mc_solve.argtypes = [vp, vp, vp, i32]
     L.mcls_gpu_mcsa_solve.argtypes = [vp, vp, vp, i32]
+    L.mcls_gpu_mc_solve_noalloc.argtypes = [vp, vp, vp]
     L.mcls_gpu_prec_residual.argtypes = [vp, vp, vp, vp, C.POINTER(f64), i32]
     L.mcls_gpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.mcls_gpu_record_paths.argtypes = [vp, vp, u64, i64, i64, C.c_int32, vp, vp]
@@ -515,6 +517,10 @@ class Context:
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
         self._check(self.L.mcls_gpu_mcsa_solve(self.h, _p(b), _p(x), MCLS_HOST))
         return x
+
+    def mc_solve_noalloc(self, b_ptr, x_ptr):
+        """mcls_gpu_mc_solve_noalloc: b, x are device pointers (n doubles)."""
+        self._check(self.L.mcls_gpu_mc_solve_noalloc(self.h, b_ptr, x_ptr))
 
     def mcsa_solve_device(self, b_ptr, x_ptr):
         """MCSA with b and x (initial guess in, solution out) already in

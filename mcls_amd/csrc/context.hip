@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -181,6 +182,7 @@ struct mcls_gpu_context {
 
   uint64_t solve_index = 0;
   mcls_gpu_stats stats{};
+  void* scratch = nullptr;  // this device's `scratch` (allocation-free solve)
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -340,6 +342,20 @@ double* tally64_of(mcls_gpu_context* c, bool record) {
 }
 float* tally32_of(mcls_gpu_context* c, bool record) {
   return (c->prm.precision == 32 && !tile_on(c, record)) ? c->t32.p : nullptr;
+}
+
+// The per-solve buffers of an MC solve: the context's own, or those the
+// allocation-free entry point carves out of the static scratch array.
+struct SolveBufs {
+  uint64_t *q, *qcdf, *ncnt, *ncdf, *scan_tmp;
+  unsigned long long *bmax, *steps, *queue;
+  double* t64;  // FP64 tally (FP64 walks, tiled FP32 walks)
+  float* t32;   // FP32 tally (untiled FP32 walks)
+};
+
+SolveBufs ctx_bufs(mcls_gpu_context* c) {
+  return SolveBufs{c->q.p,    c->qcdf.p,  c->ncnt.p,  c->ncdf.p, c->scan_tmp.p,
+                   c->bmax.p, c->steps.p, c->queue.p, c->t64.p,  c->t32.p};
 }
 
 void invalidate_graph(mcls_gpu_context* c) {
@@ -849,7 +865,8 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   return wc;
 }
 
-WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t first, int64_t last) {
+WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, uint64_t key, int64_t first,
+                   int64_t last) {
   WalkArgs a{};
   a.d.n = c->n;
   a.d.rp = c->d_rp.p;
@@ -873,18 +890,18 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   a.d.lfac64 = c->d_lfac64.p;
   a.d.lfac32 = c->d_lfac32.p;
   a.d.lrec32 = c->d_lrec32.p;
-  a.queue = c->queue.p;
+  a.queue = sb.queue;
   a.rhs = rhs;
-  a.q = c->q.p;
-  a.qcdf = c->qcdf.p;
-  a.ncdf = c->ncdf.p;
+  a.q = sb.q;
+  a.qcdf = sb.qcdf;
+  a.ncdf = sb.ncdf;
   a.key = key;
   a.first = first;
   a.last = last;
   a.history_length = c->prm.history_length;
   a.cutoff = c->prm.weight_cutoff;
-  a.tally64 = c->t64.p;
-  a.tally32 = c->t32.p;
+  a.tally64 = sb.t64;
+  a.tally32 = sb.t32;
   a.tile_slots = (int)c->prm.tally_tile;
   a.tile_shift = 32;
   for (int64_t t = c->prm.tally_tile; t > 1; t >>= 1) --a.tile_shift;
@@ -892,7 +909,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
   // consecutive states in distinct sets keep a walk cluster's states apart
   // (2.3% faster than the hash on the 256^3 27-point chosen solve)
   a.tile_direct = c->ordered ? 1 : 0;
-  a.steps = c->steps.p;
+  a.steps = sb.steps;
   a.batch = (int)c->prm.walk_batch;
   a.per_state = forward_per_state(c);
   return a;
@@ -900,19 +917,23 @@ WalkArgs walk_args(mcls_gpu_context* c, const double* rhs, uint64_t key, int64_t
 
 // Source tables for rhs (device): q, qcdf (and stratified counts).  Births
 // are allocated by |mag| (mag = rhs unless a batch reallocates them).
-int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const double* mag = nullptr) {
+int build_source(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, int64_t nh,
+                 const double* mag = nullptr) {
   hipStream_t s = c->stream;
   const int n = c->n;
   if (!mag) mag = rhs;
-  HIP_TRY(mcls::launch_fill_u64(c->bmax.p, 1, 0ULL, s));
-  HIP_TRY(mcls::launch_absmax(mag, n, c->bmax.p, s));
-  HIP_TRY(mcls::launch_quantize(mag, n, c->bmax.p, c->q.p, s));
-  HIP_TRY(mcls::launch_scan_u64(c->q.p, c->qcdf.p, n, c->scan_tmp.p, s));
+  HIP_TRY(mcls::launch_fill_u64(sb.bmax, 1, 0ULL, s));
+  HIP_TRY(mcls::launch_absmax(mag, n, sb.bmax, s));
+  HIP_TRY(mcls::launch_quantize(mag, n, sb.bmax, sb.q, s));
+  HIP_TRY(mcls::launch_scan_u64(sb.q, sb.qcdf, n, sb.scan_tmp, s));
   if (c->prm.source == mcls::SRC_STRATIFIED) {
-    HIP_TRY(mcls::launch_strat_counts(c->q.p, n, c->qcdf.p + (n - 1), (uint64_t)nh, c->ncnt.p, s));
-    HIP_TRY(mcls::launch_scan_u64(c->ncnt.p, c->ncdf.p, n, c->scan_tmp.p, s));
+    HIP_TRY(mcls::launch_strat_counts(sb.q, n, sb.qcdf + (n - 1), (uint64_t)nh, sb.ncnt, s));
+    HIP_TRY(mcls::launch_scan_u64(sb.ncnt, sb.ncdf, n, sb.scan_tmp, s));
   }
   return MCLS_OK;
+}
+int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const double* mag = nullptr) {
+  return build_source(c, ctx_bufs(c), rhs, nh, mag);
 }
 
 // One MC solve of source rhs (device) into the tally; then x (+)= tally/nh.
@@ -922,8 +943,8 @@ int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const doubl
 // window < 1: walk only that fraction of the source's histories, a run of
 // consecutive history ids from the middle of the queue (the autotuner's
 // pilots: same births per state as the full solve, fewer states).
-int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx,
-                    bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0) {
+int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, double* x, bool accumulate,
+                    uint64_t sidx, bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0) {
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
@@ -935,7 +956,7 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   const bool forward = c->prm.forward != 0;
   if (forward && c->prm.estimator != mcls::EST_COLLISION)
     return fail(MCLS_ERR_UNSUPPORTED, "the forward method tallies collisions only (Estimator Type 'Collision')");
-  if (!forward) MCLS_TRY(build_source(c, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
+  if (!forward) MCLS_TRY(build_source(c, sb, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
   if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
   const bool strat = !forward && c->prm.source == mcls::SRC_STRATIFIED;
   const int64_t last = strat ? nh + c->n : nh;
@@ -945,7 +966,7 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
     first = (last - span) / 2;
     end = first + span;
   }
-  WalkArgs a = walk_args(c, rhs, solve_key(c->prm.seed, sidx), first, end);
+  WalkArgs a = walk_args(c, sb, rhs, solve_key(c->prm.seed, sidx), first, end);
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, end - first);
   if (wc.tile != mcls::kTileOff) {
@@ -958,15 +979,21 @@ int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accu
   const bool ev = !forward && c->prm.estimator == mcls::EST_EXPECTED_VALUE;
   // forward: each state's sum over its own histories (ForwardTally::finalize)
   double inv = nh > 0 ? 1.0 / (double)(forward ? forward_per_state(c) : nh) : 0.0;
-  const uint64_t* mtot = strat ? c->ncdf.p + (c->n - 1) : nullptr;
-  double* t64 = tally64_of(c, false);
-  float* t32 = tally32_of(c, false);
+  const uint64_t* mtot = strat ? sb.ncdf + (c->n - 1) : nullptr;
+  const bool fp64_tally = c->prm.precision == 64 || tile_on(c, false);
+  double* t64 = fp64_tally ? sb.t64 : nullptr;
+  float* t32 = fp64_tally ? nullptr : sb.t32;
   if (accumulate)
     HIP_TRY(mcls::launch_apply_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
   else
     HIP_TRY(mcls::launch_store_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
   c->stats.walk_launches += nh > 0 ? 1 : 0;
   return MCLS_OK;
+}
+
+int mc_solve_device(mcls_gpu_context* c, const double* rhs, double* x, bool accumulate, uint64_t sidx,
+                    bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0) {
+  return mc_solve_device(c, ctx_bufs(c), rhs, x, accumulate, sidx, timed, key_dev, window);
 }
 
 int accumulate_mc_timing(mcls_gpu_context* c) {
@@ -979,22 +1006,23 @@ int accumulate_mc_timing(mcls_gpu_context* c) {
   return MCLS_OK;
 }
 
-int histories_done(mcls_gpu_context* c, int64_t* out) {
+int histories_done(mcls_gpu_context* c, const SolveBufs& sb, int64_t* out) {
   if (c->prm.forward) {
     *out = num_histories(c);
     return MCLS_OK;
   }
   if (c->prm.source == mcls::SRC_STRATIFIED) {
     uint64_t M = 0;
-    HIP_TRY(hipMemcpy(&M, c->ncdf.p + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&M, sb.ncdf + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
     *out = (int64_t)M;
   } else {
     uint64_t Q = 0;
-    HIP_TRY(hipMemcpy(&Q, c->qcdf.p + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&Q, sb.qcdf + (c->n - 1), sizeof(uint64_t), hipMemcpyDeviceToHost));
     *out = Q ? num_histories(c) : 0;
   }
   return MCLS_OK;
 }
+int histories_done(mcls_gpu_context* c, int64_t* out) { return histories_done(c, ctx_bufs(c), out); }
 
 // r = PL (b - A PR x) (src/MCLS_LinearProblem_impl.hpp:411-442) with a
 // diagonal or sparse PL and an optional sparse PR; partial sums of r^2 go to
@@ -1027,10 +1055,7 @@ int apply_left_prec(mcls_gpu_context* c, const double* v, double* y) {
   if (c->pl_csr.on) {
     HIP_TRY(mcls::launch_csr_spmv(c->n, c->pl_csr.rp.p, c->pl_csr.col.p, c->pl_csr.val.p, v, y, s));
   } else if (c->has_prec) {
-    // dinv_i v_i, as PL (v - A 0) with x = 0
-    HIP_TRY(mcls::launch_fill_f64(c->pt1.p, c->n, 0.0, s));
-    HIP_TRY(mcls::launch_prec_residual(c->n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->dinv.p, v,
-                                       c->pt1.p, y, c->part.p, s));
+    HIP_TRY(mcls::launch_diag_scale(c->n, c->dinv.p, v, y, s));
   } else {
     HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, v, y, s));
   }
@@ -1189,6 +1214,63 @@ int convergence_criteria(mcls_gpu_context* c, int64_t max_iters, double tol, mcl
 
 }  // namespace
 
+// ------------------------------------------ scratch of the allocation-free solve
+// mcls_gpu_mc_solve_noalloc keeps every per-call buffer of an MC solve (the
+// source tables, the tally, the work-queue and step counters, the scan
+// partials and, with a state order, b and x in the internal numbering) in
+// this array: 1 MiB for the counters and scan partials, then 64 bytes per
+// unknown for at most MCLS_NOALLOC_MAX_N unknowns (65 MiB in all).  Each GPU
+// the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in: its device address on a GPU is
+// the one hipGetSymbolAddress returns there, kept per device below.
+constexpr size_t kNoAllocFixedBytes = size_t(1) << 20;
+constexpr size_t kNoAllocPerState = 64;
+constexpr size_t kNoAllocBytes = kNoAllocFixedBytes + kNoAllocPerState * (size_t)MCLS_NOALLOC_MAX_N;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_dev;  // device address of `scratch`, per device
+
+// Once per device, with that device current (mcls_gpu_create).
+int resolve_scratch(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_dev[(size_t)device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MCLS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[(size_t)device] = p;
+  }
+  *out = g_scratch_dev[(size_t)device];
+  return MCLS_OK;
+}
+
+// The domain and the configured sampler's tables exist for the current
+// operator and parameters (what ensure_domain would otherwise build).
+bool tables_ready(const mcls_gpu_context* c) {
+  if (!c->domain_valid || c->domain_omega != c->prm.neumann_omega || c->domain_threshold != c->prm.threshold ||
+      c->domain_forward != c->prm.forward)
+    return false;
+  switch (c->prm.sampler) {
+    case mcls::SAMPLER_ALIAS:
+      return c->alias_valid;
+    case mcls::SAMPLER_ALIAS_STRIDED:
+      return c->strided_valid;
+    case mcls::SAMPLER_LEARNED_ALIAS:
+      return c->proposal_valid && c->proposal_kind == c->prm.sampler;
+    case mcls::SAMPLER_LEARNED:
+      return c->proposal_valid && c->proposal_kind == c->prm.sampler &&
+             (c->prm.precision == 32 ? c->tgt32_valid : c->tgt64_valid);
+    default:
+      return c->tgt64_valid;
+  }
+}
+
+}  // namespace
+
 // ================================================================ C ABI
 extern "C" {
 
@@ -1203,8 +1285,11 @@ int mcls_gpu_create(mcls_gpu_context** out, int device, unsigned flags) {
   if (e != hipSuccess || count == 0) return fail(MCLS_ERR_NO_DEVICE, "no HIP device available");
   if (device < 0 || device >= count) return fail(MCLS_ERR_NO_DEVICE, "device index out of range");
   HIP_TRY(hipSetDevice(device));
+  void* scr = nullptr;
+  MCLS_TRY(resolve_scratch(device, &scr));
   auto* c = new mcls_gpu_context();
   c->device = device;
+  c->scratch = scr;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
   hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1664,7 +1749,14 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
 
 int mcls_gpu_build_domain(mcls_gpu_context* c) {
   MCLS_TRY(check_ctx(c));
-  return build_domain(c);
+  MCLS_TRY(build_domain(c));
+  // the transition tables of the configured sampler, so that a solve that
+  // follows builds nothing (the learned proposal once it has an importance)
+  MCLS_TRY(ensure_alias(c));
+  MCLS_TRY(ensure_strided(c));
+  MCLS_TRY(ensure_targets(c));
+  if (learned_sampler(c) && (c->has_importance || c->model)) MCLS_TRY(ensure_proposal(c));
+  return MCLS_OK;
 }
 
 int mcls_gpu_domain_nnz(mcls_gpu_context* c, int64_t* nnz) {
@@ -1755,7 +1847,7 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   else if (first + count > nh) return fail(MCLS_ERR_INVALID_ARG, "history range beyond the forward source");
   if (c->prm.sampler == mcls::SAMPLER_ALIAS_STRIDED && !c->strided_valid)
     return fail(MCLS_ERR_STATE, "strided alias records not built");
-  WalkArgs a = walk_args(c, c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
+  WalkArgs a = walk_args(c, ctx_bufs(c), c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
   a.paths = dpaths.p;
   a.wfinal = dw.p;
   a.max_len = max_len;
@@ -1807,6 +1899,89 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   MCLS_TRY(accumulate_mc_timing(c));
   c->stats.steps = (int64_t)steps;
   MCLS_TRY(histories_done(c, &c->stats.histories));
+  c->stats.iterations = 0;
+  c->stats.converged = 1;
+  c->stats.solve_index = (int64_t)c->solve_index;
+  c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_mc_solve_noalloc(mcls_gpu_context* c, const double* b, double* x) {
+  auto t0 = std::chrono::steady_clock::now();
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  if (!b || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  if ((int64_t)c->n > (int64_t)MCLS_NOALLOC_MAX_N)
+    return fail(MCLS_ERR_INVALID_ARG, "mcls_gpu_mc_solve_noalloc: n = " + std::to_string(c->n) +
+                                          " exceeds its scratch limit of " + std::to_string(MCLS_NOALLOC_MAX_N) +
+                                          " unknowns");
+  if (c->nsys > 0) return fail(MCLS_ERR_UNSUPPORTED, "mcls_gpu_mc_solve_noalloc: not with batched systems");
+  if (general_prec(c))
+    return fail(MCLS_ERR_UNSUPPORTED, "mcls_gpu_mc_solve_noalloc: not with sparse preconditioners");
+  if (c->model)
+    return fail(MCLS_ERR_UNSUPPORTED, "mcls_gpu_mc_solve_noalloc: not with an importance network attached");
+  if (!tables_ready(c))
+    return fail(MCLS_ERR_STATE, "mcls_gpu_mc_solve_noalloc: the domain or the Transition Sampler's tables are not "
+                                "built for the current operator and parameters (call mcls_gpu_build_domain)");
+  if (!c->scratch) return fail(MCLS_ERR_STATE, "scratch array not resolved on this device");
+  hipStream_t s = c->stream;
+  const size_t n = (size_t)c->n;
+  // carve the per-call buffers out of the scratch array (256-byte aligned)
+  unsigned char* base = (unsigned char*)c->scratch;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  SolveBufs sb{};
+  sb.queue = (unsigned long long*)take(mcls::kQueueWords * sizeof(unsigned long long));
+  sb.steps = (unsigned long long*)take(sizeof(unsigned long long));
+  sb.bmax = (unsigned long long*)take(sizeof(unsigned long long));
+  sb.scan_tmp = (uint64_t*)take((mcls::scan_tmp_elems((int64_t)n) + 1) * sizeof(uint64_t));
+  if (off > kNoAllocFixedBytes) return fail(MCLS_ERR_INVALID_ARG, "scratch layout exceeds its fixed part");
+  off = kNoAllocFixedBytes;
+  double* rhs = (double*)take(n * sizeof(double));
+  sb.q = (uint64_t*)take(n * sizeof(uint64_t));
+  sb.qcdf = (uint64_t*)take(n * sizeof(uint64_t));
+  sb.ncnt = (uint64_t*)take(n * sizeof(uint64_t));
+  sb.ncdf = (uint64_t*)take(n * sizeof(uint64_t));
+  unsigned char* tally = take(n * sizeof(double));
+  sb.t64 = (double*)tally;
+  sb.t32 = (float*)tally;
+  double* b_int = c->ordered ? (double*)take(n * sizeof(double)) : nullptr;
+  double* x_int = c->ordered ? (double*)take(n * sizeof(double)) : nullptr;
+  if (off > kNoAllocBytes) return fail(MCLS_ERR_INVALID_ARG, "scratch layout exceeds the scratch array");
+
+  c->stats.walk_launches = 0;
+  c->stats.ms_walk = c->stats.ms_source = c->stats.ms_residual = 0.0;
+  // the array keeps the last call's contents: clear what must start at zero
+  // (fill kernels, so that the call can be captured in a graph)
+  HIP_TRY(mcls::launch_fill_u64(sb.steps, 1, 0ULL, s));
+  if (c->prm.precision == 64 || tile_on(c, false))
+    HIP_TRY(mcls::launch_fill_f64(sb.t64, (int64_t)n, 0.0, s));
+  else
+    HIP_TRY(mcls::launch_fill_i32((int*)sb.t32, (int64_t)n, 0, s));
+  const double* bb = b;
+  if (c->ordered) {
+    HIP_TRY(mcls::launch_gather_rows((int64_t)n, 1, sizeof(double), c->ord.p, b, b_int, s));
+    bb = b_int;
+  }
+  // buildMonteCarloSource (src/MCLS_MonteCarloSolverManager_impl.hpp:347-380): rhs = omega PL b
+  if (c->has_prec)
+    HIP_TRY(mcls::launch_diag_scale(c->n, c->dinv.p, bb, rhs, s));
+  else
+    HIP_TRY(mcls::launch_scale_copy(c->n, 1.0, bb, rhs, s));
+  if (c->prm.neumann_omega != 1.0) HIP_TRY(mcls::launch_scale_copy(c->n, c->prm.neumann_omega, rhs, rhs, s));
+  const uint64_t sidx = c->solve_index++;
+  MCLS_TRY(mc_solve_device(c, sb, rhs, c->ordered ? x_int : x, false, sidx));
+  if (c->ordered) HIP_TRY(mcls::launch_scatter_rows((int64_t)n, 1, sizeof(double), c->ord.p, x_int, x, s));
+  unsigned long long steps = 0;
+  HIP_TRY(hipMemcpyAsync(&steps, sb.steps, sizeof(steps), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  MCLS_TRY(accumulate_mc_timing(c));
+  c->stats.steps = (int64_t)steps;
+  MCLS_TRY(histories_done(c, sb, &c->stats.histories));
   c->stats.iterations = 0;
   c->stats.converged = 1;
   c->stats.solve_index = (int64_t)c->solve_index;

@@ -1959,6 +1959,11 @@ __global__ void k_scale_copy(int n, double a, const double* __restrict__ x, doub
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i] * a;
 }
 
+__global__ void k_diag_scale(int n, const double* __restrict__ d, const double* __restrict__ x,
+                             double* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = d[i] * x[i];
+}
+
 template <bool ACCUM>
 __global__ void k_tally_out(int n, double* __restrict__ x, double* __restrict__ t64, float* __restrict__ t32,
                             const double* __restrict__ rhs, int add_rhs, double inv,
@@ -2463,6 +2468,11 @@ hipError_t launch_axpy(int n, double a, const double* y, double* x, hipStream_t 
 
 hipError_t launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s) {
   k_scale_copy<<<grid_for(n, 256), 256, 0, s>>>(n, a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_scale(int n, const double* d, const double* x, double* y, hipStream_t s) {
+  k_diag_scale<<<grid_for(n, 256), 256, 0, s>>>(n, d, x, y);
   return hipGetLastError();
 }
 

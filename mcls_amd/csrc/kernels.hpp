@@ -314,6 +314,8 @@ hipError_t launch_batch_source(int n, const int* sys_of, const int64_t* off, con
 hipError_t launch_axpy(int n, double a, const double* y, double* x, hipStream_t s);
 // y = a * x
 hipError_t launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
+// y_i = d_i * x_i
+hipError_t launch_diag_scale(int n, const double* d, const double* x, double* y, hipStream_t s);
 // x += tally * inv (+ rhs for the expected-value estimator); tally zeroed.
 // inv = 1/nh, or 1/M read from *mtotal when mtotal != NULL.
 hipError_t launch_apply_tally(int n, double* x, double* t64, float* t32, const double* rhs,
