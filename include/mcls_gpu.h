@@ -231,6 +231,13 @@ int mcls_gpu_prec_residual(mcls_gpu_context* ctx, const double* b,
                            const double* x, double* r, double* norm2,
                            int ptr_kind);
 int mcls_gpu_get_stats(mcls_gpu_context* ctx, mcls_gpu_stats* out);
+/* The relative residuals ||rp||_2 / ||PL b||_2 of the last
+ * mcls_gpu_mcsa_solve: entry 0 before the first iteration, entry k after
+ * iteration k -- the values MCSASolverManager::solve prints every
+ * "Iteration Print Frequency" iterations
+ * (src/MCLS_MCSASolverManager_impl.hpp:392-406).  Copies at most `cap`
+ * entries; *count receives how many there are. */
+int mcls_gpu_get_residual_history(mcls_gpu_context* ctx, double* hist, int64_t cap, int64_t* count);
 
 /* ------------------------------------------------- convergence criteria -- */
 /* AlmostOptimalDomain::computeConvergenceCriteria
@@ -360,6 +367,15 @@ int mcls_gpu_record_paths(mcls_gpu_context* ctx, const double* rhs,
  * `solve_index`: x = normalised tally for source rhs.  Host pointers. */
 int mcls_gpu_mc_solve_raw(mcls_gpu_context* ctx, const double* rhs,
                           uint64_t solve_index, double* x);
+/* The same solve restricted to the history ids [first, first+count) of its
+ * full source: x = that window's tally normalised by the full history
+ * count, without the expected-value estimator's birth term rhs, so the
+ * windows of a partition sum to mcls_gpu_mc_solve_raw's x (minus rhs for
+ * that estimator).  Host pointers.  (MCSolver::solve over part of the
+ * source, src/MCLS_MCSolver_impl.hpp:83-112, for parity checks of large
+ * solves against a CPU restatement.) */
+int mcls_gpu_mc_solve_window(mcls_gpu_context* ctx, const double* rhs, uint64_t solve_index,
+                             int64_t first, int64_t count, double* x);
 
 #ifdef __cplusplus
 }

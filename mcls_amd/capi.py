@@ -41,7 +41,8 @@ GPU_SYMBOLS = [
     "mcls_gpu_set_importance_model",
     "mcls_gpu_set_importance_mask", "mcls_gpu_importance_features", "mcls_gpu_importance_targets",
     "mcls_gpu_set_left_prec_matrix", "mcls_gpu_set_right_prec_matrix", "mcls_gpu_clear_right_prec",
-    "mcls_gpu_set_state_order", "mcls_gpu_mc_solve_noalloc",
+    "mcls_gpu_set_state_order", "mcls_gpu_mc_solve_noalloc", "mcls_gpu_get_residual_history",
+    "mcls_gpu_mc_solve_window",
 ]
 MCLS_NOALLOC_MAX_N = 1048576
 MLP_SYMBOLS = [
@@ -169,6 +170,8 @@ def gpu_lib():
     L.mcls_gpu_mc_solve.argtypes = [vp, vp, vp, i32]
     L.mcls_gpu_mcsa_solve.argtypes = [vp, vp, vp, i32]
     L.mcls_gpu_mc_solve_noalloc.argtypes = [vp, vp, vp]
+    L.mcls_gpu_get_residual_history.argtypes = [vp, vp, i64, C.POINTER(i64)]
+    L.mcls_gpu_mc_solve_window.argtypes = [vp, vp, u64, i64, i64, vp]
     L.mcls_gpu_prec_residual.argtypes = [vp, vp, vp, vp, C.POINTER(f64), i32]
     L.mcls_gpu_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.mcls_gpu_record_paths.argtypes = [vp, vp, u64, i64, i64, C.c_int32, vp, vp]
@@ -512,6 +515,12 @@ class Context:
         self._check(self.L.mcls_gpu_mc_solve_raw(self.h, _p(rhs), int(solve_index), _p(x)))
         return x
 
+    def mc_solve_window(self, rhs, solve_index, first, count):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        x = np.empty(self.n)
+        self._check(self.L.mcls_gpu_mc_solve_window(self.h, _p(rhs), int(solve_index), int(first), int(count), _p(x)))
+        return x
+
     def mcsa_solve(self, b, x0=None):
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self.n) if x0 is None else np.array(x0, np.float64)
@@ -606,6 +615,14 @@ class Context:
         T = np.empty(self.n, np.float32)
         self._check(self.L.mcls_gpu_importance_targets(self.h, _p(v), int(solve_index), _p(T), MCLS_HOST))
         return T
+
+    def residual_history(self):
+        """||rp|| / ||PL b|| before and after each iteration of the last MCSA solve."""
+        m = C.c_int64(0)
+        self._check(self.L.mcls_gpu_get_residual_history(self.h, None, 0, C.byref(m)))
+        h = np.empty(max(m.value, 1))
+        self._check(self.L.mcls_gpu_get_residual_history(self.h, _p(h), m.value, C.byref(m)))
+        return h[: m.value]
 
     def stats(self):
         s = Stats()

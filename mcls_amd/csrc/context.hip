@@ -183,6 +183,7 @@ struct mcls_gpu_context {
   uint64_t solve_index = 0;
   mcls_gpu_stats stats{};
   void* scratch = nullptr;  // this device's `scratch` (allocation-free solve)
+  std::vector<double> res_hist;  // ||rp||_2 / ||PL b||_2 after each MCSA iteration (0: initial)
 
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -943,8 +944,15 @@ int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const doubl
 // window < 1: walk only that fraction of the source's histories, a run of
 // consecutive history ids from the middle of the queue (the autotuner's
 // pilots: same births per state as the full solve, fewer states).
+// wcount >= 0: walk only the history ids [wfirst, wfirst + wcount) of the
+// full source, still normalised by its full count, without the expected-value
+// birth term (a window of the solve; windows of a partition sum to it).
+// mag (nullable): the magnitudes births are drawn by, when they are not |rhs|
+// (a batch's reallocation to its unconverged systems, set by the MCSA
+// iteration that computed them).
 int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, double* x, bool accumulate,
-                    uint64_t sidx, bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0) {
+                    uint64_t sidx, bool timed = true, const uint64_t* key_dev = nullptr, double window = 1.0,
+                    int64_t wfirst = 0, int64_t wcount = -1, const double* mag = nullptr) {
   hipStream_t s = c->stream;
   const int64_t nh = num_histories(c);
   if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
@@ -956,7 +964,7 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
   const bool forward = c->prm.forward != 0;
   if (forward && c->prm.estimator != mcls::EST_COLLISION)
     return fail(MCLS_ERR_UNSUPPORTED, "the forward method tallies collisions only (Estimator Type 'Collision')");
-  if (!forward) MCLS_TRY(build_source(c, sb, rhs, nh, c->nsys > 0 ? c->mag.p : nullptr));
+  if (!forward) MCLS_TRY(build_source(c, sb, rhs, nh, mag));
   if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
   const bool strat = !forward && c->prm.source == mcls::SRC_STRATIFIED;
   const int64_t last = strat ? nh + c->n : nh;
@@ -966,6 +974,10 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
     first = (last - span) / 2;
     end = first + span;
   }
+  if (wcount >= 0) {
+    first = std::min(std::max<int64_t>(wfirst, 0), last);
+    end = std::min(first + wcount, last);
+  }
   WalkArgs a = walk_args(c, sb, rhs, solve_key(c->prm.seed, sidx), first, end);
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, end - first);
@@ -974,9 +986,9 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
     if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile x waves per block exceeds 64 KiB of LDS per workgroup");
   }
-  if (nh > 0) HIP_TRY(mcls::launch_walk(a, wc, s));
+  if (nh > 0 && end > first) HIP_TRY(mcls::launch_walk(a, wc, s));
   if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
-  const bool ev = !forward && c->prm.estimator == mcls::EST_EXPECTED_VALUE;
+  const bool ev = !forward && c->prm.estimator == mcls::EST_EXPECTED_VALUE && wcount < 0;
   // forward: each state's sum over its own histories (ForwardTally::finalize)
   double inv = nh > 0 ? 1.0 / (double)(forward ? forward_per_state(c) : nh) : 0.0;
   const uint64_t* mtot = strat ? sb.ncdf + (c->n - 1) : nullptr;
@@ -1119,7 +1131,9 @@ int mcsa_iteration(mcls_gpu_context* c, bool timed, const uint64_t* key_dev, uin
   }
   if (timed) HIP_TRY(hipEventRecord(c->ev[6], s));
   if (key_dev) HIP_TRY(mcls::launch_next_key(p.seed, c->sidx_dev.p, c->key_dev.p, s));
-  MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, true, sidx, timed, key_dev));  // x += delta x
+  // x += delta x; a batch draws its births by the reallocated magnitudes
+  MCLS_TRY(mc_solve_device(c, ctx_bufs(c), c->rhs.p, c->x.p, true, sidx, timed, key_dev, 1.0, 0, -1,
+                           c->nsys > 0 ? c->mag.p : nullptr));
   MCLS_TRY(residual(c, c->b.p, c->x.p, c->r.p, 0));
   if (c->nsys > 0) {
     HIP_TRY(mcls::launch_segment_norm(c->nsys, c->sys_off.p, c->r.p, nullptr, true, c->sys_r2.p, s));
@@ -1823,6 +1837,28 @@ int mcls_gpu_mc_solve_raw(mcls_gpu_context* c, const double* rhs, uint64_t solve
   return MCLS_OK;
 }
 
+int mcls_gpu_mc_solve_window(mcls_gpu_context* c, const double* rhs, uint64_t solve_index, int64_t first,
+                             int64_t count, double* x) {
+  MCLS_TRY(check_ctx(c));
+  MCLS_TRY(check_matrix(c));
+  if (!rhs || !x) return fail(MCLS_ERR_INVALID_ARG, "null vector");
+  if (first < 0 || count < 0) return fail(MCLS_ERR_INVALID_ARG, "negative history window");
+  MCLS_TRY(ensure_vectors(c));
+  MCLS_TRY(ensure_domain(c));
+  c->stats.walk_launches = 0;
+  c->stats.ms_walk = c->stats.ms_source = 0.0;
+  HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, c->stream));
+  MCLS_TRY(vec_in(c, rhs, MCLS_HOST, c->rhs.p, 1, sizeof(double)));
+  MCLS_TRY(mc_solve_device(c, ctx_bufs(c), c->rhs.p, c->x.p, false, solve_index, true, nullptr, 1.0, first, count));
+  MCLS_TRY(vec_out(c, c->x.p, x, MCLS_HOST, 1, sizeof(double)));
+  unsigned long long steps = 0;
+  HIP_TRY(hipMemcpyAsync(&steps, c->steps.p, sizeof(steps), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  MCLS_TRY(accumulate_mc_timing(c));
+  c->stats.steps = (int64_t)steps;
+  return MCLS_OK;
+}
+
 int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve_index, int64_t first,
                           int64_t count, int32_t max_len, int32_t* paths, double* weights) {
   MCLS_TRY(check_ctx(c));
@@ -2025,6 +2061,7 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   if (std::fabs(source_norm) < 10.0 * DBL_EPSILON) source_norm = 1.0;
   const double crit = p.tol * source_norm;
   double rnorm = std::sqrt(c->host_scal[0]);
+  c->res_hist.assign(1, rnorm / source_norm);
   if (!std::isfinite(rnorm)) return fail(MCLS_ERR_NUMERIC, "non-finite initial residual");
   if (p.verbosity > 0) {
     std::printf("MCLS MCSA (gfx950): N=%d domain nnz=%lld histories/iter=%lld\n", n, (long long)c->d_nnz,
@@ -2091,6 +2128,7 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
     }
     st.histories += hd;
     rnorm = std::sqrt(c->host_scal[0]);
+    c->res_hist.push_back(rnorm / source_norm);
     if (!std::isfinite(rnorm)) {
       st.iterations = iters;
       st.converged = 0;
@@ -2137,6 +2175,15 @@ int mcls_gpu_mcsa_solve(mcls_gpu_context* c, const double* b, double* x, int ptr
   }
   st.solve_index = (int64_t)c->solve_index;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return MCLS_OK;
+}
+
+int mcls_gpu_get_residual_history(mcls_gpu_context* c, double* hist, int64_t cap, int64_t* count) {
+  MCLS_TRY(check_ctx(c));
+  if (!count || (cap > 0 && !hist)) return fail(MCLS_ERR_INVALID_ARG, "null output");
+  const int64_t m = (int64_t)c->res_hist.size();
+  for (int64_t k = 0; k < m && k < cap; ++k) hist[k] = c->res_hist[(size_t)k];
+  *count = m;
   return MCLS_OK;
 }
 

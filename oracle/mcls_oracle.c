@@ -743,10 +743,10 @@ int64_t orc_forward_per_state(int n, int64_t num_histories) {
 /* Shared driver for orc_mc_solve and orc_record_paths. */
 static int run_walks(int n, const int64_t* d_rp, const int* d_col,
                      const double* d_h, const double* d_cdf, const double* d_W,
-                     const double* b, const orc_mc_params* p,
+                     const double* b, const double* mag, const orc_mc_params* p,
                      uint64_t solve_index, double* x, int64_t* steps_out,
                      int64_t rec_first, int64_t rec_count, int max_len,
-                     int* paths, double* weights) {
+                     int* paths, double* weights, int64_t win_first, int64_t win_count) {
   domain_view d;
   domain_view_init(&d, n, d_rp, d_col, d_h, d_cdf, d_W, p->precision == 32,
                    p->sampler == ORC_SAMPLER_ALIAS || p->sampler == ORC_SAMPLER_LEARNED_ALIAS,
@@ -756,7 +756,8 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
   uint64_t* q = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* qcdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
   uint64_t* ncdf = (uint64_t*)malloc((size_t)n * sizeof(uint64_t));
-  uint64_t Q = orc_source_quantize(n, b, q, qcdf);
+  /* births drawn by |mag| (a batch's reallocated magnitudes), weights from b */
+  uint64_t Q = orc_source_quantize(n, mag ? mag : b, q, qcdf);
   uint64_t nh = (uint64_t)p->num_histories;
   uint64_t M = 0;
   const int forward = p->estimator == ORC_EST_FORWARD;
@@ -780,14 +781,22 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
     total_steps = J.steps;
   } else {
     for (int i = 0; i < n; ++i) x[i] = 0.0;
+    /* the histories walked: all, or the window [win_first, win_first + win_count) */
+    uint64_t lo = 0, hi = nh;
+    if (win_count >= 0) {
+      lo = win_first > 0 ? (uint64_t)win_first : 0;
+      hi = lo + (uint64_t)win_count < nh ? lo + (uint64_t)win_count : nh;
+      if (lo > hi) lo = hi;
+    }
+    const uint64_t span = hi - lo;
     int nt = (int)(p->num_threads > 0 ? p->num_threads : 1);
-    if ((uint64_t)nt > nh) nt = nh > 0 ? (int)nh : 1;
+    if ((uint64_t)nt > span) nt = span > 0 ? (int)span : 1;
     walk_job* jobs = (walk_job*)calloc((size_t)nt, sizeof(walk_job));
     pthread_t* th = (pthread_t*)calloc((size_t)nt, sizeof(pthread_t));
     for (int t = 0; t < nt; ++t) {
       walk_job J = {&d, b, p, solve_index, q, qcdf, Q, ncdf, M,
-                    (int64_t)(nh * (uint64_t)t / (uint64_t)nt),
-                    (int64_t)(nh * (uint64_t)(t + 1) / (uint64_t)nt),
+                    (int64_t)(lo + span * (uint64_t)t / (uint64_t)nt),
+                    (int64_t)(lo + span * (uint64_t)(t + 1) / (uint64_t)nt),
                     per_state, NULL, 0, 0, NULL, NULL};
       J.x = (double*)calloc((size_t)n, sizeof(double));
       jobs[t] = J;
@@ -811,7 +820,8 @@ static int run_walks(int n, const int64_t* d_rp, const int* d_col,
       double inv = 1.0 / (double)(forward ? (uint64_t)per_state : nh);
       for (int i = 0; i < n; ++i) x[i] = x[i] * inv;
     }
-    if (p->estimator == ORC_EST_EXPECTED_VALUE)
+    /* the deterministic birth term belongs to the whole solve, not a window */
+    if (p->estimator == ORC_EST_EXPECTED_VALUE && win_count < 0)
       for (int i = 0; i < n; ++i) x[i] = x[i] + b[i];
   }
   if (steps_out) *steps_out = total_steps;
@@ -826,8 +836,17 @@ int orc_mc_solve(int n, const int64_t* d_rp, const int* d_col,
                  const double* d_h, const double* d_cdf, const double* d_W,
                  const double* b, const orc_mc_params* p,
                  uint64_t solve_index, double* x, int64_t* steps_out) {
-  return run_walks(n, d_rp, d_col, d_h, d_cdf, d_W, b, p, solve_index, x,
-                   steps_out, 0, 0, 0, NULL, NULL);
+  return run_walks(n, d_rp, d_col, d_h, d_cdf, d_W, b, NULL, p, solve_index, x,
+                   steps_out, 0, 0, 0, NULL, NULL, 0, -1);
+}
+
+int orc_mc_solve_window(int n, const int64_t* d_rp, const int* d_col,
+                        const double* d_h, const double* d_cdf, const double* d_W,
+                        const double* b, const double* mag, const orc_mc_params* p,
+                        uint64_t solve_index, int64_t first, int64_t count,
+                        double* x, int64_t* steps_out) {
+  return run_walks(n, d_rp, d_col, d_h, d_cdf, d_W, b, mag, p, solve_index, x,
+                   steps_out, 0, 0, 0, NULL, NULL, first, count);
 }
 
 int orc_record_paths(int n, const int64_t* d_rp, const int* d_col,
@@ -835,8 +854,8 @@ int orc_record_paths(int n, const int64_t* d_rp, const int* d_col,
                      const double* b, const orc_mc_params* p,
                      uint64_t solve_index, int64_t first, int64_t count,
                      int max_len, int* paths, double* weights) {
-  return run_walks(n, d_rp, d_col, d_h, d_cdf, d_W, b, p, solve_index, NULL,
-                   NULL, first, count, max_len, paths, weights);
+  return run_walks(n, d_rp, d_col, d_h, d_cdf, d_W, b, NULL, p, solve_index, NULL,
+                   NULL, first, count, max_len, paths, weights, 0, -1);
 }
 
 /* =================================================== composite operator == */

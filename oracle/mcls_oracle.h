@@ -146,6 +146,18 @@ int orc_mc_solve(int n, const int64_t* d_rp, const int* d_col,
                  const double* b, const orc_mc_params* p,
                  uint64_t solve_index, double* x, int64_t* steps_out);
 
+/* The same solve restricted to the histories [first, first+count) of the
+ * full source (ids beyond it dropped), still normalised by the full count,
+ * without the expected-value estimator's birth term, so that the windows
+ * of a partition sum to orc_mc_solve's x minus that term.  Births are drawn
+ * by |mag| when mag is given (weights b_k Q / q_k stay unbiased for b), as
+ * a batch of systems reallocates them. */
+int orc_mc_solve_window(int n, const int64_t* d_rp, const int* d_col,
+                        const double* d_h, const double* d_cdf, const double* d_W,
+                        const double* b, const double* mag, const orc_mc_params* p,
+                        uint64_t solve_index, int64_t first, int64_t count,
+                        double* x, int64_t* steps_out);
+
 int64_t orc_forward_per_state(int n, int64_t num_histories);
 
 /* Record the state-index path of histories [first, first+count): paths is

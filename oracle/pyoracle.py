@@ -127,6 +127,9 @@ def lib():
     dom = [i32, _i64p, _i32p, _f64p, _f64p, _f64p]
     L.orc_mc_solve.argtypes = dom + [_f64p, C.POINTER(MCParams), u64, _f64p, C.POINTER(i64)]
     L.orc_mc_solve.restype = i32
+    L.orc_mc_solve_window.argtypes = dom + [_f64p, C.c_void_p, C.POINTER(MCParams), u64, i64, i64, _f64p,
+                                            C.POINTER(i64)]
+    L.orc_mc_solve_window.restype = i32
     L.orc_record_paths.argtypes = dom + [_f64p, C.POINTER(MCParams), u64, i64, i64, i32,
                                          _i32p, _f64p]
     L.orc_record_paths.restype = i32
@@ -306,6 +309,18 @@ def mc_solve(dom, b, params, solve_index=0):
     steps = C.c_int64(0)
     b = np.ascontiguousarray(b, np.float64)
     lib().orc_mc_solve(*dom.args(), b, C.byref(params), int(solve_index), x, C.byref(steps))
+    return x, int(steps.value)
+
+
+def mc_solve_window(dom, b, params, first, count, solve_index=0, mag=None):
+    """Histories [first, first+count) of the solve, normalised by its full
+    count, no expected-value birth term; births by |mag| when given."""
+    x = np.zeros(dom.n)
+    steps = C.c_int64(0)
+    b = np.ascontiguousarray(b, np.float64)
+    m = None if mag is None else np.ascontiguousarray(mag, np.float64)
+    lib().orc_mc_solve_window(*dom.args(), b, _ptr(m), C.byref(params), int(solve_index), int(first), int(count), x,
+                              C.byref(steps))
     return x, int(steps.value)
 
 

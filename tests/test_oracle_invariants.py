@@ -294,3 +294,56 @@ def test_strided_alias_is_unbiased(prec, est):
         target[:z] = np.abs(dom.h[dom.rp[i]:dom.rp[i + 1]]) / dom.W[i]
         assert np.max(np.abs(p - target)) < 1e-14
         assert np.all(thr[i, z:] == 0.0) and np.all(alias[i] < z)
+
+
+@pytest.mark.parametrize("est", [po.EST_COLLISION, po.EST_EXPECTED_VALUE])
+@pytest.mark.parametrize("source", [po.SRC_RANDOM, po.SRC_STRATIFIED])
+def test_windows_partition_the_solve(est, source):
+    # the windows [0, a), [a, c), [c, end) of one solve add up to the whole
+    # solve (minus the expected-value birth term), and a window past the end
+    # walks nothing
+    A, b = _system(30, seed=5)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv)
+    mp = po.mc_params(4000, 8, 0.0, 64, est, source, seed=19)
+    full, steps = po.mc_solve(dom, b, mp, solve_index=4)
+    parts = [po.mc_solve_window(dom, b, mp, f, c, solve_index=4) for f, c in ((0, 1000), (1000, 1700), (2700, 10**6))]
+    total = sum(x for x, _ in parts)
+    if est == po.EST_EXPECTED_VALUE:
+        total = total + b
+    assert np.allclose(total, full, rtol=1e-12, atol=1e-12)
+    assert sum(s for _, s in parts) == steps
+    empty, s0 = po.mc_solve_window(dom, b, mp, 10**7, 50, solve_index=4)
+    assert s0 == 0 and not np.any(empty)
+
+
+@pytest.mark.parametrize("source", [po.SRC_RANDOM, po.SRC_STRATIFIED])
+def test_birth_magnitudes_keep_the_estimator_unbiased(source):
+    # births drawn by |mag| instead of |b| (a batch's reallocation) with
+    # weights b_k Q / q_k: the same expectation; mag = |b| is the plain solve
+    A, b = _system()
+    n = len(b)
+    Ac = po.Csr.from_dense(A)
+    dinv = po.jacobi_dinv(Ac)
+    dom = po.build_domain(Ac, dinv)
+    H = np.eye(n) - dinv[:, None] * A
+    L = 5
+    exact = sum(np.linalg.matrix_power(H, k) @ b for k in range(L))
+    mp = po.mc_params(3000, L, 0.0, 64, po.EST_COLLISION, source, seed=23)
+    x0, _ = po.mc_solve(dom, b, mp, solve_index=2)
+    x1, _ = po.mc_solve_window(dom, b, mp, 0, 10**9, solve_index=2, mag=np.abs(b))
+    assert np.array_equal(x0, x1)
+    mag = np.abs(b) * np.linspace(0.2, 3.0, n)
+    xs = np.array([po.mc_solve_window(dom, b, mp, 0, 10**9, solve_index=500 + r, mag=mag)[0] for r in range(80)])
+    _check_z(xs.mean(0), xs.std(0, ddof=1) / np.sqrt(len(xs)), exact)
+    # mag decides where histories are born: all of it on state k with one
+    # collision per history (L = 1) tallies b_k at k and nothing elsewhere
+    k = 7
+    one = np.zeros(n)
+    one[k] = 2.5
+    m1 = po.mc_params(500, 1, 0.0, 64, po.EST_COLLISION, source, seed=29)
+    x, steps = po.mc_solve_window(dom, b, m1, 0, 10**9, solve_index=1, mag=one)
+    expect = np.zeros(n)
+    expect[k] = b[k]
+    assert np.allclose(x, expect, rtol=1e-14, atol=0) and steps == 500
