@@ -174,10 +174,13 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        issues both of their record loads before waiting on either
  *        ("Alias Strided" FP32 walks, collision or forward estimator; other
  *        walks and path recording use one per lane)
- *   str "Tally Tile Precision" ("FP64" | "Walk"): the tile's partial sums
- *        in FP64 (12 bytes of LDS per entry) or in the walk's precision
- *        (FP32 walks: 8 bytes, more resident waves per CU); either way
- *        they are flushed into the FP64 tally
+ *   str "Tally Tile Precision" ("FP64" | "Walk" | "Compensated"): the
+ *        tile's partial sums in FP64 (12 bytes of LDS per entry), in the
+ *        walk's precision (FP32 walks: 8 bytes, more resident waves per
+ *        CU), or in FP32 with a compensation term per entry that collects
+ *        the exact rounding error of every addition (FP32 walks, 12
+ *        bytes); each is flushed into the FP64 tally, which every walk
+ *        precision uses (FP32 walks included, tiled or not)
  * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
 int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
 int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);

@@ -94,7 +94,7 @@ struct Params {
   int64_t walk_blocks_per_cu = 0;
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
-  int tile_prec = 64;      // tile partial sums: 64, or 32 = the walk's precision
+  int tile_kind = mcls::kTileF64;  // tile partial sums: FP64, the walk's precision, or FP32 compensated
   int64_t walks_per_lane = 1;  // 2: two histories per lane (strided alias walks)
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
@@ -173,8 +173,7 @@ struct mcls_gpu_context {
 
   // work vectors
   DBuf<double> x, b, r, rhs, part, scal;
-  DBuf<double> t64;
-  DBuf<float> t32;
+  DBuf<double> t64;  // the tally (FP64 for every walk precision)
   DBuf<uint64_t> q, qcdf, ncnt, ncdf, scan_tmp;
   DBuf<unsigned long long> bmax, steps, queue;
   DBuf<int> flags;
@@ -321,12 +320,8 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->mag.ensure(n, "source magnitudes"));
   MCLS_TRY(c->pt1.ensure(n, "prec work 1"));
   MCLS_TRY(c->pt2.ensure(n, "prec work 2"));
-  if (c->prm.precision == 32 && (!c->t32.p || c->t32.n < n)) {
-    MCLS_TRY(c->t32.ensure(n, "tally32"));
-    HIP_TRY(hipMemsetAsync(c->t32.p, 0, n * sizeof(float), c->stream));
-  }
-  // FP64 walks, and FP32 walks whose LDS tile flushes into an FP64 tally
-  if ((c->prm.precision == 64 || c->prm.tally_tile > 0) && (!c->t64.p || c->t64.n < n)) {
+  // the tally: FP64 for every walk precision
+  if (!c->t64.p || c->t64.n < n) {
     MCLS_TRY(c->t64.ensure(n, "tally64"));
     HIP_TRY(hipMemsetAsync(c->t64.p, 0, n * sizeof(double), c->stream));
   }
@@ -337,12 +332,10 @@ bool tile_on(const mcls_gpu_context* c, bool record) {
   return !record && c->prm.tally_tile > 0 && c->prm.estimator == mcls::EST_COLLISION && !c->prm.forward;
 }
 
-// The tally a walk writes: FP64 for FP64 walks and for tiled FP32 walks.
-double* tally64_of(mcls_gpu_context* c, bool record) {
-  return (c->prm.precision == 64 || tile_on(c, record)) ? c->t64.p : nullptr;
-}
-float* tally32_of(mcls_gpu_context* c, bool record) {
-  return (c->prm.precision == 32 && !tile_on(c, record)) ? c->t32.p : nullptr;
+// The LDS tile kind of the walk (FP64 walks keep FP64 partial sums).
+int tile_kind_of(const Params& p) {
+  if (p.precision == 64) return mcls::kTileF64;
+  return p.tile_kind;
 }
 
 // The per-solve buffers of an MC solve: the context's own, or those the
@@ -350,13 +343,12 @@ float* tally32_of(mcls_gpu_context* c, bool record) {
 struct SolveBufs {
   uint64_t *q, *qcdf, *ncnt, *ncdf, *scan_tmp;
   unsigned long long *bmax, *steps, *queue;
-  double* t64;  // FP64 tally (FP64 walks, tiled FP32 walks)
-  float* t32;   // FP32 tally (untiled FP32 walks)
+  double* t64;  // the tally (FP64 for every walk precision)
 };
 
 SolveBufs ctx_bufs(mcls_gpu_context* c) {
   return SolveBufs{c->q.p,    c->qcdf.p,  c->ncnt.p,  c->ncdf.p, c->scan_tmp.p,
-                   c->bmax.p, c->steps.p, c->queue.p, c->t64.p,  c->t32.p};
+                   c->bmax.p, c->steps.p, c->queue.p, c->t64.p};
 }
 
 void invalidate_graph(mcls_gpu_context* c) {
@@ -853,7 +845,7 @@ WalkConfig walk_config(const mcls_gpu_context* c, bool record, int64_t nh) {
   wc.source = c->prm.source;
   wc.sampler = c->prm.sampler;
   wc.record = record;
-  wc.tile = !tile_on(c, record) ? mcls::kTileOff : c->prm.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64;
+  wc.tile = !tile_on(c, record) ? mcls::kTileOff : tile_kind_of(c->prm);
   wc.walks_per_lane = (int)c->prm.walks_per_lane;
   wc.block = (int)c->prm.walk_block;
   int64_t per_cu = c->prm.walk_blocks_per_cu > 0 ? c->prm.walk_blocks_per_cu : 8;
@@ -902,7 +894,6 @@ WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, 
   a.history_length = c->prm.history_length;
   a.cutoff = c->prm.weight_cutoff;
   a.tally64 = sb.t64;
-  a.tally32 = sb.t32;
   a.tile_slots = (int)c->prm.tally_tile;
   a.tile_shift = 32;
   for (int64_t t = c->prm.tally_tile; t > 1; t >>= 1) --a.tile_shift;
@@ -982,7 +973,7 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
   a.key_dev = key_dev;
   WalkConfig wc = walk_config(c, false, end - first);
   if (wc.tile != mcls::kTileOff) {
-    const size_t elem = mcls::tile_entry_bytes(wc.tile, c->prm.precision == 32 ? 4 : 8);  // key + partial sum
+    const size_t elem = mcls::tile_entry_bytes(wc.tile, c->prm.precision == 32 ? 4 : 8);  // key + partial sum(s)
     if ((size_t)(wc.block / 64) * (size_t)c->prm.tally_tile * elem > 65536)
       return fail(MCLS_ERR_INVALID_ARG, "Tally Tile x waves per block exceeds 64 KiB of LDS per workgroup");
   }
@@ -992,13 +983,10 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
   // forward: each state's sum over its own histories (ForwardTally::finalize)
   double inv = nh > 0 ? 1.0 / (double)(forward ? forward_per_state(c) : nh) : 0.0;
   const uint64_t* mtot = strat ? sb.ncdf + (c->n - 1) : nullptr;
-  const bool fp64_tally = c->prm.precision == 64 || tile_on(c, false);
-  double* t64 = fp64_tally ? sb.t64 : nullptr;
-  float* t32 = fp64_tally ? nullptr : sb.t32;
   if (accumulate)
-    HIP_TRY(mcls::launch_apply_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
+    HIP_TRY(mcls::launch_apply_tally(c->n, x, sb.t64, nullptr, rhs, ev, inv, mtot, s));
   else
-    HIP_TRY(mcls::launch_store_tally(c->n, x, t64, t32, rhs, ev, inv, mtot, s));
+    HIP_TRY(mcls::launch_store_tally(c->n, x, sb.t64, nullptr, rhs, ev, inv, mtot, s));
   c->stats.walk_launches += nh > 0 ? 1 : 0;
   return MCLS_OK;
 }
@@ -1746,8 +1734,9 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Walk Precision '") + v + "'");
     if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
   } else if (key_is(key, "Tally Tile Precision")) {
-    if (key_is(v, "FP64")) p.tile_prec = 64;
-    else if (key_is(v, "Walk")) p.tile_prec = 32;
+    if (key_is(v, "FP64")) p.tile_kind = mcls::kTileF64;
+    else if (key_is(v, "Walk")) p.tile_kind = mcls::kTileWalk;
+    else if (key_is(v, "Compensated")) p.tile_kind = mcls::kTileComp;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Tally Tile Precision '") + v + "'");
   } else if (key_is(key, "Fixed Point Type")) {
     if (!key_is(v, "Richardson"))
@@ -1890,9 +1879,7 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   WalkConfig wc = walk_config(c, true, count);
   HIP_TRY(mcls::launch_walk(a, wc, s));
   // clear the tally the recorded walks wrote
-  double* t64 = tally64_of(c, true);
-  float* t32 = tally32_of(c, true);
-  HIP_TRY(mcls::launch_store_tally(c->n, c->x.p, t64, t32, c->rhs.p, false, 0.0, nullptr, s));
+  HIP_TRY(mcls::launch_store_tally(c->n, c->x.p, c->t64.p, nullptr, c->rhs.p, false, 0.0, nullptr, s));
   HIP_TRY(hipMemcpyAsync(paths, dpaths.p, sizeof(int) * (size_t)count * (size_t)(max_len + 1),
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(weights, dw.p, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost, s));
@@ -1982,9 +1969,7 @@ int mcls_gpu_mc_solve_noalloc(mcls_gpu_context* c, const double* b, double* x) {
   sb.qcdf = (uint64_t*)take(n * sizeof(uint64_t));
   sb.ncnt = (uint64_t*)take(n * sizeof(uint64_t));
   sb.ncdf = (uint64_t*)take(n * sizeof(uint64_t));
-  unsigned char* tally = take(n * sizeof(double));
-  sb.t64 = (double*)tally;
-  sb.t32 = (float*)tally;
+  sb.t64 = (double*)take(n * sizeof(double));
   double* b_int = c->ordered ? (double*)take(n * sizeof(double)) : nullptr;
   double* x_int = c->ordered ? (double*)take(n * sizeof(double)) : nullptr;
   if (off > kNoAllocBytes) return fail(MCLS_ERR_INVALID_ARG, "scratch layout exceeds the scratch array");
@@ -1994,10 +1979,7 @@ int mcls_gpu_mc_solve_noalloc(mcls_gpu_context* c, const double* b, double* x) {
   // the array keeps the last call's contents: clear what must start at zero
   // (fill kernels, so that the call can be captured in a graph)
   HIP_TRY(mcls::launch_fill_u64(sb.steps, 1, 0ULL, s));
-  if (c->prm.precision == 64 || tile_on(c, false))
-    HIP_TRY(mcls::launch_fill_f64(sb.t64, (int64_t)n, 0.0, s));
-  else
-    HIP_TRY(mcls::launch_fill_i32((int*)sb.t32, (int64_t)n, 0, s));
+  HIP_TRY(mcls::launch_fill_f64(sb.t64, (int64_t)n, 0.0, s));
   const double* bb = b;
   if (c->ordered) {
     HIP_TRY(mcls::launch_gather_rows((int64_t)n, 1, sizeof(double), c->ord.p, b, b_int, s));
@@ -2307,7 +2289,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     const int64_t shape_tile = best.tally_tile;
     for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
       if (tiles[k] == shape_tile) continue;
-      const size_t elem = mcls::tile_entry_bytes(saved.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64,
+      const size_t elem = mcls::tile_entry_bytes(tile_kind_of(saved),
                                                  saved.precision == 32 ? 4 : 8);
       if ((size_t)(best.block / 64) * (size_t)tiles[k] * elem > 65536) continue;
       rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, tiles[k], &ms);
@@ -2328,7 +2310,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     for (int i = 0; i < 3 && rc == MCLS_OK; ++i) {
       if (block_bpc[i] == 0) continue;
       if (best.tally_tile > 0 && saved.estimator == mcls::EST_COLLISION && !saved.forward) {
-        const size_t elem = mcls::tile_entry_bytes(saved.tile_prec == 32 ? mcls::kTileWalk : mcls::kTileF64,
+        const size_t elem = mcls::tile_entry_bytes(tile_kind_of(saved),
                                                    saved.precision == 32 ? 4 : 8);
         if ((size_t)(blocks[i] / 64) * (size_t)best.tally_tile * elem > 65536) continue;
       }

@@ -105,8 +105,7 @@ struct WalkArgs {
   int64_t last;            // one past the last history id (random source)
   int64_t history_length;  // max transitions
   double cutoff;           // relative weight cutoff
-  double* tally64;         // FP64 walks
-  float* tally32;          // FP32 walks
+  double* tally64;         // the tally (FP64 for every walk precision)
   unsigned long long* steps;  // total transitions (atomic)
   unsigned long long* queue;  // kQueueWords counters, zeroed by launch_walk
   int* paths;              // recording: (last-first) x (max_len+1)
@@ -120,11 +119,13 @@ struct WalkArgs {
                            // states); 0: Fibonacci hash of the state
 };
 
-// LDS tally tile: off, FP64 partial sums, or partial sums in the walk's
-// precision (the same as FP64 for FP64 walks)
-enum TileKind { kTileOff = 0, kTileF64 = 1, kTileWalk = 2 };
-// LDS bytes per tile entry: an int key and the partial sum
+// LDS tally tile: off, FP64 partial sums, partial sums in the walk's
+// precision (the same as FP64 for FP64 walks), or FP32 partial sums with an
+// FP32 compensation term each (FP32 walks)
+enum TileKind { kTileOff = 0, kTileF64 = 1, kTileWalk = 2, kTileComp = 3 };
+// LDS bytes per tile entry: an int key, the partial sum (and compensation)
 inline size_t tile_entry_bytes(int tile, size_t real_bytes) {
+  if (tile == kTileComp) return sizeof(int) + 2 * sizeof(float);
   return sizeof(int) + (tile == kTileWalk ? real_bytes : sizeof(double));
 }
 

@@ -41,7 +41,7 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
         p, w = ctx.record_paths(rhs, 5, 0, 3000, 200)
         assert np.array_equal(p, ref_p) and np.array_equal(w, ref_w)
         x = ctx.mc_solve_raw(rhs, 6)
-        tol = 1e-12 if prec == 64 else 1e-4  # FP32 atomics: summation order
+        tol = 1e-12  # FP64 tallies (every walk precision): summation order only
         assert np.linalg.norm(x - ref_x) <= tol * np.linalg.norm(ref_x)
     ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0,
                     "Monte Carlo Type": "Adjoint"})
@@ -54,30 +54,31 @@ def test_walk_batch_validation(ctx):
             ctx.set_params({"Walk Batch": bad})
 
 
-@pytest.mark.parametrize("tile_prec", ["FP64", "Walk"])
+@pytest.mark.parametrize("tile_prec", ["FP64", "Walk", "Compensated"])
 @pytest.mark.parametrize("sampler,prec", [("Alias", 32), ("CDF", 64), ("CDF", 32)])
 def test_tally_tile_matches_direct_tally(ctx, sampler, prec, tile_prec):
     """The LDS tile changes only where the additions happen: same paths, same
     tallies to the rounding of the summation order.  Its partial sums are
     FP64, or in the walk's precision ("Walk"), and are flushed into an FP64
-    tally either way."""
+    tally either way, which every walk precision uses; compensated FP32
+    partial sums ("Compensated") are as good as FP64 ones."""
     prob, rhs = _poisson(ctx, sampler, prec)
     ctx.set_params({"Tally Tile": 0, "Tally Tile Precision": tile_prec})
     ref = ctx.mc_solve_raw(rhs, 3)
     for tile, blk in ((64, 256), (512, 128), (1024, 64)):
         ctx.set_params({"Tally Tile": tile, "Walk Block Size": blk, "Walk Blocks Per CU": 0})
         x = ctx.mc_solve_raw(rhs, 3)
-        tol = 1e-12 if prec == 64 else 1e-4
+        tol = 1e-4 if (prec == 32 and tile_prec == "Walk") else 1e-12
         assert np.linalg.norm(x - ref) <= tol * np.linalg.norm(ref), tile
     ctx.set_params({"Tally Tile": 0, "Walk Block Size": 256})
     with pytest.raises(capi.GpuError):
         ctx.set_params({"Tally Tile": 100})
     with pytest.raises(capi.GpuError):
         ctx.set_params({"Tally Tile Precision": "FP16"})
-    # 4 waves x 2048 entries x 12 bytes (FP64 partial sums) > 64 KiB; with
-    # FP32 partial sums 8 bytes an entry fit exactly
+    # 4 waves x 2048 entries x 12 bytes (FP64 or compensated partial sums)
+    # > 64 KiB; with FP32 partial sums 8 bytes an entry fit exactly
     ctx.set_params({"Tally Tile": 2048, "Walk Block Size": 256})
-    if tile_prec == "FP64" or prec == 64:
+    if tile_prec != "Walk" or prec == 64:
         with pytest.raises(capi.GpuError):
             ctx.mc_solve_raw(rhs, 4)
     else:
@@ -86,7 +87,7 @@ def test_tally_tile_matches_direct_tally(ctx, sampler, prec, tile_prec):
     ctx.set_params({"Tally Tile": 0, "Tally Tile Precision": "FP64"})
 
 
-@pytest.mark.parametrize("tile_prec", ["FP64", "Walk"])
+@pytest.mark.parametrize("tile_prec", ["FP64", "Walk", "Compensated"])
 def test_mcsa_with_tally_tile_converges(ctx, tile_prec):
     prob = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 16, seed=3, p0=0.6, p1=0.9, p2=0.8, p3=1.2),
                               want64=False, want32=True)
