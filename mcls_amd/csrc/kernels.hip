@@ -76,6 +76,26 @@ __device__ __forceinline__ int upper_bound_u64(const uint64_t* __restrict__ cdf,
   return lo;
 }
 
+// First k in [lo, n) with cdf[k] > t (n when none), found by the whole wave
+// in a few dependent round trips instead of log2(n): each round the 64 lanes
+// probe 64 evenly spaced entries of the range left and the ballot of
+// "cdf <= t" (a prefix of the lanes: cdf is non-decreasing) narrows it
+// 64-fold.  Wave-uniform arguments; every lane of the wave must call it and
+// every lane gets the answer.
+__device__ __forceinline__ int wave_upper_bound_u64(const uint64_t* __restrict__ cdf, int lo, int n, uint64_t t,
+                                                    int lane) {
+  int hi = n;  // the answer lies in [lo, hi]
+  while (hi - lo > 64) {
+    const int64_t span = hi - lo;
+    const int p = lo + (int)(span * (lane + 1) / 64) - 1;  // in [lo, hi - 1]
+    const int c = __popcll(__ballot(cdf[p] <= t));
+    const int nlo = c > 0 ? lo + (int)(span * c / 64) : lo;  // one past probe c - 1
+    if (c < 64) hi = lo + (int)(span * (c + 1) / 64) - 1;    // probe c: cdf > t there
+    lo = nlo;
+  }
+  return lo + __popcll(__ballot(lane < hi - lo && cdf[lo + lane] <= t));
+}
+
 // first k in [0, n) with cdf[k] >= u  (std::lower_bound)
 template <typename Real>
 __device__ __forceinline__ int lower_bound_cdf(const Real* __restrict__ cdf, int n, Real u) {
@@ -356,6 +376,15 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   constexpr bool CARRY = (SAMP != SAMPLER_ALIAS || FUSED) && !STRIDED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
   int slo = 0, shi = 0;         // stratified: birth states of the current batch
+  // Stratified births of a batch whose states fit one wave (the window
+  // [slo, shi] of at most 64 states, wave-uniform `window`): lane L holds
+  // state slo + L's last history id relative to the batch start (win_c) and
+  // its birth weight (win_w0), so a birth is a shuffle search, not a chain of
+  // dependent loads and a division.
+  bool window = false;
+  int64_t qbase = 0;
+  uint32_t win_c = 0xFFFFFFFFu;
+  double win_w0 = 0.0;
   bool exhausted = false;       // wave-uniform
   bool alive = false;
   int64_t hid = 0;
@@ -387,21 +416,48 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             break;
           }
           if (SRC == SRC_STRATIFIED && EST != EST_FORWARD) {
-            // the batch's ids are consecutive: bound its birth states once,
-            // so each birth searches only that window of the count table
-            int lo = 0, hi = 0;
-            if (lane == leader) {
-              lo = upper_bound_u64(a.ncdf, n, (uint64_t)qnext);
-              hi = upper_bound_u64(a.ncdf, n, (uint64_t)(qend - 1));
+            // the batch's ids are consecutive: bound its birth states once
+            // (every lane of the wave is here), so each birth searches only
+            // that window of the count table
+            slo = wave_upper_bound_u64(a.ncdf, 0, n, (uint64_t)qnext, lane);
+            qbase = qnext;
+            const int k = slo + lane;
+            const uint64_t ck = k < n ? a.ncdf[k] : ~0ULL;
+            const int c = __popcll(__ballot(ck <= (uint64_t)(qend - 1)));
+            window = c < 64;
+            if (window) {
+              shi = slo + c;
+              win_c = 0xFFFFFFFFu;
+              win_w0 = 0.0;
+              if (lane <= c) {
+                const uint64_t nk = ck - (k ? a.ncdf[k - 1] : 0);
+                const uint64_t rel = ck - (uint64_t)qbase;
+                win_c = rel < 0xFFFFFFFEULL ? (uint32_t)rel : 0xFFFFFFFEu;
+                win_w0 = nk ? a.rhs[k] * (totald / (double)nk) : 0.0;
+              }
+            } else {
+              shi = wave_upper_bound_u64(a.ncdf, slo + 64, n, (uint64_t)(qend - 1), lane);
             }
-            slo = __shfl(lo, leader);
-            shi = __shfl(hi, leader);
           }
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
         const int take = (int64_t)cnt < avail ? cnt : (int)avail;
         const int rank = __popcll(need & __lanemask_lt());
+        // window births: the first window state whose last id passes this
+        // lane's id, by binary lifting over the lanes (wave-uniform: every
+        // lane shuffles, the lanes not being born ignore the result)
+        int bpos = 0;
+        double bw0 = 0.0;
+        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && window) {
+          const uint32_t keyb = (uint32_t)(qnext - qbase) + (uint32_t)rank;
+#pragma unroll
+          for (int step = 32; step >= 1; step >>= 1) {
+            const uint32_t cv = __shfl(win_c, bpos + step - 1);
+            if (cv <= keyb) bpos += step;
+          }
+          bw0 = __shfl(win_w0, bpos);
+        }
         if (!alive && rank < take) {
           hid = qnext + rank;
           rng = history_state(key, (uint64_t)hid);
@@ -415,6 +471,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             const uint64_t t = mul_u53(xorshift64(rng), total);
             state = upper_bound_u64(a.qcdf, n, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
+          } else if (window) {
+            state = slo + bpos;
+            w0 = bw0;
           } else {
             // first k with ncdf[k] > hid lies in [slo, shi] for this batch
             state = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
