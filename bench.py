@@ -64,9 +64,10 @@ FORMS = {
                "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
                # alias tables padded to the row stride: one 16-byte record per step
                "Transition Sampler": "Alias Strided", "Monte Carlo Type": "Adjoint",
-               # LDS tally tile partial sums in FP32 (flushed into the FP64
-               # tally): 8 bytes per entry, more resident waves than FP64's 12
-               "Tally Tile Precision": "Walk"},
+               # LDS tally tile partial sums in FP32 with a compensation term
+               # each (the exact rounding error of every LDS addition), flushed
+               # together into the FP64 tally
+               "Tally Tile Precision": "Compensated"},
     "plain": {"Walk Precision": "FP64", "Source Sampling Type": "Random",
               "Estimator Type": "Collision", "Weight Cutoff": 1e-4, "History Length": 1000,
               "Transition Sampler": "CDF", "Monte Carlo Type": "Adjoint", "Tally Tile Precision": "FP64"},
@@ -219,6 +220,45 @@ def measured_traffic(workload, form, steps_per_launch, tally_tile=None, state_or
     return float(rec["traffic_bytes_per_launch"])
 
 
+# The attribution ladder: from the plain form to the chosen one a change at
+# a time (each line autotuned, same operator, numbering and tolerance).
+LADDER = [
+    ("plain estimator -> chosen estimator (stratified, cutoff 1e-1), FP64 CDF walk",
+     {"Walk Precision": "FP64", "Transition Sampler": "CDF", "Source Sampling Type": "Stratified",
+      "Weight Cutoff": 1e-1, "Tally Tile Precision": "FP64"}),
+    ("+ FP32 walk (CDF)",
+     {"Walk Precision": "FP32", "Transition Sampler": "CDF", "Source Sampling Type": "Stratified",
+      "Weight Cutoff": 1e-1, "Tally Tile Precision": "Compensated"}),
+    ("chosen kernel at the plain estimator (random, cutoff 1e-4)",
+     {"Walk Precision": "FP32", "Transition Sampler": "Alias Strided", "Source Sampling Type": "Random",
+      "Weight Cutoff": 1e-4, "Tally Tile Precision": "Compensated"}),
+]
+
+
+def error_bound(prob, x, dinv):
+    """FP64 error of x against the exact solution without solving: with
+    H = I - D^-1 A and e = x* - x, D^-1 (b - A x) = (I - H) e, so
+    ||e||_inf <= ||D^-1 r||_inf / (1 - ||H||_inf) whenever ||H||_inf < 1
+    (the 27-point workload's rows of |H| sum to s_i <= 0.9).  r in FP64 on
+    the host from the FP32 matrix values widened exactly."""
+    import scipy.sparse as sp
+
+    val = prob.val32 if prob.val32 is not None else prob.values()
+    A = sp.csr_matrix((val.astype(np.float64), prob.col, prob.rp), shape=(prob.n, prob.n))
+    r = prob.b - A @ x
+    absA = abs(A)
+    rowsum = np.asarray(absA.sum(axis=1)).ravel()
+    diag = np.abs(A.diagonal())
+    hnorm = float(np.max((rowsum - diag) / diag))
+    del absA, A
+    bound = float(np.max(np.abs(dinv * r))) / (1.0 - hnorm)
+    xinf = float(np.max(np.abs(x)))
+    return {"bound_inf_abs": bound, "bound_inf_rel": bound / xinf, "norm_H_inf": hnorm,
+            "rel_residual_2": float(np.linalg.norm(dinv * r) / np.linalg.norm(dinv * prob.b)),
+            "method": "||x - x*||_inf <= ||D^-1 (b - A x)||_inf / (1 - ||I - D^-1 A||_inf), "
+                      "residual in FP64 on the host"}
+
+
 def cpu_baseline(prob, w, plain_iterations=None, nh_sample=1 << 24):
     """The oracle's restatement of the MCLS algorithm in reference semantics
     (oracle/cpu_baseline.py: FP64 CDF lower_bound, one Xorshift stream per
@@ -234,6 +274,7 @@ def cpu_baseline(prob, w, plain_iterations=None, nh_sample=1 << 24):
     iters = plain_iterations or 7
     per_iter = r["extrapolated"]["seconds_mc_per_iteration"]
     return {"value": r["histories_per_s"], "unit": "histories/s", "cores": r["cores"], "kind": "port",
+            "host": r["host"],
             "sample": f"1 MCSA iteration of the MCLS-semantics CPU restatement (FP64, CDF lower_bound, "
                       f"{r['sets']} sets x {r['histories_per_set']} histories, weight cutoff 1e-4) on the same "
                       f"operator ({prob.n} unknowns); MC transport time only",
@@ -301,23 +342,29 @@ def main():
             "Random Seed": 2024}
     ctx.set_params(base)
     ctx.set_params(FORMS[args.form])
+    setup = {}
     t0 = time.perf_counter()
     ctx.build_domain()
     ctx.synchronize()
     st = ctx.stats()
+    setup["domain_and_tables_s"] = time.perf_counter() - t0
     log(f"domain built: nnz={st.domain_nnz} in {time.perf_counter() - t0:.2f}s (kernel {st.ms_domain:.1f} ms)")
     z = st.domain_nnz / prob.n
     learned_rec, net = None, None
     if args.form == "chosen" and not args.no_learned:
         learned_rec, net = learned_stage(ctx, prob, w)
+        setup["learned_stage_s"] = learned_rec["seconds"]
         log(f"learned proposal: {learned_rec['choice']} (relative variance "
             f"{ {k: round(v, 5) for k, v in learned_rec['relative_variance'].items()} }, walk ms "
             f"{ {k: round(v, 2) for k, v in learned_rec['walk_ms_per_solve'].items()} })")
     sampler = LEARNED_SAMPLER if learned_rec and learned_rec["choice"] != "plain" else FORMS[args.form]["Transition Sampler"]
     tuning = None
     if not args.no_tune:
+        t0 = time.perf_counter()
         tuning = ctx.autotune(prob.b)
+        setup["autotune_s"] = time.perf_counter() - t0
         log(f"autotune: {tuning}")
+    setup["total_s"] = sum(setup.values())
     ctx_tile = tuning["tally_tile"] if tuning else 0  # "Tally Tile" defaults to 0
 
     dev = DeviceVectors(prob.b)
@@ -377,7 +424,14 @@ def main():
                      "avg_launch_ms": avg_launch_s * 1e3},
         "tuning": tuning,
         "learned_proposal": learned_rec,
+        "setup_s": setup,
     }
+    # the FP64 error of the chosen solution, bounded without a solve
+    if args.workload.startswith("cfg4"):
+        dev.reset()
+        ctx.mcsa_solve_device(dev.b.ptr.value, dev.x.ptr.value)
+        out["error_vs_fp64"] = error_bound(prob, dev.x.download(), ctx.get_left_prec_diag())
+        log(f"error bound: {out['error_vs_fp64']['bound_inf_rel']:.2e} (relative, inf-norm)")
     if not args.no_forward and args.form == "chosen":
         # the forward method with the chosen form's walk (reported beside
         # the headline, which stays the adjoint solve)
@@ -424,6 +478,21 @@ def main():
         out["speedup_vs_plain"] = dt / out["time_to_tol_s"]
         if "forward" in out:
             out["forward"]["speedup_vs_plain"] = dt / out["forward"]["time_to_tol_s"]
+        # the attribution ladder: what each change buys on its own
+        ladder = []
+        for label, params in LADDER:
+            ctx.set_params(dict(FORMS["chosen"], **params))
+            if not args.no_tune:
+                ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0})
+                ctx.autotune(prob.b)
+            solve_once(ctx, dev)
+            ldt, lst = solve_once(ctx, dev)
+            ladder.append({"form": label, "time_to_tol_s": ldt, "mcsa_iterations": lst.iterations,
+                           "achieved_tol": lst.achieved_tol, "steps_per_history": lst.steps / max(lst.histories, 1),
+                           "speedup_vs_plain": dt / ldt})
+            log(f"ladder: {label}: {ldt:.3f}s iters={lst.iterations}")
+        out["attribution"] = {"plain_s": dt, "chosen_s": out["time_to_tol_s"], "ladder": ladder}
+        out["plain_same_estimator"] = ladder[0]
     dev.b.close()
     dev.x.close()
     ctx.close()

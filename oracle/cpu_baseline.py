@@ -51,13 +51,39 @@ def make_problem(name, nthreads=None):
     return capi.build_problem(spec, want64=not f32, want32=f32, nthreads=nthreads)
 
 
+def host_info():
+    """What the host offers this process: the CPU model, the CPUs the OS
+    shows and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "cpus_visible": os.cpu_count() or 1, "cpus_allowed": allowed}
+
+
+def default_cores():
+    """One set (one thread, the reference's one MPI rank) per host core this
+    process may use, at most 16: a one-GPU job on the GPU boxes gets a
+    16-core share of the host, whatever the OS shows."""
+    return max(1, min(16, host_info()["cpus_allowed"]))
+
+
 def run(prob, f32, histories, cores=None, iters=1, tol=1e-8, cutoff=1e-4, history_length=1000, quirk=False,
         full=False, workload_histories=None, seed=1):
     """Time the reference-semantics MCSA on `prob`.  Returns a dict: histories/s
     over the MC transport time, per-iteration time, steps per history,
     residual history, convergence and (without `full`) extrapolations to
     `workload_histories` per iteration."""
-    cores = int(cores or min(16, os.cpu_count() or 1))
+    cores = int(cores or default_cores())
     A = po.Csr(prob.rp, prob.col, prob.val32.astype(np.float64) if f32 else prob.values())
     dinv = po.jacobi_dinv(A)
     per_set = max(1, int(histories) // cores)
@@ -71,7 +97,8 @@ def run(prob, f32, histories, cores=None, iters=1, tol=1e-8, cutoff=1e-4, histor
            "seconds_mc": res.seconds_mc, "seconds_total": res.seconds_total,
            "seconds_mc_per_iteration": res.seconds_mc / max(res.iters, 1),
            "steps_per_history": res.steps / max(res.histories, 1),
-           "residual_history": [float(v) for v in hist], "reference_quirk": bool(quirk)}
+           "residual_history": [float(v) for v in hist], "reference_quirk": bool(quirk),
+           "host": dict(host_info(), threads_used=cores)}
     if not full and workload_histories:
         per_iter = workload_histories / hps
         out["extrapolated"] = {"histories_per_iteration": int(workload_histories),
@@ -82,7 +109,7 @@ def run(prob, f32, histories, cores=None, iters=1, tol=1e-8, cutoff=1e-4, histor
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", default="cfg4-64", choices=sorted(WORKLOADS))
-    ap.add_argument("--cores", type=int, default=None, help="sets = threads (default min(16, cores))")
+    ap.add_argument("--cores", type=int, default=None, help="sets = threads (default: the allowed host cores, at most 16)")
     ap.add_argument("--histories", type=int, default=None, help="histories per MCSA iteration (default: the workload's)")
     ap.add_argument("--iters", type=int, default=1, help="MCSA iterations (the cap with --full)")
     ap.add_argument("--tol", type=float, default=1e-8)
