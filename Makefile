@@ -15,7 +15,8 @@ CXXFLAGS := -O2 -std=c++17 -fPIC -Wall -Iinclude
 GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip $(SRC)/mlp.hip
 GPU_HDRS := $(SRC)/kernels.hpp $(SRC)/mlp.hpp include/mcls_gpu.h include/mcls_mlp.h
 
-all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade tests/cpp/bin/test_mm oracle
+all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade tests/cpp/bin/test_mm \
+     tests/cpp/bin/test_traits oracle
 
 $(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
 	mkdir -p $(LIBDIR)
@@ -34,6 +35,11 @@ tests/cpp/bin/test_facade: tests/cpp/test_facade.cpp mcls_amd/host/*.hpp include
 tests/cpp/bin/test_mm: tests/cpp/test_mm.cpp mcls_amd/host/MCLS_MatrixMarket.hpp mcls_amd/host/MCLS_Core.hpp
 	mkdir -p tests/cpp/bin
 	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_mm.cpp
+
+# traits and linear-problem checks: host-only (headers only, no GPU library)
+tests/cpp/bin/test_traits: tests/cpp/test_traits.cpp mcls_amd/host/*.hpp include/mcls_gpu.h
+	mkdir -p tests/cpp/bin
+	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_traits.cpp
 
 oracle:
 	$(MAKE) -C oracle

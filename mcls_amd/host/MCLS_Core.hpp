@@ -142,20 +142,21 @@ class CrsMatrix {
   const std::vector<int64_t>& rowPtr() const { return d_rp; }
   const std::vector<int32_t>& colInd() const { return d_col; }
   const std::vector<Scalar>& values() const { return d_val; }
+  std::vector<Scalar>& valuesNonConst() { return d_val; }
+  // replace the (fill-complete) contents with another CSR of the same rows
+  void setCrs(std::vector<int64_t> rp, std::vector<int32_t> col, std::vector<Scalar> val) {
+    MCLS_INSIST((int64_t)rp.size() == d_n + 1 && rp.back() == (int64_t)col.size() && col.size() == val.size(),
+                "CSR sizes");
+    d_rp = std::move(rp);
+    d_col = std::move(col);
+    d_val = std::move(val);
+    d_rows.clear();
+    d_filled = true;
+  }
   int64_t maxRowEntries() const {
     int64_t m = 0;
     for (int64_t i = 0; i < d_n; ++i) m = std::max(m, d_rp[(size_t)i + 1] - d_rp[(size_t)i]);
     return m;
-  }
-  // diagonal-only matrix: returns true and fills d with its diagonal
-  bool isDiagonal(std::vector<double>* d) const {
-    d->assign((size_t)d_n, 0.0);
-    for (int64_t i = 0; i < d_n; ++i)
-      for (int64_t k = d_rp[(size_t)i]; k < d_rp[(size_t)i + 1]; ++k) {
-        if (d_col[(size_t)k] != i) return false;
-        (*d)[(size_t)i] = (double)d_val[(size_t)k];
-      }
-    return true;
   }
 
  private:

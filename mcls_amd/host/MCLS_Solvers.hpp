@@ -10,6 +10,10 @@
 //   MultiSetLinearProblem (1 set)     src/MCLS_MultiSetLinearProblem.hpp
 //   SolverFactory                     src/MCLS_SolverFactory_impl.hpp:58-146
 //   TpetraPointJacobiPreconditioner   tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:105-139
+//   Preconditioner                    src/MCLS_Preconditioner.hpp:59-101
+// The managers and the linear problem reach the operator, the
+// preconditioners and the vectors only through MatrixTraits / VectorTraits
+// (MCLS_Traits.hpp): another matrix type plugs in by specialising them.
 // Non-zero C ABI status is rethrown as MCLS::Assertion.  The domain is
 // rebuilt only when the operator object changes (pointer identity, as
 // src/MCLS_MonteCarloSolverManager_impl.hpp:161-178).  A diagonal left
@@ -90,23 +94,81 @@ class LinearProblem {
   bool isRightPrec() const { return (bool)d_PR; }
   const RCP<Vector>& getResidual() const { return d_r; }
   const RCP<Vector>& getPrecResidual() const { return d_rp; }
+  // PL (A PR) and PR^T (A^T PL^T), each product filtered by `threshold`, on
+  // the host through the traits (src/MCLS_LinearProblem_impl.hpp:139-221);
+  // the device builds the same operators itself for the walks.
+  RCP<const Matrix> getCompositeOperator(double threshold = 0.0) const {
+    if (d_PL && d_PR) return MT::multiply(d_PL, false, MT::multiply(d_A, false, d_PR, false, threshold), false, threshold);
+    if (d_PL) return MT::multiply(d_PL, false, d_A, false, threshold);
+    if (d_PR) return MT::multiply(d_A, false, d_PR, false, threshold);
+    return d_A;
+  }
+  RCP<const Matrix> getTransposeCompositeOperator(double threshold = 0.0) const {
+    RCP<const Matrix> AT = MT::copyTranspose(*d_A);
+    if (d_PL && d_PR) {
+      RCP<const Matrix> t = MT::multiply(AT, false, RCP<const Matrix>(MT::copyTranspose(*d_PL)), false, threshold);
+      return MT::multiply(RCP<const Matrix>(MT::copyTranspose(*d_PR)), false, t, false, threshold);
+    }
+    if (d_PR) return MT::multiply(RCP<const Matrix>(MT::copyTranspose(*d_PR)), false, AT, false, threshold);
+    if (d_PL) return MT::multiply(AT, false, RCP<const Matrix>(MT::copyTranspose(*d_PL)), false, threshold);
+    return AT;
+  }
+  // y = PL x, y = PR x, y = A x; the identity when that operator is absent
+  // (src/MCLS_LinearProblem_impl.hpp:332-386)
+  void applyLeftPrec(const Vector& x, Vector& y) const { applyOrCopy(d_PL, x, y); }
+  void applyRightPrec(const Vector& x, Vector& y) const { applyOrCopy(d_PR, x, y); }
+  void applyOp(const Vector& x, Vector& y) const { applyOrCopy(d_A, x, y); }
+  // y = PL A PR x, the composite operator (src/MCLS_LinearProblem_impl.hpp:252-286)
+  void apply(const Vector& x, Vector& y) const {
+    auto t = VT::clone(y);
+    applyRightPrec(x, *t);
+    auto u = VT::clone(y);
+    MT::apply(*d_A, *t, *u);
+    applyLeftPrec(*u, y);
+  }
+  // y = (PL A PR)^T x = PR^T A^T PL^T x (src/MCLS_LinearProblem_impl.hpp:292-326)
+  void applyTranspose(const Vector& x, Vector& y) const {
+    auto t = VT::clone(y);
+    if (d_PL) MT::applyTranspose(*d_PL, x, *t);
+    else VT::update(*t, 0.0, x, 1.0);
+    auto u = VT::clone(y);
+    MT::applyTranspose(*d_A, *t, *u);
+    if (d_PR) MT::applyTranspose(*d_PR, *u, y);
+    else VT::update(y, 0.0, *u, 1.0);
+  }
+  // x += PR update (src/MCLS_LinearProblem_impl.hpp:228-245)
+  void updateSolution(const RCP<Vector>& update) {
+    auto t = VT::clone(*update);
+    applyRightPrec(*update, *t);
+    VT::update(*d_x, 1.0, *t, 1.0);
+  }
   // r = b - A x  (src/MCLS_LinearProblem_impl.hpp:390-403)
   void updateResidual() {
     MT::apply(*d_A, *d_x, *d_r);
     VT::update(*d_r, -1.0, *d_b, 1.0);
   }
-  // rp = PL (b - A x)  (src/MCLS_LinearProblem_impl.hpp:411-442)
+  // rp = PL (b - A PR x)  (src/MCLS_LinearProblem_impl.hpp:411-442), x the
+  // right-preconditioned unknown
   void updatePrecResidual() {
-    MCLS_INSIST(!d_PR, "right preconditioning is not provided by this build");
-    MT::apply(*d_A, *d_x, *d_rp);
+    if (d_PR) {
+      auto t = VT::clone(*d_rp);
+      MT::apply(*d_PR, *d_x, *t);
+      MT::apply(*d_A, *t, *d_rp);
+    } else {
+      MT::apply(*d_A, *d_x, *d_rp);
+    }
     VT::update(*d_rp, -1.0, *d_b, 1.0);
     if (d_PL) {
       auto tmp = VT::deepCopy(*d_rp);
-      MatrixTraits<Vector, Matrix>::apply(*d_PL, *tmp, *d_rp);
+      MT::apply(*d_PL, *tmp, *d_rp);
     }
   }
 
  private:
+  static void applyOrCopy(const RCP<const Matrix>& M, const Vector& x, Vector& y) {
+    if (M) MT::apply(*M, x, y);
+    else VT::update(y, 0.0, x, 1.0);
+  }
   RCP<const Matrix> d_A;
   RCP<Vector> d_x;
   RCP<const Vector> d_b;
@@ -150,26 +212,30 @@ template <class Vector, class Matrix>
 class DeviceProblem {
  public:
   using Problem = LinearProblem<Vector, Matrix>;
+  using MT = MatrixTraits<Vector, Matrix>;
   DeviceProblem() : d_ctx(std::make_shared<GpuContext>(0)) {}
 
   // Upload the operator when its object changed; refresh PL and PR when
   // their objects changed.
+  // Every access to a matrix goes through MT (fill state, local rows, the
+  // local CSR arrays), so any matrix type with MatrixTraits works here.
   void sync(const Problem& p, const ParameterList& pl) {
     const Matrix& A = *p.getOperator();
-    MCLS_INSIST(A.isFillComplete(), "operator must be fill-complete");
+    MCLS_INSIST(MT::isFillComplete(A), "operator must be fill-complete");
+    const int64_t n = (int64_t)MT::getLocalNumRows(A);
     if (d_order_dirty) {
       // a new numbering drops the device operator and what was set for it
-      MCLS_INSIST(d_order.empty() || (int64_t)d_order.size() == A.numRows(), "state order has the wrong length");
+      MCLS_INSIST(d_order.empty() || (int64_t)d_order.size() == n, "state order has the wrong length");
       GpuContext::check(mcls_gpu_set_state_order(d_ctx->get(), (int64_t)d_order.size(),
                                                  d_order.empty() ? nullptr : d_order.data(), MCLS_HOST));
       d_op = nullptr;
       d_order_dirty = false;
     }
     if (d_op != p.getOperator().get()) {
-      const void* val = A.values().data();
-      int dt = std::is_same<typename Matrix::scalar_type, float>::value ? MCLS_F32 : MCLS_F64;
-      GpuContext::check(mcls_gpu_set_matrix_csr(d_ctx->get(), A.numRows(), A.nnz(), A.rowPtr().data(),
-                                                A.colInd().data(), val, dt, MCLS_HOST));
+      const LocalCrs c = MT::getLocalCrs(A);
+      MCLS_INSIST(c.num_rows == n, "local CSR rows differ from getLocalNumRows");
+      GpuContext::check(mcls_gpu_set_matrix_csr(d_ctx->get(), c.num_rows, c.nnz, c.row_ptr, c.col, c.val,
+                                                c.f32 ? MCLS_F32 : MCLS_F64, MCLS_HOST));
       d_op = p.getOperator().get();
       d_pl = nullptr;
       d_pr = nullptr;
@@ -178,14 +244,15 @@ class DeviceProblem {
     if (p.isLeftPrec()) {
       const Matrix& PL = *p.getLeftPrec();
       if (d_pl != &PL) {
-        MCLS_INSIST(PL.isFillComplete() && PL.numRows() == A.numRows(), "left preconditioner must match the operator");
+        MCLS_INSIST(MT::isFillComplete(PL) && (int64_t)MT::getLocalNumRows(PL) == n,
+                    "left preconditioner must match the operator");
+        const LocalCrs c = MT::getLocalCrs(PL);
         std::vector<double> d;
-        if (PL.isDiagonal(&d)) {
+        if (crsIsDiagonal(c, &d)) {
           GpuContext::check(mcls_gpu_set_left_prec_diag(d_ctx->get(), d.data(), MCLS_HOST));
         } else {
-          const std::vector<double> v = values64(PL);
-          GpuContext::check(mcls_gpu_set_left_prec_matrix(d_ctx->get(), PL.rowPtr().data(), PL.colInd().data(),
-                                                          v.data(), MCLS_HOST));
+          const std::vector<double> v = values64(c);
+          GpuContext::check(mcls_gpu_set_left_prec_matrix(d_ctx->get(), c.row_ptr, c.col, v.data(), MCLS_HOST));
         }
         d_pl = &PL;
       }
@@ -196,10 +263,11 @@ class DeviceProblem {
     if (p.isRightPrec()) {
       const Matrix& PR = *p.getRightPrec();
       if (d_pr != &PR) {
-        MCLS_INSIST(PR.isFillComplete() && PR.numRows() == A.numRows(), "right preconditioner must match the operator");
-        const std::vector<double> v = values64(PR);
-        GpuContext::check(mcls_gpu_set_right_prec_matrix(d_ctx->get(), PR.rowPtr().data(), PR.colInd().data(),
-                                                         v.data(), MCLS_HOST));
+        MCLS_INSIST(MT::isFillComplete(PR) && (int64_t)MT::getLocalNumRows(PR) == n,
+                    "right preconditioner must match the operator");
+        const LocalCrs c = MT::getLocalCrs(PR);
+        const std::vector<double> v = values64(c);
+        GpuContext::check(mcls_gpu_set_right_prec_matrix(d_ctx->get(), c.row_ptr, c.col, v.data(), MCLS_HOST));
         d_pr = &PR;
       }
     } else if (d_pr) {
@@ -211,7 +279,7 @@ class DeviceProblem {
       if (d_imp_mode == ImportanceMode::Diagonal) {
         GpuContext::check(mcls_gpu_set_importance_diagonal(d_ctx->get()));
       } else {
-        MCLS_INSIST((int64_t)d_imp.size() == A.numRows(), "importance has the wrong length");
+        MCLS_INSIST((int64_t)d_imp.size() == n, "importance has the wrong length");
         GpuContext::check(mcls_gpu_set_importance(d_ctx->get(), d_imp.data(), MCLS_HOST));
       }
     }
@@ -246,8 +314,10 @@ class DeviceProblem {
  private:
   enum class ImportanceMode { None, Diagonal, Explicit };
   RCP<GpuContext> d_ctx;
-  static std::vector<double> values64(const Matrix& M) {
-    return std::vector<double>(M.values().begin(), M.values().end());
+  static std::vector<double> values64(const LocalCrs& c) {
+    std::vector<double> v((size_t)c.nnz);
+    for (int64_t k = 0; k < c.nnz; ++k) v[(size_t)k] = c.value(k);
+    return v;
   }
   const void* d_op = nullptr;
   const void* d_pl = nullptr;
@@ -310,15 +380,27 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
     MCLS_INSIST(params, "null parameter list");
     d_plist = params;
   }
-  // x = MC estimate of the solution for the source omega PL b.
+  // x = MC estimate of the solution for the source omega PL b, and x = PR y
+  // when right preconditioned.  An internal solver (the one MCSA drives)
+  // takes its right-hand side as already preconditioned: source omega b, no
+  // recovery (src/MCLS_MonteCarloSolverManager_impl.hpp:200-235,347-380).
   bool solve() override {
     MCLS_INSIST(d_problem, "no problem set");
     d_dev.sync(*d_problem, *d_plist);
     GpuContext::check(mcls_gpu_set_param_str(d_dev.ctx(), "Monte Carlo Type", kMethod));
     Vector& x = *d_problem->getLHS();
-    GpuContext::check(mcls_gpu_mc_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
+    if (d_internal) {
+      const double omega = d_plist->get<double>("Neumann Relaxation", 1.0);
+      auto src = VT::deepCopy(*d_problem->getRHS());
+      VT::scale(*src, omega);
+      GpuContext::check(mcls_gpu_mc_solve_raw(d_dev.ctx(), VT::view(*src), d_internal_solves++, VT::viewNonConst(x)));
+    } else {
+      GpuContext::check(
+          mcls_gpu_mc_solve(d_dev.ctx(), VT::view(*d_problem->getRHS()), VT::viewNonConst(x), MCLS_HOST));
+    }
     return true;  // a direct solve (src/MCLS_MonteCarloSolverManager.hpp:170-171)
   }
+  bool isInternalSolver() const { return d_internal; }
   bool getConvergedStatus() const override { return true; }
   mcls_gpu_stats deviceStats() const { return d_dev.stats(); }
   // Learned-proposal importance (no counterpart in the reference): the
@@ -333,6 +415,7 @@ class MonteCarloSolverManager : public SolverManager<Vector, Matrix> {
   RCP<Problem> d_problem;
   RCP<ParameterList> d_plist;
   bool d_internal;
+  uint64_t d_internal_solves = 0;  // solve numbers (random streams) of an internal solver
   detail::DeviceProblem<Vector, Matrix> d_dev;
 };
 
@@ -436,32 +519,66 @@ class SolverFactory {
   }
 };
 
-// Point Jacobi: PL = diag(1/a_ii) as a diagonal CrsMatrix.
-template <class Scalar>
-class PointJacobiPreconditioner {
+// src/MCLS_Preconditioner.hpp:59-101: the preconditioner plugin interface.
+template <class Matrix>
+class Preconditioner {
  public:
-  void setOperator(const RCP<const CrsMatrix<Scalar>>& A) { d_A = A; }
-  void buildPreconditioner() {
-    MCLS_INSIST(d_A && d_A->isFillComplete(), "operator must be set and fill-complete");
-    const int64_t n = d_A->numRows();
-    Vector d(n);
-    MatrixTraits<Vector, CrsMatrix<Scalar>>::getLocalDiagCopy(*d_A, d);
+  using matrix_type = Matrix;
+  virtual ~Preconditioner() = default;
+  virtual RCP<const ParameterList> getValidParameters() const = 0;
+  virtual RCP<const ParameterList> getCurrentParameters() const = 0;
+  virtual void setParameters(const RCP<ParameterList>& params) = 0;
+  virtual void setOperator(const RCP<const Matrix>& A) = 0;
+  virtual const Matrix& getOperator() const = 0;
+  virtual void buildPreconditioner() = 0;
+  virtual RCP<const Matrix> getLeftPreconditioner() const = 0;
+  virtual RCP<const Matrix> getRightPreconditioner() const = 0;
+};
+
+// Point Jacobi (tpetra/src/MCLS_TpetraPointJacobiPreconditioner_impl.hpp:
+// 105-139): PL = diag(1/a_ii) as a diagonal CrsMatrix, no right
+// preconditioner, no parameters.  The diagonal is read through the traits.
+// (The device solver can also build it itself: mcls_gpu_set_left_prec_point_jacobi.)
+template <class Scalar>
+class PointJacobiPreconditioner : public Preconditioner<CrsMatrix<Scalar>> {
+ public:
+  using Matrix = CrsMatrix<Scalar>;
+  using MT = MatrixTraits<Vector, Matrix>;
+  PointJacobiPreconditioner() : d_plist(parameterList()) {}
+  RCP<const ParameterList> getValidParameters() const override { return parameterList(); }
+  RCP<const ParameterList> getCurrentParameters() const override { return d_plist; }
+  void setParameters(const RCP<ParameterList>& params) override {
+    MCLS_INSIST(params, "null parameter list");
+    d_plist = params;
+  }
+  void setOperator(const RCP<const Matrix>& A) override { d_A = A; }
+  const Matrix& getOperator() const override {
+    MCLS_INSIST(d_A, "no operator set");
+    return *d_A;
+  }
+  void buildPreconditioner() override {
+    MCLS_INSIST(d_A && MT::isFillComplete(*d_A), "operator must be set and fill-complete");
+    const int64_t n = MT::getLocalNumRows(*d_A);
+    auto d = MT::cloneVectorFromMatrixRows(*d_A);
+    MT::getLocalDiagCopy(*d_A, *d);
     std::vector<int64_t> rp((size_t)n + 1);
     std::vector<int32_t> col((size_t)n);
     std::vector<Scalar> val((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
-      MCLS_INSIST(d[i] != 0.0, "zero diagonal in row " + std::to_string(i));
+      MCLS_INSIST((*d)[i] != 0.0, "zero diagonal in row " + std::to_string(i));
       rp[(size_t)i + 1] = i + 1;
       col[(size_t)i] = (int32_t)i;
-      val[(size_t)i] = (Scalar)(1.0 / d[i]);
+      val[(size_t)i] = (Scalar)(1.0 / (*d)[i]);
     }
-    d_P = std::make_shared<CrsMatrix<Scalar>>(n, rp, col, val);
+    d_P = std::make_shared<Matrix>(n, rp, col, val);
   }
-  RCP<const CrsMatrix<Scalar>> getLeftPreconditioner() const { return d_P; }
+  RCP<const Matrix> getLeftPreconditioner() const override { return d_P; }
+  RCP<const Matrix> getRightPreconditioner() const override { return nullptr; }
 
  private:
-  RCP<const CrsMatrix<Scalar>> d_A;
-  RCP<const CrsMatrix<Scalar>> d_P;
+  RCP<ParameterList> d_plist;
+  RCP<const Matrix> d_A;
+  RCP<const Matrix> d_P;
 };
 
 }  // namespace MCLS

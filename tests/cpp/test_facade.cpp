@@ -453,7 +453,274 @@ static void test_state_order() {
   std::printf("ok state order: max diff %.2e / %.2e of %.2e\n", d1, d2, nx);
 }
 
+// ---------------------------------------------------------------------------
+// A second matrix type, shaped like Tpetra::CrsMatrix / Tpetra::Vector: rows
+// inserted by global index, fillComplete, and the local rows read back as a
+// (row_map, entries, values) view the way getLocalMatrix() returns Kokkos'
+// CRS arrays.  The library knows nothing of it: the two specialisations
+// below are the whole binding, what a Tpetra build would add.
+namespace mock {
+
+struct LocalMatrix {
+  std::vector<int64_t> row_map;
+  std::vector<int32_t> entries;
+  std::vector<double> values;
+};
+
+class TpetraLikeCrsMatrix {
+ public:
+  explicit TpetraLikeCrsMatrix(int64_t n) : d_rows((size_t)n) {}
+  void insertGlobalValues(int64_t row, const std::vector<int64_t>& cols, const std::vector<double>& vals) {
+    for (size_t k = 0; k < cols.size(); ++k) d_rows[(size_t)row].emplace_back(cols[k], vals[k]);
+  }
+  void fillComplete() {
+    d_local.row_map.assign(1, 0);
+    for (auto& r : d_rows) {
+      std::sort(r.begin(), r.end());
+      for (size_t k = 0; k < r.size(); ++k) {
+        if (k && r[k].first == r[k - 1].first) {
+          d_local.values.back() += r[k].second;
+          continue;
+        }
+        d_local.entries.push_back((int32_t)r[k].first);
+        d_local.values.push_back(r[k].second);
+      }
+      d_local.row_map.push_back((int64_t)d_local.entries.size());
+    }
+    d_filled = true;
+  }
+  bool isFillComplete() const { return d_filled; }
+  int64_t getNodeNumRows() const { return (int64_t)d_rows.size(); }
+  const LocalMatrix& getLocalMatrix() const { return d_local; }
+
+ private:
+  std::vector<std::vector<std::pair<int64_t, double>>> d_rows;
+  LocalMatrix d_local;
+  bool d_filled = false;
+};
+
+class TpetraLikeVector {
+ public:
+  explicit TpetraLikeVector(int64_t n) : d_v((size_t)n, 0.0) {}
+  int64_t getLocalLength() const { return (int64_t)d_v.size(); }
+  const double* getData() const { return d_v.data(); }
+  double* getDataNonConst() { return d_v.data(); }
+
+ private:
+  std::vector<double> d_v;
+};
+
+}  // namespace mock
+
+namespace MCLS {
+template <>
+class VectorTraits<mock::TpetraLikeVector> {
+ public:
+  using V = mock::TpetraLikeVector;
+  static RCP<V> clone(const V& v) { return std::make_shared<V>(v.getLocalLength()); }
+  static RCP<V> deepCopy(const V& v) { return std::make_shared<V>(v); }
+  static int64_t getLocalLength(const V& v) { return v.getLocalLength(); }
+  static const double* view(const V& v) { return v.getData(); }
+  static double* viewNonConst(V& v) { return v.getDataNonConst(); }
+  static void putScalar(V& v, double a) { std::fill(v.getDataNonConst(), v.getDataNonConst() + v.getLocalLength(), a); }
+  static void scale(V& v, double a) {
+    for (int64_t i = 0; i < v.getLocalLength(); ++i) v.getDataNonConst()[i] *= a;
+  }
+  static void update(V& A, double alpha, const V& B, double beta) {
+    for (int64_t i = 0; i < A.getLocalLength(); ++i)
+      A.getDataNonConst()[i] = (alpha == 0.0 ? 0.0 : alpha * A.getData()[i]) + beta * B.getData()[i];
+  }
+  static double normInf(const V& v) {
+    double m = 0.0;
+    for (int64_t i = 0; i < v.getLocalLength(); ++i) m = std::max(m, std::fabs(v.getData()[i]));
+    return m;
+  }
+};
+
+template <>
+class MatrixTraits<mock::TpetraLikeVector, mock::TpetraLikeCrsMatrix> {
+ public:
+  using M = mock::TpetraLikeCrsMatrix;
+  using V = mock::TpetraLikeVector;
+  static RCP<V> cloneVectorFromMatrixRows(const M& A) { return std::make_shared<V>(A.getNodeNumRows()); }
+  static bool isFillComplete(const M& A) { return A.isFillComplete(); }
+  static int64_t getLocalNumRows(const M& A) { return A.getNodeNumRows(); }
+  static void getLocalRowCopy(const M& A, int64_t row, std::vector<int64_t>& cols, std::vector<double>& vals,
+                              size_t& n) {
+    const auto& L = A.getLocalMatrix();
+    n = (size_t)(L.row_map[(size_t)row + 1] - L.row_map[(size_t)row]);
+    cols.assign(L.entries.begin() + L.row_map[(size_t)row], L.entries.begin() + L.row_map[(size_t)row + 1]);
+    vals.assign(L.values.begin() + L.row_map[(size_t)row], L.values.begin() + L.row_map[(size_t)row + 1]);
+  }
+  static void apply(const M& A, const V& x, V& y) {
+    const auto& L = A.getLocalMatrix();
+    for (int64_t i = 0; i < A.getNodeNumRows(); ++i) {
+      double s = 0.0;
+      for (int64_t k = L.row_map[(size_t)i]; k < L.row_map[(size_t)i + 1]; ++k)
+        s += L.values[(size_t)k] * x.getData()[L.entries[(size_t)k]];
+      y.getDataNonConst()[i] = s;
+    }
+  }
+  // the local CRS arrays as they are (Tpetra: getLocalMatrix().graph.row_map, entries, values)
+  static LocalCrs getLocalCrs(const M& A) {
+    const auto& L = A.getLocalMatrix();
+    LocalCrs c;
+    c.num_rows = A.getNodeNumRows();
+    c.nnz = (int64_t)L.entries.size();
+    c.row_ptr = L.row_map.data();
+    c.col = L.entries.data();
+    c.val = L.values.data();
+    return c;
+  }
+};
+}  // namespace MCLS
+
+// a Preconditioner implementation for the second type: point Jacobi
+class MockPointJacobi : public Preconditioner<mock::TpetraLikeCrsMatrix> {
+ public:
+  using M = mock::TpetraLikeCrsMatrix;
+  RCP<const ParameterList> getValidParameters() const override { return parameterList(); }
+  RCP<const ParameterList> getCurrentParameters() const override { return parameterList(); }
+  void setParameters(const RCP<ParameterList>&) override {}
+  void setOperator(const RCP<const M>& A) override { d_A = A; }
+  const M& getOperator() const override { return *d_A; }
+  void buildPreconditioner() override {
+    const auto& L = d_A->getLocalMatrix();
+    auto P = std::make_shared<M>(d_A->getNodeNumRows());
+    for (int64_t i = 0; i < d_A->getNodeNumRows(); ++i)
+      for (int64_t k = L.row_map[(size_t)i]; k < L.row_map[(size_t)i + 1]; ++k)
+        if (L.entries[(size_t)k] == i) P->insertGlobalValues(i, {i}, {1.0 / L.values[(size_t)k]});
+    P->fillComplete();
+    d_P = P;
+  }
+  RCP<const M> getLeftPreconditioner() const override { return d_P; }
+  RCP<const M> getRightPreconditioner() const override { return nullptr; }
+
+ private:
+  RCP<const M> d_A, d_P;
+};
+
+// Both managers on the second matrix type give what they give on the
+// facade's own CrsMatrix for the same operator, right-hand side and streams.
+static void test_second_matrix_type() {
+  using MV = mock::TpetraLikeVector;
+  using MM = mock::TpetraLikeCrsMatrix;
+  const int n = 30;
+  auto A1 = build_operator(n, 2.0, -0.5);
+  auto A2 = std::make_shared<MM>(n);
+  for (int64_t i = 0; i < n; ++i) {
+    std::vector<int64_t> c;
+    std::vector<double> v;
+    for (int64_t k = A1->rowPtr()[(size_t)i]; k < A1->rowPtr()[(size_t)i + 1]; ++k) {
+      c.push_back(A1->colInd()[(size_t)k]);
+      v.push_back(A1->values()[(size_t)k]);
+    }
+    A2->insertGlobalValues(i, c, v);
+  }
+  A2->fillComplete();
+  auto plist = parameterList();
+  plist->set<double>("Convergence Tolerance", 1.0e-10);
+  plist->set<int>("Maximum Iterations", 40);
+  plist->set<double>("Sample Ratio", 50.0);
+  plist->set<int>("Random Seed", 77);
+  // the facade's own types
+  auto b1 = MT::cloneVectorFromMatrixRows(*A1);
+  for (int i = 0; i < n; ++i) (*b1)[i] = 1.0 - 0.02 * i;
+  auto x1 = MT::cloneVectorFromMatrixRows(*A1);
+  auto p1 = std::make_shared<LinearProblem<VectorType, MatrixType>>(A1, x1, b1);
+  PointJacobiPreconditioner<double> pj1;
+  pj1.setOperator(A1);
+  pj1.buildPreconditioner();
+  p1->setLeftPrec(pj1.getLeftPreconditioner());
+  MCSASolverManager<VectorType, MatrixType, AdjointTag> s1(p1, plist);
+  CHECK(s1.solve());
+  // the mock types, touched by the library only through the traits
+  using MTM = MatrixTraits<MV, MM>;
+  auto b2 = MTM::cloneVectorFromMatrixRows(*A2);
+  for (int i = 0; i < n; ++i) b2->getDataNonConst()[i] = 1.0 - 0.02 * i;
+  auto x2 = MTM::cloneVectorFromMatrixRows(*A2);
+  auto p2 = std::make_shared<LinearProblem<MV, MM>>(A2, x2, b2);
+  MockPointJacobi pj2;
+  pj2.setOperator(A2);
+  pj2.buildPreconditioner();
+  p2->setLeftPrec(pj2.getLeftPreconditioner());
+  MCSASolverManager<MV, MM, AdjointTag> s2(p2, plist);
+  CHECK(s2.solve() && s2.achievedTol() <= 1e-10);
+  CHECK(std::abs(s2.getNumIters() - s1.getNumIters()) <= 1);
+  double d = 0.0, nx = 0.0;
+  for (int i = 0; i < n; ++i) {
+    d = std::max(d, std::fabs(x2->getData()[i] - (*x1)[i]));
+    nx = std::max(nx, std::fabs((*x1)[i]));
+  }
+  CHECK(d <= 1e-8 * nx);
+  // the host residual of the second type (updatePrecResidual through MT::apply)
+  p2->updatePrecResidual();
+  CHECK(VectorTraits<MV>::normInf(*p2->getPrecResidual()) <= 1e-8 * VectorTraits<MV>::normInf(*b2));
+  // the MC manager (same source, same solve number on a fresh context)
+  auto y1 = MT::cloneVectorFromMatrixRows(*A1);
+  auto y2 = MTM::cloneVectorFromMatrixRows(*A2);
+  auto q1 = std::make_shared<LinearProblem<VectorType, MatrixType>>(A1, y1, b1);
+  auto q2 = std::make_shared<LinearProblem<MV, MM>>(A2, y2, b2);
+  q1->setLeftPrec(pj1.getLeftPreconditioner());
+  q2->setLeftPrec(pj2.getLeftPreconditioner());
+  MonteCarloSolverManager<VectorType, MatrixType, AdjointTag> m1(q1, plist);
+  MonteCarloSolverManager<MV, MM, AdjointTag> m2(q2, plist);
+  CHECK(m1.solve() && m2.solve());
+  d = 0.0;
+  nx = 0.0;
+  for (int i = 0; i < n; ++i) {
+    d = std::max(d, std::fabs(y2->getData()[i] - (*y1)[i]));
+    nx = std::max(nx, std::fabs((*y1)[i]));
+  }
+  CHECK(d <= 1e-12 * nx);
+  std::printf("ok second matrix type: MCSA %d / %d iterations, solution diff %.2e\n", s2.getNumIters(),
+              s1.getNumIters(), d);
+}
+
+// An internal MC solver (MCSA's inner one) takes its right-hand side as
+// already preconditioned: omega b, no left preconditioning of the source
+// (src/MCLS_MonteCarloSolverManager_impl.hpp:347-380).
+static void test_internal_mc_solver() {
+  const int n = 20;
+  auto A = build_operator(n, 2.0, -0.5);
+  PointJacobiPreconditioner<double> pj;
+  pj.setOperator(A);
+  pj.buildPreconditioner();
+  auto b = MT::cloneVectorFromMatrixRows(*A);
+  for (int i = 0; i < n; ++i) (*b)[i] = 0.5 + 0.1 * i;
+  auto plist = parameterList();
+  plist->set<double>("Sample Ratio", 200.0);
+  plist->set<int>("Random Seed", 9);
+  plist->set<double>("Neumann Relaxation", 1.0);
+  // internal: source b (taken as preconditioned); external with PL = I: source b
+  auto xi = MT::cloneVectorFromMatrixRows(*A), xe = MT::cloneVectorFromMatrixRows(*A);
+  auto pi = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, xi, b);
+  pi->setLeftPrec(pj.getLeftPreconditioner());
+  MonteCarloSolverManager<VectorType, MatrixType, AdjointTag> si(pi, plist, 0, true);
+  CHECK(si.isInternalSolver() && si.solve());
+  // the same operator's domain with the source PL^-1 b makes the external
+  // solver's source PL PL^-1 b = b
+  auto bs = MT::cloneVectorFromMatrixRows(*A);
+  Vector d(n);
+  MT::getLocalDiagCopy(*A, d);
+  for (int i = 0; i < n; ++i) (*bs)[i] = (*b)[i] * d[i];
+  auto pe = std::make_shared<LinearProblem<VectorType, MatrixType>>(A, xe, bs);
+  pe->setLeftPrec(pj.getLeftPreconditioner());
+  MonteCarloSolverManager<VectorType, MatrixType, AdjointTag> se(pe, plist);
+  CHECK(se.solve());
+  double diff = 0.0, nx = 0.0;
+  for (int i = 0; i < n; ++i) {
+    diff = std::max(diff, std::fabs((*xi)[i] - (*xe)[i]));
+    nx = std::max(nx, std::fabs((*xe)[i]));
+  }
+  // same source up to the rounding of b d / d; same streams (solve 0)
+  CHECK(diff <= 1e-12 * nx && nx > 0.0);
+  std::printf("ok internal MC solver: diff %.2e of %.2e\n", diff, nx);
+}
+
 int main() {
+  test_second_matrix_type();
+  test_internal_mc_solver();
   test_mcsa_adjoint(false);
   test_mcsa_adjoint(true);
   test_mc_adjoint();
