@@ -1,14 +1,17 @@
 // Importance network of the learned proposal (include/mcls_mlp.h): forward,
-// backward and Adam kernels for gfx950.  Matrix products use
-// v_mfma_f32_16x16x16_bf16 (bf16 operands, FP32 accumulation) on 16-row
-// tiles staged in LDS; one wave owns one tile at a time.
+// backward and Adam kernels for gfx950.  Matrix products use bf16 MFMA with
+// FP32 accumulation on 16-row tiles staged in LDS; one wave owns one tile at
+// a time.  Products over the 64 hidden units (K = 64) use gfx950's
+// v_mfma_f32_16x16x32_bf16 (two per 16 x 16 output tile); the input layer
+// (K = 16 padded inputs) and the weight gradients (K = the tile's 16 rows)
+// use v_mfma_f32_16x16x16_bf16.
 //
-// MFMA 16x16x16 operand layout (lane l of 64):
-//   A (16 x K-chunk 16): row l % 16, k = 4 (l / 16) + v, v = 0..3
-//   B (K-chunk 16 x 16): col l % 16, k = 4 (l / 16) + v
-//   D (16 x 16):         col l % 16, row = 4 (l / 16) + v
+// MFMA operand layouts (lane l of 64), K-chunk of 16 or 32:
+//   A (16 x K): row l % 16, k = (K / 4) (l / 16) + v, v = 0 .. K/4 - 1
+//   B (K x 16): col l % 16, k = (K / 4) (l / 16) + v
+//   D (16 x 16): col l % 16, row = 4 (l / 16) + v, v = 0..3
 // The GPU tests check forward outputs and gradients against the CPU oracle,
-// which pins this layout.
+// which pins these layouts.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +33,7 @@ static_assert(NPARAM == MCLS_MLP_PARAMS, "parameter layout");
 constexpr int WAVES = 4, THREADS = 64 * WAVES, TILE = 16;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> bfloat16, round to nearest even (finite inputs)
@@ -65,8 +69,21 @@ __device__ __forceinline__ s4 frag_col(const unsigned short* S, int ld, int r0, 
   return v;
 }
 
+// Fragment of eight consecutive k of M[r][k] stored row-major: rows
+// r0 + l%16, k = k0 + 8 (l/16) + v (the 16 x 32 operands).
+__device__ __forceinline__ bf8 frag_row8(const unsigned short* T, int ld, int r0, int k0, int lane) {
+  const unsigned short* p = T + (r0 + (lane & 15)) * ld + k0 + 8 * (lane >> 4);
+  bf8 v;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = __builtin_bit_cast(__bf16, p[e]);
+  return v;
+}
+
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 struct Weights {
@@ -116,8 +133,13 @@ __device__ void dense_relu(const unsigned short* in, const unsigned short* wt, c
   for (int nt = 0; nt < HID / 16; ++nt) {
     const float bj = b[nt * 16 + (lane & 15)];
     f4 acc = {bj, bj, bj, bj};
-    for (int kt = 0; kt < K / 16; ++kt)
-      acc = mfma(frag_row(in, K, 0, kt * 16, lane), frag_row(wt, K, nt * 16, kt * 16, lane), acc);
+    if constexpr (K % 32 == 0) {
+      for (int kt = 0; kt < K / 32; ++kt)
+        acc = mfma32(frag_row8(in, K, 0, kt * 32, lane), frag_row8(wt, K, nt * 16, kt * 32, lane), acc);
+    } else {
+      for (int kt = 0; kt < K / 16; ++kt)
+        acc = mfma(frag_row(in, K, 0, kt * 16, lane), frag_row(wt, K, nt * 16, kt * 16, lane), acc);
+    }
     for (int v = 0; v < 4; ++v) out[(4 * (lane >> 4) + v) * HID + nt * 16 + (lane & 15)] = f2bf(fmaxf(acc[v], 0.f));
   }
 }
@@ -163,8 +185,8 @@ __device__ void weight_grad(const unsigned short* A, const unsigned short* D, fl
 __device__ void input_grad(const unsigned short* dZ, const unsigned short* Wrm, f4 da[4], int lane) {
   for (int nt = 0; nt < HID / 16; ++nt) {
     f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kt = 0; kt < HID / 16; ++kt)
-      acc = mfma(frag_row(dZ, HID, 0, kt * 16, lane), frag_row(Wrm, HID, nt * 16, kt * 16, lane), acc);
+    for (int kt = 0; kt < HID / 32; ++kt)
+      acc = mfma32(frag_row8(dZ, HID, 0, kt * 32, lane), frag_row8(Wrm, HID, nt * 16, kt * 32, lane), acc);
     da[nt] = acc;
   }
 }
@@ -254,9 +276,29 @@ __global__ __launch_bounds__(THREADS) void k_mlp_forward(const float* __restrict
   }
 }
 
+__device__ void grad_body(const float* __restrict__ P, const float* __restrict__ X, const float* __restrict__ T,
+                          int64_t rows, float inv_b, float* __restrict__ grad, double* loss);
+
 __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float* __restrict__ P, const float* __restrict__ X,
                                                       const float* __restrict__ T, int64_t rows, float inv_b,
                                                       float* __restrict__ grad, double* loss) {
+  grad_body(P, X, T, rows, inv_b, grad, loss);
+}
+
+// The gradient of minibatch order[bi] of the training set (rows
+// order[bi] * batch .. + batch), the minibatch found on the device so that
+// an epoch's launches can be replayed from a graph with a new order.
+__global__ __launch_bounds__(THREADS) void k_mlp_grad_batch(const float* __restrict__ P, const float* __restrict__ X,
+                                                            const float* __restrict__ T, int64_t total,
+                                                            int64_t batch, const int64_t* __restrict__ order, int64_t bi,
+                                                            float* __restrict__ grad, double* loss) {
+  const int64_t r0 = order[bi] * batch;
+  const int64_t rows = total - r0 < batch ? total - r0 : batch;
+  grad_body(P, X + IN * r0, T + r0, rows, 1.0f / (float)rows, grad, loss);
+}
+
+__device__ void grad_body(const float* __restrict__ P, const float* __restrict__ X, const float* __restrict__ T,
+                          int64_t rows, float inv_b, float* __restrict__ grad, double* loss) {
   __shared__ Weights W;
   __shared__ WaveBuf B[WAVES];
   __shared__ Grads G;
@@ -294,17 +336,31 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const float* __restrict__ 
 }
 
 // Adam (Kingma & Ba 2015) with bias correction, FP32 master parameters.
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, int n, float lr, float c1, float c2) {
+// One Adam step by a single workgroup, its step number kept on the device
+// (bias corrections from it), the gradient zeroed once read so that the next
+// minibatch accumulates into a clear buffer.
+__global__ __launch_bounds__(1024) void k_adam_step(float* __restrict__ p, float* __restrict__ g,
+                                                    float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                                                    int64_t* step) {
+  __shared__ float c1, c2;
+  if (threadIdx.x == 0) {
+    const float st = (float)(*step + 1);
+    c1 = 1.f - powf(0.9f, st);
+    c2 = 1.f - powf(0.999f, st);
+  }
+  __syncthreads();
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const float gi = g[i];
+    g[i] = 0.f;
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
     p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) *step += 1;
 }
 
 template <typename T>
@@ -344,6 +400,15 @@ struct mcls_mlp {
   float* scratch_T = nullptr;
   float* scratch_o = nullptr;
   int64_t scratch_rows = 0;
+  // training: one epoch of (gradient, Adam step) launches captured in a
+  // hipGraph and replayed with each epoch's minibatch order
+  int64_t* step_dev = nullptr;  // Adam step count
+  int64_t* order_dev = nullptr;
+  int64_t order_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_rows = -1, graph_batch = -1;
+  float graph_lr = 0.f;
 };
 
 namespace mcls {
@@ -390,6 +455,41 @@ int grad_batch(mcls_mlp* m, const float* X, const float* T, int64_t rows) {
 
 }  // namespace
 
+namespace {
+
+void drop_graph(mcls_mlp* m) {
+  if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->graph_exec = nullptr;
+  m->graph = nullptr;
+  m->graph_rows = m->graph_batch = -1;
+}
+
+// Capture one epoch: nb minibatch gradients, each followed by its Adam step.
+int capture_epoch(mcls_mlp* m, int64_t nb, int64_t batch_rows, float lr) {
+  drop_graph(m);
+  const int64_t tiles = (batch_rows + TILE - 1) / TILE;
+  int64_t g = (tiles + WAVES - 1) / WAVES;
+  if (g > 128) g = 128;
+  MLP_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+  for (int64_t bi = 0; bi < nb; ++bi) {
+    k_mlp_grad_batch<<<(unsigned)g, THREADS, 0, m->stream>>>(m->params, m->X, m->T, m->rows, batch_rows, m->order_dev,
+                                                             bi, m->grad, m->loss);
+    k_adam_step<<<1, 1024, 0, m->stream>>>(m->params, m->grad, m->m1, m->m2, NPARAM, lr, m->step_dev);
+  }
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(m->stream, &gr);
+  MLP_HIP(e);
+  m->graph = gr;
+  MLP_HIP(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+  m->graph_rows = m->rows;
+  m->graph_batch = batch_rows;
+  m->graph_lr = lr;
+  return MCLS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mcls_mlp_create(mcls_mlp** out, int device, uint64_t seed) {
@@ -406,7 +506,8 @@ int mcls_mlp_create(mcls_mlp** out, int device, uint64_t seed) {
       hipMalloc((void**)&m->grad, sizeof(float) * NPARAM) != hipSuccess ||
       hipMalloc((void**)&m->m1, sizeof(float) * NPARAM) != hipSuccess ||
       hipMalloc((void**)&m->m2, sizeof(float) * NPARAM) != hipSuccess ||
-      hipMalloc((void**)&m->loss, sizeof(double)) != hipSuccess) {
+      hipMalloc((void**)&m->loss, sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&m->step_dev, sizeof(int64_t)) != hipSuccess) {
     mcls_mlp_destroy(m);
     return mcls::set_error(MCLS_ERR_HIP, "mcls_mlp_create: allocation failed");
   }
@@ -438,8 +539,10 @@ int mcls_mlp_destroy(mcls_mlp* m) {
   if (!m) return MCLS_OK;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  drop_graph(m);
   for (void* p : {(void*)m->params, (void*)m->grad, (void*)m->m1, (void*)m->m2, (void*)m->loss, (void*)m->X,
-                  (void*)m->T, (void*)m->scratch_X, (void*)m->scratch_T, (void*)m->scratch_o})
+                  (void*)m->T, (void*)m->scratch_X, (void*)m->scratch_T, (void*)m->scratch_o, (void*)m->step_dev,
+                  (void*)m->order_dev})
     if (p) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -462,6 +565,7 @@ int mcls_mlp_set_params(mcls_mlp* m, const float* params) {
   MLP_HIP(hipMemcpyAsync(m->params, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice, m->stream));
   k_zero<float><<<64, 256, 0, m->stream>>>(m->m1, NPARAM);
   k_zero<float><<<64, 256, 0, m->stream>>>(m->m2, NPARAM);
+  k_zero<int64_t><<<1, 64, 0, m->stream>>>(m->step_dev, 1);
   MLP_HIP(hipGetLastError());
   MLP_HIP(hipStreamSynchronize(m->stream));
   m->step = 0;
@@ -516,6 +620,7 @@ int mcls_mlp_loss_grad(mcls_mlp* m, const float* X, const float* T, int64_t rows
 int mcls_mlp_dataset_clear(mcls_mlp* m) {
   if (!m) return mcls::set_error(MCLS_ERR_INVALID_ARG, "null network");
   m->rows = 0;
+  drop_graph(m);
   return MCLS_OK;
 }
 
@@ -544,6 +649,7 @@ int mcls_mlp_dataset_append(mcls_mlp* m, const float* X, const float* T, int64_t
     m->X = nX;
     m->T = nT;
     m->cap = cap;
+    drop_graph(m);  // the captured epoch names the old buffers
   }
   const hipMemcpyKind k = ptr_kind == MCLS_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   MLP_HIP(hipMemcpyAsync(m->X + IN * m->rows, X, sizeof(float) * IN * (size_t)rows, k, m->stream));
@@ -553,11 +659,25 @@ int mcls_mlp_dataset_append(mcls_mlp* m, const float* X, const float* T, int64_t
   return MCLS_OK;
 }
 
+
 int mcls_mlp_train(mcls_mlp* m, int epochs, int64_t batch_rows, float lr, uint64_t seed, double* final_loss) {
   if (!m || epochs < 0 || batch_rows <= 0 || !(lr > 0.f)) return mcls::set_error(MCLS_ERR_INVALID_ARG, "bad train arguments");
   if (m->rows == 0) return mcls::set_error(MCLS_ERR_STATE, "empty training set");
   MLP_HIP(hipSetDevice(m->device));
   const int64_t nb = (m->rows + batch_rows - 1) / batch_rows;
+  if (nb > m->order_cap) {
+    (void)hipFree(m->order_dev);
+    m->order_dev = nullptr;
+    MLP_HIP(hipMalloc((void**)&m->order_dev, sizeof(int64_t) * (size_t)nb));
+    m->order_cap = nb;
+    drop_graph(m);  // the graph names the old order buffer
+  }
+  if (!m->graph_exec || m->graph_rows != m->rows || m->graph_batch != batch_rows || m->graph_lr != lr) {
+    int rc = capture_epoch(m, nb, batch_rows, lr);
+    if (rc != MCLS_OK) return rc;
+  }
+  // a clear gradient before the first minibatch (each Adam step clears it after)
+  k_zero<float><<<64, 256, 0, m->stream>>>(m->grad, NPARAM);
   std::vector<int64_t> order((size_t)nb);
   uint64_t st = seed;
   double last = 0.0;
@@ -568,17 +688,10 @@ int mcls_mlp_train(mcls_mlp* m, int epochs, int64_t batch_rows, float lr, uint64
       const int64_t j = (int64_t)(splitmix64(st) % (uint64_t)(b + 1));
       std::swap(order[(size_t)b], order[(size_t)j]);
     }
+    MLP_HIP(hipMemcpyAsync(m->order_dev, order.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, m->stream));
     k_zero<double><<<1, 1, 0, m->stream>>>(m->loss, 1);
-    for (int64_t bi = 0; bi < nb; ++bi) {
-      const int64_t r0 = order[(size_t)bi] * batch_rows;
-      const int64_t rows = std::min(batch_rows, m->rows - r0);
-      int rc = grad_batch(m, m->X + IN * r0, m->T + r0, rows);
-      if (rc != MCLS_OK) return rc;
-      ++m->step;
-      const float c1 = 1.f - std::pow(0.9f, (float)m->step), c2 = 1.f - std::pow(0.999f, (float)m->step);
-      k_adam<<<(NPARAM + 255) / 256, 256, 0, m->stream>>>(m->params, m->grad, m->m1, m->m2, NPARAM, lr, c1, c2);
-      MLP_HIP(hipGetLastError());
-    }
+    MLP_HIP(hipGraphLaunch(m->graph_exec, m->stream));
+    m->step += nb;
     double l = 0.0;
     MLP_HIP(hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
     MLP_HIP(hipStreamSynchronize(m->stream));

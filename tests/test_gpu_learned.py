@@ -62,6 +62,45 @@ def test_mlp_training_reduces_loss():
     assert l2 < 0.1 * l0 and l1 < 0.2 * l0
 
 
+def test_training_epochs_match_oracle_adam():
+    """Each training epoch replays one captured graph of (gradient, Adam
+    step) launches, with the host's epoch bookkeeping (new minibatch order,
+    cleared loss, synchronisation) between replays.  With one minibatch per
+    epoch, epoch e is Adam step e on the full-batch gradient: after each of
+    four epochs the parameters follow the oracle's Adam iterations on its
+    bf16-emulated gradients."""
+    net = capi.Mlp(0, seed=21)
+    X = _inputs(2048, 6)
+    T = np.sin(X[:, 0]).astype(np.float32)
+    net.dataset_clear()
+    net.dataset_append(X, T)
+    p = net.params().astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    lr = 1e-3
+    for epoch in range(1, 5):
+        net.train(epochs=1, batch_rows=2048, lr=lr, seed=epoch)
+        _, g = mo.loss_grad(p, X, T, emulate=True)
+        p, m, v = mo.adam(p, g, m, v, epoch, lr=lr)
+        _check_adam_track(net.params().astype(np.float64), p, lr, epoch)
+    # two epochs in one call replay the graph twice
+    net.train(epochs=2, batch_rows=2048, lr=lr, seed=9)
+    for t in (5, 6):
+        _, g = mo.loss_grad(p, X, T, emulate=True)
+        p, m, v = mo.adam(p, g, m, v, t, lr=lr)
+    _check_adam_track(net.params().astype(np.float64), p, lr, 6)
+
+
+def _check_adam_track(got, want, lr, steps):
+    # Adam moves a parameter by about lr per step whatever the gradient's
+    # size, so a missed or repeated step shows on nearly every parameter;
+    # FP32 accumulation order only perturbs the moves, except where a
+    # gradient component is so near zero that its sign is noise
+    err = np.abs(got - want)
+    assert np.mean(err <= 0.05 * lr * steps) >= 0.98, np.quantile(err, [0.5, 0.98, 1.0])
+    assert np.max(err) <= 2.1 * lr * steps
+
+
 def _varcoef(n=24, seed=9):
     return capi.build_problem(capi.make_spec(capi.VARCOEF2D_5PT, n, n, seed=seed, p0=0.1, p1=1.0, p2=64))
 
