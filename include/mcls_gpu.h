@@ -276,8 +276,11 @@ int mcls_gpu_convergence_criteria(mcls_gpu_context* ctx, int64_t max_iters, doub
  * (collision estimator) the LDS tally tile: off, 128, 512 or 1024 entries
  * per wave (those that fit 64 KiB of LDS); last, the best shape of each
  * block size walks the whole source once, and the fastest of those sets
- * the block size.  `ms_best` is the pilot time of the chosen batch and
- * tile.  The winner is stored in "Walk
+ * the block size.  `ms_best` is the window pilot time of the configuration
+ * returned and `ms_confirmed` its walk of the whole source.  Shapes whose
+ * tile does not fit the LDS are skipped; an entry configuration that does
+ * not fit is not timed (`ms_default` = -1) and the search starts untiled.
+ * The winner is stored in "Walk
  * Block Size", "Walk Blocks Per CU", "Walk Batch" and "Tally Tile" and
  * reported in `out`.  Estimates do not
  * depend on the launch shape (each history's random stream depends only on
@@ -291,6 +294,7 @@ typedef struct mcls_gpu_tuning {
   double ms_default;      /* pilot walk time with the configuration on entry */
   double ms_best;         /* pilot walk time with the chosen configuration   */
   int64_t candidates;     /* configurations timed                            */
+  double ms_confirmed;    /* full-source walk time of the chosen configuration */
 } mcls_gpu_tuning;
 int mcls_gpu_autotune(mcls_gpu_context* ctx, const double* b, int ptr_kind, int64_t pilot_histories,
                       mcls_gpu_tuning* out);

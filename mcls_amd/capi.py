@@ -95,6 +95,7 @@ class Tuning(C.Structure):
         ("ms_default", C.c_double),
         ("ms_best", C.c_double),
         ("candidates", C.c_int64),
+        ("ms_confirmed", C.c_double),
     ]
 
     def as_dict(self):

@@ -95,6 +95,7 @@ struct Params {
   int64_t walk_batch = 64;
   int64_t tally_tile = 0;  // LDS tally tile entries per wave (0 = off)
   int tile_kind = mcls::kTileF64;  // tile partial sums: FP64, the walk's precision, or FP32 compensated
+  bool residual_csr = false;        // "Residual Format" = "CSR": vector-CSR residual kernel
   int64_t walks_per_lane = 1;  // 2: two histories per lane (strided alias walks)
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
@@ -122,6 +123,12 @@ struct mcls_gpu_context {
   bool has_matrix = false;
   DBuf<int> a_rp, a_col;
   DBuf<char> a_val;
+  // sliced-ELL copy of A for the residual (slice widths, entry offsets,
+  // columns, values in A's precision); sell_on when built
+  DBuf<int> sell_w, sell_col;
+  DBuf<int64_t> sell_off;
+  DBuf<char> sell_val;
+  bool sell_on = false;
   bool has_prec = false;
   DBuf<double> dinv;
   // general (sparse) left / right preconditioners: FP64 CSR, ascending rows
@@ -1024,6 +1031,61 @@ int histories_done(mcls_gpu_context* c, const SolveBufs& sb, int64_t* out) {
 }
 int histories_done(mcls_gpu_context* c, int64_t* out) { return histories_done(c, ctx_bufs(c), out); }
 
+// r = D (b - A x), D diagonal or none, by the sliced-ELL kernel when its copy
+// of A is built and selected, else by the vector-CSR one; partial sums of r^2
+// to part, *nparts of them.
+int unprec_residual(mcls_gpu_context* c, const double* dinv, const double* b, const double* x, double* r,
+                    int* nparts) {
+  const int n = c->n;
+  if (c->sell_on && !c->prm.residual_csr) {
+    HIP_TRY(mcls::launch_sell_residual(n, c->sell_w.p, c->sell_off.p, c->sell_col.p, c->sell_val.p, c->f32, dinv, b,
+                                       x, r, c->part.p, c->stream));
+    *nparts = mcls::sell_blocks((n + 63) / 64);
+  } else {
+    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, dinv, b, x, r,
+                                       c->part.p, c->stream));
+    *nparts = mcls::residual_blocks(n, c->res_lanes);
+  }
+  return MCLS_OK;
+}
+
+// Sliced-ELL copy of A (n x n, rows as stored on the device) for the
+// residual.  Skipped, leaving the CSR kernel in use, when padding the slices
+// to their longest rows would add more than a quarter to the entries or the
+// copy does not fit in device memory.
+int build_sell(mcls_gpu_context* c) {
+  c->sell_on = false;
+  const int n = c->n;
+  const int ns = (n + 63) / 64;
+  if (ns == 0) return MCLS_OK;
+  MCLS_TRY(c->sell_w.ensure((size_t)ns, "sell widths"));
+  HIP_TRY(mcls::launch_sell_widths(n, c->a_rp.p, c->sell_w.p, c->stream));
+  std::vector<int> w((size_t)ns);
+  HIP_TRY(hipMemcpyAsync(w.data(), c->sell_w.p, sizeof(int) * (size_t)ns, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<int64_t> off((size_t)ns);
+  int64_t tot = 0;
+  for (int t = 0; t < ns; ++t) {
+    off[(size_t)t] = tot;
+    tot += 64 * (int64_t)w[(size_t)t];
+  }
+  if (tot > c->nnz + c->nnz / 4 + 64) return MCLS_OK;
+  const size_t vsz = c->f32 ? sizeof(float) : sizeof(double);
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  const size_t need = (size_t)tot * (sizeof(int) + vsz) + sizeof(int64_t) * (size_t)ns;
+  if (need + (size_t)(1u << 30) > free_b) return MCLS_OK;
+  MCLS_TRY(c->sell_off.ensure((size_t)ns, "sell offsets"));
+  MCLS_TRY(c->sell_col.ensure((size_t)tot, "sell columns"));
+  MCLS_TRY(c->sell_val.ensure((size_t)tot * vsz, "sell values"));
+  HIP_TRY(hipMemcpyAsync(c->sell_off.p, off.data(), sizeof(int64_t) * (size_t)ns, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(mcls::launch_sell_fill(n, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, c->sell_w.p, c->sell_off.p,
+                                 c->sell_col.p, c->sell_val.p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->sell_on = true;
+  return MCLS_OK;
+}
+
 // r = PL (b - A PR x) (src/MCLS_LinearProblem_impl.hpp:411-442) with a
 // diagonal or sparse PL and an optional sparse PR; partial sums of r^2 go to
 // part, *nparts of them.
@@ -1036,15 +1098,13 @@ int prec_residual(mcls_gpu_context* c, const double* b, const double* x, double*
     xin = c->pt1.p;
   }
   if (c->pl_csr.on) {
-    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32, nullptr, b, xin,
-                                       c->pt2.p, c->part.p, s));
+    int np = 0;
+    MCLS_TRY(unprec_residual(c, nullptr, b, xin, c->pt2.p, &np));
     HIP_TRY(mcls::launch_csr_spmv(n, c->pl_csr.rp.p, c->pl_csr.col.p, c->pl_csr.val.p, c->pt2.p, r, s));
     HIP_TRY(mcls::launch_scaled_sq(n, nullptr, r, c->part.p, s));
     *nparts = mcls::residual_blocks(n, 1);
   } else {
-    HIP_TRY(mcls::launch_prec_residual(n, c->res_lanes, c->a_rp.p, c->a_col.p, c->a_val.p, c->f32,
-                                       c->has_prec ? c->dinv.p : nullptr, b, xin, r, c->part.p, s));
-    *nparts = mcls::residual_blocks(n, c->res_lanes);
+    MCLS_TRY(unprec_residual(c, c->has_prec ? c->dinv.p : nullptr, b, xin, r, nparts));
   }
   return MCLS_OK;
 }
@@ -1447,6 +1507,7 @@ int mcls_gpu_set_matrix_csr(mcls_gpu_context* c, int64_t n, int64_t nnz, const i
   c->f32 = val_dtype == MCLS_F32;
   c->has_matrix = true;
   c->res_lanes = mcls::residual_lanes(n, nnz);
+  MCLS_TRY(build_sell(c));
   c->nsys = 0;  // system offsets describe the previous operator
   c->has_importance = false;
   c->proposal_valid = false;
@@ -1738,6 +1799,10 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     else if (key_is(v, "Walk")) p.tile_kind = mcls::kTileWalk;
     else if (key_is(v, "Compensated")) p.tile_kind = mcls::kTileComp;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Tally Tile Precision '") + v + "'");
+  } else if (key_is(key, "Residual Format")) {
+    if (key_is(v, "Sliced ELL")) p.residual_csr = false;
+    else if (key_is(v, "CSR")) p.residual_csr = true;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Residual Format '") + v + "'");
   } else if (key_is(key, "Fixed Point Type")) {
     if (!key_is(v, "Richardson"))
       return fail(MCLS_ERR_UNSUPPORTED, std::string("Fixed Point Type '") + v + "' (only Richardson)");
@@ -2237,6 +2302,12 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   auto pilot_ms = [&](int64_t blk, int64_t bpc, int64_t bat, int64_t tile, double* ms) -> int {
     return pilot_win(blk, bpc, bat, tile, window, ms);
   };
+  // a tile of `tile` entries per wave fits the 64 KiB of LDS of a block
+  const bool tiled_est = saved.estimator == mcls::EST_COLLISION && !saved.forward;
+  const size_t tile_elem = mcls::tile_entry_bytes(tile_kind_of(saved), saved.precision == 32 ? 4 : 8);
+  auto tile_fits = [&](int64_t blk, int64_t tile) {
+    return !tiled_est || tile <= 0 || (size_t)(blk / 64) * (size_t)tile * tile_elem <= 65536;
+  };
   int rc = MCLS_OK;
   mcls_gpu_tuning best{};
   best.block = saved.walk_block;
@@ -2244,12 +2315,20 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   best.batch = saved.walk_batch;
   best.tally_tile = saved.tally_tile;
   double ms = 0.0;
-  // warm-up, then the configuration on entry
-  rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
-  if (rc == MCLS_OK) rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
-  best.ms_default = ms;
-  best.ms_best = ms;
-  best.candidates = 1;
+  // warm-up, then the configuration on entry; an entry tile over the LDS
+  // limit is not timed (ms_default = -1) and the search starts untiled
+  best.candidates = 0;
+  if (tile_fits(best.block, best.tally_tile)) {
+    rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
+    if (rc == MCLS_OK) rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, best.tally_tile, &ms);
+    best.ms_default = ms;
+    best.ms_best = ms;
+    best.candidates = 1;
+  } else {
+    best.tally_tile = 0;
+    best.ms_default = -1.0;
+    best.ms_best = -1.0;
+  }
   // launch shape: 8, 16 or 32 resident waves per CU at each block size
   const int64_t blocks[3] = {64, 128, 256};
   const int64_t waves[3] = {8, 16, 32};
@@ -2258,13 +2337,14 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   for (int i = 0; i < 3 && rc == MCLS_OK; ++i)
     for (int j = 0; j < 3 && rc == MCLS_OK; ++j) {
       const int64_t bpc = waves[j] * 64 / blocks[i];
+      if (!tile_fits(blocks[i], best.tally_tile)) continue;
       rc = pilot_ms(blocks[i], bpc, best.batch, best.tally_tile, &ms);
       ++best.candidates;
       if (rc == MCLS_OK && (block_bpc[i] == 0 || ms < block_ms[i])) {
         block_ms[i] = ms;
         block_bpc[i] = bpc;
       }
-      if (rc == MCLS_OK && ms < best.ms_best) {
+      if (rc == MCLS_OK && (best.ms_best < 0.0 || ms < best.ms_best)) {
         best.ms_best = ms;
         best.block = blocks[i];
         best.blocks_per_cu = bpc;
@@ -2284,14 +2364,11 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
   }
   // LDS tally tile (adjoint collision estimator): off, 128, 512 or 1024
   // entries per wave, within 64 KiB of LDS per workgroup
-  if (saved.estimator == mcls::EST_COLLISION && !saved.forward) {
+  if (tiled_est) {
     const int64_t tiles[4] = {0, 128, 512, 1024};
     const int64_t shape_tile = best.tally_tile;
     for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
-      if (tiles[k] == shape_tile) continue;
-      const size_t elem = mcls::tile_entry_bytes(tile_kind_of(saved),
-                                                 saved.precision == 32 ? 4 : 8);
-      if ((size_t)(best.block / 64) * (size_t)tiles[k] * elem > 65536) continue;
+      if (tiles[k] == shape_tile || !tile_fits(best.block, tiles[k])) continue;
       rc = pilot_ms(best.block, best.blocks_per_cu, best.batch, tiles[k], &ms);
       ++best.candidates;
       if (rc == MCLS_OK && ms < best.ms_best) {
@@ -2308,12 +2385,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     double full_best = -1.0;
     int64_t fb = best.block, fbpc = best.blocks_per_cu;
     for (int i = 0; i < 3 && rc == MCLS_OK; ++i) {
-      if (block_bpc[i] == 0) continue;
-      if (best.tally_tile > 0 && saved.estimator == mcls::EST_COLLISION && !saved.forward) {
-        const size_t elem = mcls::tile_entry_bytes(tile_kind_of(saved),
-                                                   saved.precision == 32 ? 4 : 8);
-        if ((size_t)(blocks[i] / 64) * (size_t)best.tally_tile * elem > 65536) continue;
-      }
+      if (block_bpc[i] == 0 || !tile_fits(blocks[i], best.tally_tile)) continue;
       rc = pilot_win(blocks[i], block_bpc[i], best.batch, best.tally_tile, 1.0, &ms);
       ++best.candidates;
       if (rc == MCLS_OK && (full_best < 0.0 || ms < full_best)) {
@@ -2322,8 +2394,15 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
         fbpc = block_bpc[i];
       }
     }
+    // ms_best is the window pilot of the configuration returned
+    if (rc == MCLS_OK && (fb != best.block || fbpc != best.blocks_per_cu)) {
+      rc = pilot_ms(fb, fbpc, best.batch, best.tally_tile, &ms);
+      ++best.candidates;
+      best.ms_best = ms;
+    }
     best.block = fb;
     best.blocks_per_cu = fbpc;
+    best.ms_confirmed = full_best;
   }
   c->prm = saved;
   c->nsys = saved_nsys;

@@ -30,6 +30,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kRedThreads = 256;
+constexpr int kWave = 64;  // gfx950 wavefront
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
   z += kGolden;
@@ -1866,6 +1867,152 @@ __global__ __launch_bounds__(kRedThreads) void k_prec_residual(int n, const int*
   if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// ------------------------------------------------ sliced-ELL residual
+// A copy of A for the residual: slices of 64 consecutive rows (one wave),
+// slice t of width w_t = its longest row, stored entry-major from off[t]:
+// entry k of the slice's lane l at off[t] + 64 k + l, column -1 (value 0)
+// past the row's end.  A wave then reads 256 contiguous bytes of columns and
+// of FP32 values per entry, and sums each row left to right, so r is the
+// same sum as the CSR one in stored order.
+__global__ void k_sell_widths(int n, const int* __restrict__ rp, int* __restrict__ width) {
+  const int t = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int i = t * kWave + lane;
+  int len = i < n ? rp[i + 1] - rp[i] : 0;
+  for (int off = kWave / 2; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off));
+  if (lane == 0 && t * kWave < n) width[t] = len;
+}
+
+template <typename AV>
+__global__ void k_sell_fill(int n, int nslices, const int* __restrict__ rp, const int* __restrict__ col,
+                            const AV* __restrict__ val, const int* __restrict__ width,
+                            const int64_t* __restrict__ off, int* __restrict__ scol, AV* __restrict__ sval) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = (int)(g / kWave), lane = (int)(g % kWave);
+  if (t >= nslices) return;
+  const int i = t * kWave + lane;
+  const int beg = i < n ? rp[i] : 0, len = i < n ? rp[i + 1] - beg : 0;
+  const int w = width[t];
+  int* c = scol + off[t] + lane;
+  AV* v = sval + off[t] + lane;
+  for (int k = 0; k < w; ++k) {
+    c[(int64_t)k * kWave] = k < len ? col[beg + k] : -1;
+    v[(int64_t)k * kWave] = k < len ? val[beg + k] : (AV)0;
+  }
+}
+
+// r = PL (b - A x) over the sliced copy, one lane per row; entries in
+// rounds of U whose column and value loads are issued together
+// (non-temporal: they are read once), then the U gathers of x.  The grid is
+// a multiple of 8 and block b works on the eighth of the slices numbered
+// b mod 8 -- the XCD that round-robin dispatch gives it -- so each XCD's L2
+// gathers x over one contiguous eighth of the rows and its halo instead of
+// the whole vector.  Fixed grid, per-block partial sums of r^2.
+template <typename AV, int U>
+__global__ __launch_bounds__(kRedThreads) void k_sell_residual(int n, int nslices, const int* __restrict__ width,
+                                                               const int64_t* __restrict__ off,
+                                                               const int* __restrict__ scol,
+                                                               const AV* __restrict__ sval,
+                                                               const double* __restrict__ dinv,
+                                                               const double* __restrict__ b,
+                                                               const double* __restrict__ x,
+                                                               double* __restrict__ r, double* part) {
+  __shared__ double sm[kRedThreads];
+  constexpr int WPB = kRedThreads / kWave;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int xcd = blockIdx.x % 8, per_xcd = gridDim.x / 8, local = blockIdx.x / 8;
+  const int span = (nslices + 7) / 8;
+  const int t0 = xcd * span, t1 = min(nslices, t0 + span);
+  const int tstep = per_xcd * WPB;
+  double sq = 0.0;
+  int t = t0 + local * WPB + wave;
+  if (t < t1) {
+    // One software pipeline over all of the wave's rounds: round j's
+    // gathers are issued before round j + 1's column and value loads, also
+    // where round j + 1 opens the wave's next slice, so the stream of A
+    // never waits for the gathers' answers.  The next slice's width and
+    // offset are read a slice ahead.  All control flow is wave-uniform.
+    int w = width[t];
+    int64_t o = off[t];
+    int tn = t + tstep;
+    int wn = tn < t1 ? width[tn] : 0;
+    int64_t on = tn < t1 ? off[tn] : 0;
+    // b and PL of a slice's rows are loaded with its first round, so that
+    // finishing the slice waits for nothing issued after them
+    int i = t * kWave + lane;
+    double bi = i < n ? b[i] : 0.0, di = dinv && i < n ? dinv[i] : 1.0;
+    int cc[U];
+    AV vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = o + (int64_t)u * kWave + lane;
+      cc[u] = u < w ? __builtin_nontemporal_load(scol + e) : -1;
+      vv[u] = u < w ? __builtin_nontemporal_load(sval + e) : (AV)0;
+    }
+    int k = 0;
+    double s = 0.0;
+    while (true) {
+      double xx[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+      const bool last = k + U >= w;  // this round ends the slice
+      const int k2 = last ? 0 : k + U;
+      const int w2 = last ? (tn < t1 ? wn : 0) : w;
+      const int64_t o2 = last ? on : o;
+      int cn[U];
+      AV vn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = k2 + u;
+        const int64_t e = o2 + (int64_t)kk * kWave + lane;
+        cn[u] = kk < w2 ? __builtin_nontemporal_load(scol + e) : -1;
+        vn[u] = kk < w2 ? __builtin_nontemporal_load(sval + e) : (AV)0;
+      }
+      const int in = tn * kWave + lane;
+      double bn = 0.0, dn = 1.0;
+      if (last && tn < t1 && in < n) {
+        bn = b[in];
+        if (dinv) dn = dinv[in];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (cc[u] >= 0) s = __dadd_rn(s, __dmul_rn((double)vv[u], xx[u]));
+      if (last) {
+        if (i < n) {
+          const double d = __dadd_rn(bi, -s);
+          const double ri = dinv ? __dmul_rn(di, d) : d;
+          r[i] = ri;
+          sq = sq + ri * ri;
+        }
+        s = 0.0;
+        if (tn >= t1) break;
+        i = in;
+        bi = bn;
+        di = dn;
+        t = tn;
+        w = wn;
+        o = on;
+        tn = t + tstep;
+        wn = tn < t1 ? width[tn] : 0;
+        on = tn < t1 ? off[tn] : 0;
+      }
+      k = k2;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        cc[u] = cn[u];
+        vv[u] = vn[u];
+      }
+    }
+  }
+  sm[threadIdx.x] = sq;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
 __global__ __launch_bounds__(kRedThreads) void k_scaled_sq(int n, const double* __restrict__ dinv,
                                                            const double* __restrict__ b, double* part) {
   __shared__ double sm[kRedThreads];
@@ -2515,6 +2662,51 @@ hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col,
                                 hipStream_t s) {
   return f32 ? prec_residual_t<float>(n, lanes, rp, col, val, dinv, b, x, r, part, s)
              : prec_residual_t<double>(n, lanes, rp, col, val, dinv, b, x, r, part, s);
+}
+
+constexpr int kSellRound = 16;  // entries per lane per round of loads
+
+int sell_blocks(int nslices) {
+  // a multiple of 8 (one share per XCD), at most kResidualPartials
+  int64_t g = ((int64_t)nslices * kWave + kRedThreads - 1) / kRedThreads;
+  g = (g + 7) / 8 * 8;
+  if (g < 8) g = 8;
+  return (int)(g < kResidualPartials ? g : kResidualPartials);
+}
+
+hipError_t launch_sell_widths(int n, const int* rp, int* width, hipStream_t s) {
+  const int nslices = (n + kWave - 1) / kWave;
+  if (nslices == 0) return hipSuccess;
+  k_sell_widths<<<(nslices + 3) / 4, 4 * kWave, 0, s>>>(n, rp, width);
+  return hipGetLastError();
+}
+
+hipError_t launch_sell_fill(int n, const int* rp, const int* col, const void* val, bool f32, const int* width,
+                            const int64_t* off, int* scol, void* sval, hipStream_t s) {
+  const int nslices = (n + kWave - 1) / kWave;
+  if (nslices == 0) return hipSuccess;
+  const int64_t threads = (int64_t)nslices * kWave;
+  const int g = (int)((threads + 255) / 256);
+  if (f32)
+    k_sell_fill<float><<<g, 256, 0, s>>>(n, nslices, rp, col, (const float*)val, width, off, scol, (float*)sval);
+  else
+    k_sell_fill<double><<<g, 256, 0, s>>>(n, nslices, rp, col, (const double*)val, width, off, scol,
+                                          (double*)sval);
+  return hipGetLastError();
+}
+
+hipError_t launch_sell_residual(int n, const int* width, const int64_t* off, const int* scol, const void* sval,
+                                bool f32, const double* dinv, const double* b, const double* x, double* r,
+                                double* part, hipStream_t s) {
+  const int nslices = (n + kWave - 1) / kWave;
+  const int g = sell_blocks(nslices);
+  if (f32)
+    k_sell_residual<float, kSellRound><<<g, kRedThreads, 0, s>>>(n, nslices, width, off, scol, (const float*)sval,
+                                                                 dinv, b, x, r, part);
+  else
+    k_sell_residual<double, kSellRound><<<g, kRedThreads, 0, s>>>(n, nslices, width, off, scol,
+                                                                  (const double*)sval, dinv, b, x, r, part);
+  return hipGetLastError();
 }
 
 hipError_t launch_domain_power(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,

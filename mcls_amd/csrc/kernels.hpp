@@ -282,6 +282,18 @@ int residual_blocks(int n, int lanes);
 hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col, const void* val,
                                 bool f32, const double* dinv, const double* b, const double* x,
                                 double* r, double* part, hipStream_t s);
+// Sliced-ELL copy of A for the residual (slices of 64 rows, entry-major,
+// column -1 past a row's end): widths of the (n + 63) / 64 slices, then the
+// entries at the 64 * width exclusive prefix offsets.
+hipError_t launch_sell_widths(int n, const int* rp, int* width, hipStream_t s);
+hipError_t launch_sell_fill(int n, const int* rp, const int* col, const void* val, bool f32, const int* width,
+                            const int64_t* off, int* scol, void* sval, hipStream_t s);
+// r = PL (b - A x) over the sliced copy, each row summed left to right;
+// partial sums of r^2 per block into part (sell_blocks(slices) entries).
+int sell_blocks(int nslices);
+hipError_t launch_sell_residual(int n, const int* width, const int64_t* off, const int* scol, const void* sval,
+                                bool f32, const double* dinv, const double* b, const double* x, double* r,
+                                double* part, hipStream_t s);
 // out[0] = sum of part[0..m) in a fixed order; out_host (pinned, nullable)
 hipError_t launch_sum_partials(const double* part, int m, double* out, hipStream_t s);
 // Convergence criteria of the domain operator M (rows of H^T).

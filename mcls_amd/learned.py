@@ -57,16 +57,18 @@ def variance_mask(plain_samples, learned_samples):
     return (vl <= vp).astype(np.int8)
 
 
-def compare(ctx, rhs, reps=8, first_index=1000, net=None, diagonal=True):
+def compare(ctx, rhs, reps=8, first_index=1000, net=None, diagonal=True, samplers=("CDF", "Learned")):
     """Relative per-solve variance of plain almost-optimal sampling against
     the learned proposal (network ``net``) and the diagonal importance
-    g = |a_jj|, all on the same source and solve numbers.  Leaves the
+    g = |a_jj|, all on the same source and solve numbers.  ``samplers``
+    names the plain and the importance-reshaped "Transition Sampler" (by CDF
+    inversion, or "Alias" / "Learned Alias" for FP32 walks).  Leaves the
     context on the plain sampler with no network attached."""
     out = {}
-    ctx.set_params({"Transition Sampler": "CDF"})
+    ctx.set_params({"Transition Sampler": samplers[0]})
     plain = solve_samples(ctx, rhs, reps, first_index)
     out["plain"] = relative_variance(plain)
-    ctx.set_params({"Transition Sampler": "Learned"})
+    ctx.set_params({"Transition Sampler": samplers[1]})
     if net is not None:
         ctx.set_importance_model(net)
         ctx.set_importance_mask(None)
@@ -76,23 +78,23 @@ def compare(ctx, rhs, reps=8, first_index=1000, net=None, diagonal=True):
     if diagonal:
         ctx.set_importance_diagonal()
         out["diagonal"] = relative_variance(solve_samples(ctx, rhs, reps, first_index))
-    ctx.set_params({"Transition Sampler": "CDF"})
+    ctx.set_params({"Transition Sampler": samplers[0]})
     return out
 
 
-def select_proposal(ctx, rhs, net=None, reps=4, first_index=2000):
+def select_proposal(ctx, rhs, net=None, reps=4, first_index=2000, samplers=("CDF", "Learned")):
     """Instance-level fallback: measure the pilot variance of plain sampling,
     the diagonal importance and (when given) the network on ``rhs``, then
     leave the context configured with the lowest.  Returns (choice,
     variances).  Switching g state by state (``variance_mask``) makes g
     discontinuous, which raises variance on diffusion operators; choosing
     per instance keeps each candidate's g whole."""
-    v = compare(ctx, rhs, reps=reps, first_index=first_index, net=net, diagonal=True)
+    v = compare(ctx, rhs, reps=reps, first_index=first_index, net=net, diagonal=True, samplers=samplers)
     choice = min(v, key=v.get)
     if choice == "plain":
-        ctx.set_params({"Transition Sampler": "CDF"})
+        ctx.set_params({"Transition Sampler": samplers[0]})
     else:
-        ctx.set_params({"Transition Sampler": "Learned"})
+        ctx.set_params({"Transition Sampler": samplers[1]})
         if choice == "diagonal":
             ctx.set_importance_diagonal()
         else:
@@ -100,5 +102,202 @@ def select_proposal(ctx, rhs, net=None, reps=4, first_index=2000):
     return choice, v
 
 
+# The learned-proposal family of the workloads: 2-D variable-coefficient
+# diffusion, kappa = exp(Gaussian random field), correlation length 0.1,
+# standard deviation 1 (512 random Fourier features), b ~ U[0, 1).
+def family_instance(seed, n=256):
+    from mcls_amd import capi
+
+    return capi.build_problem(capi.make_spec(capi.VARCOEF2D_5PT, n, n, seed=seed, p0=0.1, p1=1.0, p2=512))
+
+
+STUDY_PARAMS = {"Walk Precision": "FP32", "Source Sampling Type": "Stratified", "Estimator Type": "Collision",
+                "Weight Cutoff": 1e-4, "History Length": 20000, "Random Seed": 7}
+STUDY_SAMPLERS = ("Alias", "Learned Alias")
+
+
+def _load(ctx, prob, per_unknown):
+    ctx.set_problem(prob)
+    ctx.set_point_jacobi()
+    ctx.set_params(dict(STUDY_PARAMS, **{"Number of Histories": per_unknown * prob.n,
+                                         "Transition Sampler": STUDY_SAMPLERS[0]}))
+    return prob.b * ctx.get_left_prec_diag()
+
+
+def _prefetch(seeds, n):
+    """Yield (seed, problem) for ``seeds``, building the next instance on a
+    host thread while the caller's GPU work on the current one runs."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    seeds = list(seeds)
+    if not seeds:
+        return
+    with ThreadPoolExecutor(1) as pool:
+        fut = pool.submit(family_instance, seeds[0], n)
+        for k, s in enumerate(seeds):
+            prob = fut.result()
+            if k + 1 < len(seeds):
+                fut = pool.submit(family_instance, seeds[k + 1], n)
+            yield s, prob
+
+
+def family_train(n_train, n=256, rows_per_instance=1024, per_unknown=8, epochs=30, batch_rows=4096, lr=2e-3,
+                 seed0=100000, log=print):
+    """Train a network on ``n_train`` instances of the family: per instance,
+    features and targets (one plain MC solve's tally at ``per_unknown``
+    histories per unknown) of ``rows_per_instance`` sampled states.  Returns
+    (network, record)."""
+    import time
+
+    from mcls_amd import capi
+
+    t0 = time.perf_counter()
+    net = capi.Mlp(0, seed=1)
+    Xs, Ts = [], []
+    with capi.Context(0) as ctx:
+        for k, (s, prob) in enumerate(_prefetch(range(seed0, seed0 + n_train), n)):
+            rhs = _load(ctx, prob, per_unknown)
+            X, T = gather(ctx, rhs, solve_index=k)
+            rows = np.random.default_rng(k).choice(prob.n, size=min(prob.n, rows_per_instance), replace=False)
+            Xs.append(X[rows])
+            Ts.append(T[rows])
+            if k % 100 == 0:
+                log(f"train instance {k}: {time.perf_counter() - t0:.1f}s")
+    t1 = time.perf_counter()
+    loss = train(net, np.concatenate(Xs), np.concatenate(Ts), epochs=epochs, batch_rows=batch_rows, lr=lr, seed=1)
+    rec = {"n": n, "train_instances": n_train, "train_seeds": [seed0, seed0 + n_train - 1],
+           "rows_per_instance": rows_per_instance, "histories_per_unknown": per_unknown, "epochs": epochs,
+           "batch_rows": batch_rows, "lr": lr, "train_loss": float(loss), "gather_seconds": t1 - t0,
+           "train_seconds": time.perf_counter() - t1, "params": STUDY_PARAMS}
+    log(f"trained on {len(Xs) * rows_per_instance} rows: loss {loss:.4f} in {rec['train_seconds']:.1f}s")
+    return net, rec
+
+
+def family_eval(net, n_eval, n=256, per_unknown=4, reps=4, pilot_reps=3, seed0=200000, log=print):
+    """Evaluate ``net`` on ``n_eval`` held-out instances: per instance, the
+    relative variance of one MC solve with plain almost-optimal sampling, the
+    diagonal importance and the network (``reps`` solves each on the same
+    streams), and the instance-level selection (``pilot_reps`` pilot solves on
+    other streams; it keeps the lowest pilot variance).  Returns a record with
+    the per-instance rows and a summary."""
+    import time
+
+    from mcls_amd import capi
+
+    t0 = time.perf_counter()
+    rows_out = []
+    with capi.Context(0) as ctx:
+        for h, (s, prob) in enumerate(_prefetch(range(seed0, seed0 + n_eval), n)):
+            rhs = _load(ctx, prob, per_unknown)
+            v = compare(ctx, rhs, reps=reps, first_index=1000, net=net, diagonal=True, samplers=STUDY_SAMPLERS)
+            choice, pilot = select_proposal(ctx, rhs, net=net, reps=pilot_reps, first_index=5000,
+                                            samplers=STUDY_SAMPLERS)
+            ctx.set_importance_model(None)
+            rows_out.append({"seed": s, "variance": v, "selected": choice, "pilot": pilot})
+            if h % 20 == 0:
+                log(f"eval instance {h}: {v} -> {choice} ({time.perf_counter() - t0:.1f}s)")
+    rec = {"n": n, "eval_instances": n_eval, "eval_seeds": [seed0, seed0 + n_eval - 1],
+           "histories_per_unknown": per_unknown, "reps": reps, "pilot_reps": pilot_reps, "eval": rows_out,
+           "seconds": time.perf_counter() - t0}
+    if rows_out:
+        q = [0.0, 0.1, 0.5, 0.9, 1.0]
+        names = ["min", "p10", "median", "p90", "max"]
+        r_net = np.array([r["variance"]["plain"] / r["variance"]["learned"] for r in rows_out])
+        r_diag = np.array([r["variance"]["plain"] / r["variance"]["diagonal"] for r in rows_out])
+        rec["summary"] = {
+            "variance_reduction_network": dict(zip(names, np.quantile(r_net, q).tolist())),
+            "variance_reduction_diagonal": dict(zip(names, np.quantile(r_diag, q).tolist())),
+            "network_below_plain_fraction": float(np.mean(r_net > 1.0)),
+            "network_beats_diagonal_fraction": float(np.mean(r_net > r_diag)),
+            "selected": {c: int(sum(r["selected"] == c for r in rows_out)) for c in ("plain", "diagonal", "learned")},
+            "selection_kept_worse_than_plain": int(sum(r["pilot"][r["selected"]] > r["pilot"]["plain"]
+                                                      for r in rows_out)),
+        }
+    return rec
+
+
+def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smoother_steps=10, log=print):
+    """MCSA (damped Richardson smoothing, tolerance 1e-8) on the family
+    instances ``seeds`` with plain sampling, the diagonal importance and the
+    network; a diverging solve ends in a non-finite residual and is recorded
+    with the last residual norms."""
+    from mcls_amd import capi
+
+    out = []
+    with capi.Context(0) as ctx:
+        for s, prob in _prefetch(seeds, n):
+            res = {"seed": s}
+            for name in ("plain", "diagonal", "network"):
+                _load(ctx, prob, per_unknown)
+                ctx.set_params({"Convergence Tolerance": 1e-8, "Maximum Iterations": iters,
+                                "Richardson Relaxation": relaxation, "Smoother Steps": smoother_steps})
+                if name != "plain":
+                    ctx.set_params({"Transition Sampler": STUDY_SAMPLERS[1]})
+                    if name == "diagonal":
+                        ctx.set_importance_diagonal()
+                    else:
+                        ctx.set_importance_model(net)
+                try:
+                    ctx.mcsa_solve(prob.b)
+                    st = ctx.stats()
+                    res[name] = {"iterations": int(st.iterations), "converged": bool(st.converged),
+                                 "achieved_tol": float(st.achieved_tol), "seconds": st.ms_total / 1e3,
+                                 "residual_history": [float(x) for x in ctx.residual_history()]}
+                except capi.GpuError as e:  # a diverging solve overflows
+                    res[name] = {"error": str(e), "residual_history": [float(x) for x in ctx.residual_history()]}
+                ctx.set_importance_model(None)
+                log(f"mcsa seed {s} {name}: {({k: v for k, v in res[name].items() if k != 'residual_history'})}")
+            out.append(res)
+    return {"n": n, "histories_per_unknown": per_unknown, "max_iterations": iters, "relaxation": relaxation,
+            "smoother_steps": smoother_steps, "instances": out}
+
+
 __all__ = ["gather", "train", "solve_samples", "relative_variance", "variance_mask", "compare",
-           "select_proposal"]
+           "select_proposal", "family_instance", "family_train", "family_eval", "family_mcsa"]
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mcls_amd import capi
+
+    ap = argparse.ArgumentParser(description="train and evaluate the learned proposal on its workload family")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--train", type=int, default=2000, help="training instances (0: load --net)")
+    ap.add_argument("--eval", type=int, default=200, help="held-out instances")
+    ap.add_argument("--rows", type=int, default=1024, help="sampled states per training instance")
+    ap.add_argument("--train-per-unknown", type=int, default=8, help="histories per unknown of the target solves")
+    ap.add_argument("--eval-per-unknown", type=int, default=4, help="histories per unknown of the compared solves")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--reps", type=int, default=4)
+    ap.add_argument("--mcsa", type=int, default=0, help="held-out instances solved by MCSA with each proposal")
+    ap.add_argument("--mcsa-per-unknown", type=int, default=32)
+    ap.add_argument("--mcsa-iters", type=int, default=30)
+    ap.add_argument("--net", default=None, help="network weights (.npy): written after training, read when --train 0")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    rec = {}
+    if a.train:
+        net, rec["train"] = family_train(a.train, n=a.n, rows_per_instance=a.rows, per_unknown=a.train_per_unknown,
+                                         epochs=a.epochs, log=log)
+        if a.net:
+            np.save(a.net, net.params())
+    else:
+        net = capi.Mlp(0, seed=1)
+        net.set_params(np.load(a.net))
+        rec["network"] = os.path.basename(a.net)
+    if a.eval:
+        rec["eval"] = family_eval(net, a.eval, n=a.n, per_unknown=a.eval_per_unknown, reps=a.reps, log=log)
+    if a.mcsa:
+        rec["mcsa"] = family_mcsa(net, range(200000, 200000 + a.mcsa), n=a.n, per_unknown=a.mcsa_per_unknown,
+                                  iters=a.mcsa_iters, log=log)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(json.dumps(rec))
+    print(json.dumps({k: v.get("summary", {k2: v2 for k2, v2 in v.items() if k2 != "instances"})
+                      if isinstance(v, dict) else v for k, v in rec.items()}))

@@ -28,12 +28,16 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
     t = ctx.autotune(prob.b, pilot_histories=1 << 16)
     print("tuning", t)
     assert ctx.stats().solve_index == before
-    # shapes, batches, tiles (not for forward walks), then one full walk per block size
-    assert t["candidates"] == (16 if method == "Adjoint" else 13) + 3
+    # shapes, batches, tiles (not for forward walks), one full walk per block
+    # size, and a window pilot of the final shape when the full walks changed it
+    base = (16 if method == "Adjoint" else 13) + 3
+    assert t["candidates"] in (base, base + 1)
     assert t["tally_tile"] in (0, 128, 512, 1024)
     assert t["block"] in (64, 128, 256) and t["batch"] in (64, 128, 256, 512)
     assert t["blocks_per_cu"] * t["block"] // 64 in (8, 16, 32)
-    assert 0.0 < t["ms_best"] <= t["ms_default"]
+    assert t["ms_best"] > 0.0 and t["ms_confirmed"] > 0.0
+    if t["candidates"] == base:
+        assert t["ms_best"] <= t["ms_default"]
     ref_p, ref_w = ctx.record_paths(rhs, 5, 0, 3000, 200)
     ref_x = ctx.mc_solve_raw(rhs, 6)
     for blk, bpc, bat in ((64, 32, 512), (256, 2, 64), (128, 8, 128)):
@@ -45,6 +49,24 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
         assert np.linalg.norm(x - ref_x) <= tol * np.linalg.norm(ref_x)
     ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0,
                     "Monte Carlo Type": "Adjoint"})
+
+
+def test_autotune_entry_tile_over_lds(ctx):
+    """A caller's tile that does not fit the LDS at the entry block size (4
+    waves x 2048 entries x 12 bytes > 64 KiB) is not timed: the search starts
+    untiled and returns a shape whose tile fits."""
+    prob, rhs = _poisson(ctx, "Alias", 32)
+    ctx.set_params({"Tally Tile": 2048, "Tally Tile Precision": "FP64", "Walk Block Size": 256,
+                    "Walk Blocks Per CU": 0})
+    try:
+        t = ctx.autotune(prob.b, pilot_histories=1 << 16)
+        assert t["ms_default"] == -1.0 and t["ms_best"] > 0.0 and t["ms_confirmed"] > 0.0
+        assert t["tally_tile"] in (0, 128, 512, 1024)
+        assert t["block"] // 64 * t["tally_tile"] * 12 <= 65536
+        ctx.mc_solve_raw(rhs, 1)  # the returned configuration runs
+    finally:
+        ctx.set_params({"Walk Block Size": 256, "Walk Blocks Per CU": 0, "Walk Batch": 64, "Tally Tile": 0,
+                        "Tally Tile Precision": "FP64"})
 
 
 def test_walk_batch_validation(ctx):

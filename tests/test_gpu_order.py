@@ -114,6 +114,37 @@ def test_mc_tally_and_residual_match_permuted_problem(form):
         assert np.array_equal(co.get_left_prec_diag()[order], cp.get_left_prec_diag())
 
 
+@pytest.mark.parametrize("tile_prec", ["Walk", "Compensated"])
+@pytest.mark.parametrize("walks", [1, 2])
+@pytest.mark.parametrize("tile", [256, 1024])
+def test_ordered_tile_tally_matches_oracle(tile, walks, tile_prec):
+    """The strided FP32 walk with a state order, whose LDS tile maps a state
+    to set (state mod sets) instead of a hash, against the oracle on
+    P A P^T: same step count, tallies to the FP32 partial sums' rounding
+    ("Walk") or to the FP64 summation order ("Compensated")."""
+    prob, pprob, order = _system(10)
+    A = po.Csr(pprob.rp, pprob.col, pprob.val32.astype(np.float64))
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = pprob.b * dinv
+    nh, L = 40000, 100
+    params = {"Number of Histories": nh, "History Length": L, "Weight Cutoff": 1e-2, "Random Seed": 5,
+              "Tally Tile": tile, "Tally Tile Precision": tile_prec, "Walks Per Lane": walks,
+              "Walk Block Size": 64, "Walk Blocks Per CU": 0}
+    user_rhs = np.empty_like(rhs)
+    user_rhs[order] = rhs
+    with _ctx(prob, "chosen", order, params) as co:
+        co.set_params({"Transition Sampler": "Alias Strided"})
+        xg = co.mc_solve_raw(user_rhs, 3)
+        steps = co.stats().steps
+    mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION, po.SRC_STRATIFIED, seed=5,
+                      sampler=po.SAMPLER_ALIAS_STRIDED)
+    xo, steps_o = po.mc_solve(dom, rhs, mp, solve_index=3)
+    assert steps == steps_o
+    tol = 2e-5 if tile_prec == "Walk" else 1e-12
+    assert np.linalg.norm(xg[order] - xo) <= tol * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("form", ["chosen", "plain"])
 def test_mcsa_ordered_converges(form):
     prob, pprob, order = _system()

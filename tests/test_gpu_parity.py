@@ -44,7 +44,7 @@ def _setup(ctx, prob, f32, jacobi=True, params=None):
             "Walk Precision": "FP64", "Source Sampling Type": "Random",
             "Estimator Type": "Collision", "Weight Cutoff": 0.0, "History Length": 10,
             "Number of Histories": 0, "Random Seed": 0, "Transition Sampler": "CDF",
-            "Monte Carlo Type": "Adjoint"}
+            "Monte Carlo Type": "Adjoint", "Residual Format": "Sliced ELL"}
     base.update(params or {})
     ctx.set_params(base)
 
@@ -244,18 +244,68 @@ def test_mcsa_converges_poisson(ctx):
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
-def test_residual_matches_oracle(ctx):
+@pytest.mark.parametrize("fmt", ["Sliced ELL", "CSR"])
+def test_residual_matches_oracle(ctx, fmt):
     prob, f32 = _problem("s27")
-    _setup(ctx, prob, f32)
+    _setup(ctx, prob, f32, params={"Residual Format": fmt})
     rng = np.random.default_rng(3)
     x = rng.random(prob.n)
-    r, nrm = ctx.prec_residual(prob.b, x)
+    try:
+        r, nrm = ctx.prec_residual(prob.b, x)
+    finally:
+        ctx.set_params({"Residual Format": "Sliced ELL"})
     A = _csr(prob, f32)
     ro = po.prec_residual(A, prob.b, x, po.jacobi_dinv(A))
-    # the device sums each row with a lane butterfly: same terms, other order
-    scale = po.spmv(po.Csr(A.rp, A.col, np.abs(A.val)), np.abs(x)) * np.abs(po.jacobi_dinv(A))
-    assert np.all(np.abs(r - ro) <= 64 * np.finfo(float).eps * (scale + np.abs(ro)))
+    if fmt == "Sliced ELL":
+        # one lane per row, entries summed left to right without contraction,
+        # as the oracle sums them
+        assert np.array_equal(r, ro)
+    else:
+        # a lane butterfly per row: same terms, other order
+        scale = po.spmv(po.Csr(A.rp, A.col, np.abs(A.val)), np.abs(x)) * np.abs(po.jacobi_dinv(A))
+        assert np.all(np.abs(r - ro) <= 64 * np.finfo(float).eps * (scale + np.abs(ro)))
+        assert not np.array_equal(r, ro)  # the format switch took effect
     assert abs(nrm - np.linalg.norm(ro)) <= 1e-14 * np.linalg.norm(ro)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+@pytest.mark.parametrize("n", [1, 63, 1000, 4160])
+def test_sliced_residual_ragged_rows(ctx, f32, n):
+    """Rows of 20 to 24 entries, some empty, n not a multiple of the
+    64-row slice: the sliced copy pads each slice to its longest row and the
+    residual equals the oracle's bit for bit, with and without a diagonal
+    left preconditioner."""
+    rng = np.random.default_rng(n)
+    lens = rng.integers(20, 25, n)
+    lens[rng.random(n) < 0.03] = 0
+    lens = np.minimum(lens, n)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, size=k, replace=False)) for k in lens]).astype(np.int32) \
+        if rp[-1] else np.zeros(0, np.int32)
+    val = rng.standard_normal(int(rp[-1]))
+    if f32:
+        val = val.astype(np.float32)
+    ctx.set_matrix(rp, col, val)
+    b = rng.standard_normal(n)
+    x = rng.standard_normal(n)
+    A = po.Csr(rp, col, val.astype(np.float64))
+    ctx.clear_left_prec()
+    ctx.set_params({"Residual Format": "Sliced ELL"})
+    r, nrm = ctx.prec_residual(b, x)
+    ro = po.prec_residual(A, b, x, None)
+    assert np.array_equal(r, ro)
+    assert abs(nrm - np.linalg.norm(ro)) <= 1e-14 * np.linalg.norm(ro)
+    d = rng.random(n) + 0.5
+    ctx.set_left_prec_diag(d)
+    r, _ = ctx.prec_residual(b, x)
+    assert np.array_equal(r, po.prec_residual(A, b, x, d))
+    ctx.set_params({"Residual Format": "CSR"})
+    try:
+        rc, _ = ctx.prec_residual(b, x)
+    finally:
+        ctx.set_params({"Residual Format": "Sliced ELL"})
+    assert np.allclose(rc, r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
 
 
 def test_errors(ctx):
