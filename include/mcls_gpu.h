@@ -181,6 +181,12 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        the exact rounding error of every addition (FP32 walks, 12
  *        bytes); each is flushed into the FP64 tally, which every walk
  *        precision uses (FP32 walks included, tiled or not)
+ *   str "Residual Format"      ("Sliced ELL" | "CSR"): the residual
+ *        PL (b - A PR y) reads a sliced-ELL copy of A (slices of 64 rows
+ *        padded to their longest row, built with the operator when padding
+ *        adds at most a quarter to the entries and it fits in memory; each
+ *        row summed left to right, as the CSR SpMV) or A's CSR arrays with
+ *        a few lanes per row
  * Unknown keys and wrong types return MCLS_ERR_INVALID_ARG. */
 int mcls_gpu_set_param_i64(mcls_gpu_context* ctx, const char* key, int64_t v);
 int mcls_gpu_set_param_f64(mcls_gpu_context* ctx, const char* key, double v);

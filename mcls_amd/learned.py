@@ -10,6 +10,8 @@ wherever g raises variance.  The reference has no counterpart: its
 transitions are always P_ij = |h_ij| / W_i
 (src/MCLS_AlmostOptimalDomain_impl.hpp:342-363).
 """
+import os
+
 import numpy as np
 
 
@@ -109,6 +111,21 @@ def family_instance(seed, n=256):
     from mcls_amd import capi
 
     return capi.build_problem(capi.make_spec(capi.VARCOEF2D_5PT, n, n, seed=seed, p0=0.1, p1=1.0, p2=512))
+
+
+# Weights (include/mcls_mlp.h layout, FP32) of the network trained on 2,000
+# instances of the family at 256 x 256 (seeds 100000-101999, 1,024 sampled
+# states each, targets from 8 histories per unknown, 30 epochs).
+FAMILY_NETWORK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "importance_varcoef2d_256.npy")
+
+
+def load_network(path=FAMILY_NETWORK, device=0):
+    """An importance network with the weights stored at ``path``."""
+    from mcls_amd import capi
+
+    net = capi.Mlp(device, seed=1)
+    net.set_params(np.load(path))
+    return net
 
 
 STUDY_PARAMS = {"Walk Precision": "FP32", "Source Sampling Type": "Stratified", "Estimator Type": "Collision",
@@ -216,11 +233,18 @@ def family_eval(net, n_eval, n=256, per_unknown=4, reps=4, pilot_reps=3, seed0=2
     return rec
 
 
-def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smoother_steps=10, log=print):
-    """MCSA (damped Richardson smoothing, tolerance 1e-8) on the family
-    instances ``seeds`` with plain sampling, the diagonal importance and the
-    network; a diverging solve ends in a non-finite residual and is recorded
-    with the last residual norms."""
+def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smoother_steps=10, chunk=20,
+                log=print):
+    """MCSA (damped Richardson smoothing, ||r|| <= 1e-8 ||PL b||) on the
+    family instances ``seeds`` with plain sampling, the diagonal importance
+    and the network, up to ``iters`` iterations.  The solve runs in calls of
+    ``chunk`` iterations, each starting from the previous one's x: MCSA
+    carries nothing else between iterations, the MC streams follow the
+    context's solve counter and the test is relative to ||PL b||, so this is
+    the same iteration sequence.  A diverging solve ends in a non-finite
+    residual and is recorded with its residual norms."""
+    import time
+
     from mcls_amd import capi
 
     out = []
@@ -229,37 +253,50 @@ def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smo
             res = {"seed": s}
             for name in ("plain", "diagonal", "network"):
                 _load(ctx, prob, per_unknown)
-                ctx.set_params({"Convergence Tolerance": 1e-8, "Maximum Iterations": iters,
-                                "Richardson Relaxation": relaxation, "Smoother Steps": smoother_steps})
                 if name != "plain":
                     ctx.set_params({"Transition Sampler": STUDY_SAMPLERS[1]})
                     if name == "diagonal":
                         ctx.set_importance_diagonal()
                     else:
                         ctx.set_importance_model(net)
-                try:
-                    ctx.mcsa_solve(prob.b)
+                x = np.zeros(prob.n)
+                hist, done, t0 = [], 0, time.perf_counter()
+                rec = {}
+                while done < iters:
+                    ctx.set_params({"Convergence Tolerance": 1e-8, "Maximum Iterations": min(chunk, iters - done),
+                                    "Richardson Relaxation": relaxation, "Smoother Steps": smoother_steps})
+                    try:
+                        x = ctx.mcsa_solve(prob.b, x)
+                    except capi.GpuError as e:  # a diverging solve overflows
+                        h = [float(v) for v in ctx.residual_history()]
+                        hist += h if not hist else h[1:]
+                        rec = {"error": str(e)}
+                        break
                     st = ctx.stats()
-                    res[name] = {"iterations": int(st.iterations), "converged": bool(st.converged),
-                                 "achieved_tol": float(st.achieved_tol), "seconds": st.ms_total / 1e3,
-                                 "residual_history": [float(x) for x in ctx.residual_history()]}
-                except capi.GpuError as e:  # a diverging solve overflows
-                    res[name] = {"error": str(e), "residual_history": [float(x) for x in ctx.residual_history()]}
+                    h = [float(v) for v in ctx.residual_history()]
+                    hist += h if not hist else h[1:]  # a call's first norm is the previous call's last
+                    done += int(st.iterations)
+                    rec = {"iterations": done, "converged": bool(st.converged), "achieved_tol": hist[-1]}
+                    log(f"mcsa seed {s} {name}: {done} iterations, relative residual {hist[-1]:.3e}")
+                    if st.converged or int(st.iterations) == 0:
+                        break
+                rec["seconds"] = time.perf_counter() - t0
+                rec["residual_history"] = hist
+                res[name] = rec
                 ctx.set_importance_model(None)
-                log(f"mcsa seed {s} {name}: {({k: v for k, v in res[name].items() if k != 'residual_history'})}")
+                log(f"mcsa seed {s} {name}: {({k: v for k, v in rec.items() if k != 'residual_history'})}")
             out.append(res)
     return {"n": n, "histories_per_unknown": per_unknown, "max_iterations": iters, "relaxation": relaxation,
-            "smoother_steps": smoother_steps, "instances": out}
+            "smoother_steps": smoother_steps, "chunk": chunk, "instances": out}
 
 
 __all__ = ["gather", "train", "solve_samples", "relative_variance", "variance_mask", "compare",
-           "select_proposal", "family_instance", "family_train", "family_eval", "family_mcsa"]
+           "select_proposal", "family_instance", "load_network", "family_train", "family_eval", "family_mcsa"]
 
 
 if __name__ == "__main__":
     import argparse
     import json
-    import os
     import sys
 
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -288,9 +325,8 @@ if __name__ == "__main__":
         if a.net:
             np.save(a.net, net.params())
     else:
-        net = capi.Mlp(0, seed=1)
-        net.set_params(np.load(a.net))
-        rec["network"] = os.path.basename(a.net)
+        net = load_network(a.net or FAMILY_NETWORK)
+        rec["network"] = os.path.basename(a.net or FAMILY_NETWORK)
     if a.eval:
         rec["eval"] = family_eval(net, a.eval, n=a.n, per_unknown=a.eval_per_unknown, reps=a.reps, log=log)
     if a.mcsa:

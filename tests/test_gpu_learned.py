@@ -317,7 +317,7 @@ def test_diagonal_importance():
 def test_network_trained_on_tallies_reduces_variance():
     """A network trained on the solver's own tallies (8 instances, 32x32,
     targets from FP64 tallies at 64 histories per unknown) against plain
-    almost-optimal sampling on 3 held-out instances.  The gain varies widely
+    almost-optimal sampling on 8 held-out instances.  The gain varies widely
     from instance to instance (measured 0.03x to 80x over seeds), so the
     test asserts the median gain and that the instance-level selection never
     keeps a proposal its own pilot found worse than plain; the diagonal
@@ -336,7 +336,7 @@ def test_network_trained_on_tallies_reduces_variance():
         loss = learned.train(net, np.concatenate(Xs), np.concatenate(Ts), epochs=40, batch_rows=256, lr=2e-3, seed=1)
         assert np.isfinite(loss)
         gains = []
-        for h in range(3):
+        for h in range(8):
             rhs = _load(ctx, _family(32, 77 + h), 16)
             v = learned.compare(ctx, rhs, reps=8, first_index=100, net=net, diagonal=True)
             gains.append(v["plain"] / v["learned"])
@@ -347,6 +347,26 @@ def test_network_trained_on_tallies_reduces_variance():
             assert v["diagonal"] < 0.05 * v["plain"]  # plain variance estimates are heavy-tailed
     print("held-out gains of the network over plain", gains)
     assert np.median(gains) > 1.0
+
+
+def test_family_256_trained_network_and_selection():
+    """The network trained on 2,000 instances of the 256x256 family
+    (mcls_amd/data) on 4 instances it never saw: the diagonal importance
+    beats plain sampling by >= 20x, the network beats plain on most
+    instances (it did on 93.5% of 200 held-out ones, median 18.9x) but not
+    the diagonal rule, and the instance-level selection keeps the lowest
+    pilot variance, never one above plain's."""
+    from mcls_amd import learned
+    net = learned.load_network()
+    rec = learned.family_eval(net, 4, n=256, per_unknown=4, reps=4, pilot_reps=3, seed0=300000)
+    print(rec["summary"])
+    for r in rec["eval"]:
+        v, pilot = r["variance"], r["pilot"]
+        assert r["selected"] == min(pilot, key=pilot.get)
+        assert pilot[r["selected"]] <= pilot["plain"]
+        assert v["diagonal"] < v["plain"] / 20
+    assert rec["summary"]["selection_kept_worse_than_plain"] == 0
+    assert rec["summary"]["variance_reduction_network"]["median"] > 1.0
 
 
 def test_mcsa_diagonal_importance_converges_where_plain_does_not():
