@@ -1141,6 +1141,19 @@ int residual(mcls_gpu_context* c, const double* b, const double* x, double* r, i
   return MCLS_OK;
 }
 
+// Refuse a non-finite MC source (a NaN or Inf in b or PL): its births and
+// weights would be meaningless.  ||v||^2 by the fixed-order reduction into
+// scal[1]; synchronises the stream.
+int check_finite_source(mcls_gpu_context* c, const double* v) {
+  hipStream_t s = c->stream;
+  HIP_TRY(mcls::launch_scaled_sq(c->n, nullptr, v, c->part.p, s));
+  HIP_TRY(mcls::launch_sum_partials(c->part.p, mcls::residual_blocks(c->n, 1), c->scal.p + 1, s));
+  HIP_TRY(hipMemcpyAsync(c->host_scal + 1, c->scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (!std::isfinite(c->host_scal[1])) return fail(MCLS_ERR_NUMERIC, "non-finite Monte Carlo source");
+  return MCLS_OK;
+}
+
 int set_param_common(mcls_gpu_context* c) {
   invalidate_graph(c);
   return MCLS_OK;
@@ -1880,6 +1893,7 @@ int mcls_gpu_mc_solve_raw(mcls_gpu_context* c, const double* rhs, uint64_t solve
   c->stats.ms_walk = c->stats.ms_source = 0.0;
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, c->stream));
   MCLS_TRY(vec_in(c, rhs, MCLS_HOST, c->rhs.p, 1, sizeof(double)));
+  MCLS_TRY(check_finite_source(c, c->rhs.p));
   MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, solve_index));
   MCLS_TRY(vec_out(c, c->x.p, x, MCLS_HOST, 1, sizeof(double)));
   unsigned long long steps = 0;
@@ -1903,6 +1917,7 @@ int mcls_gpu_mc_solve_window(mcls_gpu_context* c, const double* rhs, uint64_t so
   c->stats.ms_walk = c->stats.ms_source = 0.0;
   HIP_TRY(mcls::launch_fill_u64(c->steps.p, 1, 0ULL, c->stream));
   MCLS_TRY(vec_in(c, rhs, MCLS_HOST, c->rhs.p, 1, sizeof(double)));
+  MCLS_TRY(check_finite_source(c, c->rhs.p));
   MCLS_TRY(mc_solve_device(c, ctx_bufs(c), c->rhs.p, c->x.p, false, solve_index, true, nullptr, 1.0, first, count));
   MCLS_TRY(vec_out(c, c->x.p, x, MCLS_HOST, 1, sizeof(double)));
   unsigned long long steps = 0;
@@ -1976,6 +1991,7 @@ int mcls_gpu_mc_solve(mcls_gpu_context* c, const double* b, double* x, int ptr_k
   // rhs = omega * PL b
   MCLS_TRY(apply_left_prec(c, c->b.p, c->rhs.p));
   if (c->prm.neumann_omega != 1.0) HIP_TRY(mcls::launch_scale_copy(c->n, c->prm.neumann_omega, c->rhs.p, c->rhs.p, s));
+  MCLS_TRY(check_finite_source(c, c->rhs.p));
   const uint64_t sidx = c->solve_index++;
   MCLS_TRY(mc_solve_device(c, c->rhs.p, c->x.p, false, sidx));
   // right preconditioning: x = PR x (src/MCLS_MonteCarloSolverManager_impl.hpp:220-229)

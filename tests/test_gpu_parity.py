@@ -308,6 +308,27 @@ def test_sliced_residual_ragged_rows(ctx, f32, n):
     assert np.allclose(rc, r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_nonfinite_right_hand_side_is_refused(ctx, bad):
+    """A NaN or Inf in b: MCSA stops at its initial residual and the MC
+    solves at their source, each with MCLS_ERR_NUMERIC; the context then
+    solves a finite system as before."""
+    prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, 16, 16, seed=0))
+    _setup(ctx, prob, False, params={"History Length": 500, "Weight Cutoff": 1e-4, "Number of Histories": 4000,
+                                     "Convergence Tolerance": 1e-8, "Maximum Iterations": 100, "Random Seed": 3})
+    b = prob.b.copy()
+    b[37] = bad
+    for call in (lambda: ctx.mcsa_solve(b), lambda: ctx.mc_solve(b), lambda: ctx.mc_solve_raw(b, 1),
+                 lambda: ctx.mc_solve_window(b, 1, 0, 100)):
+        with pytest.raises(capi.GpuError) as e:
+            call()
+        assert e.value.code == capi.MCLS_ERR_NUMERIC
+    x = ctx.mcsa_solve(prob.b)
+    assert ctx.stats().converged == 1
+    S = sp.csr_matrix((prob.values(), prob.col, prob.rp), shape=(prob.n, prob.n))
+    assert np.linalg.norm(prob.b - S @ x) <= 1e-6 * np.linalg.norm(prob.b)
+
+
 def test_errors(ctx):
     rp = np.array([0, 1, 2], np.int64)
     with pytest.raises(capi.GpuError) as e:
