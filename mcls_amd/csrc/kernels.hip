@@ -1938,9 +1938,11 @@ __global__ __launch_bounds__(kRedThreads) void k_sell_residual(int n, int nslice
     int wn = tn < t1 ? width[tn] : 0;
     int64_t on = tn < t1 ? off[tn] : 0;
     // b and PL of a slice's rows are loaded with its first round, so that
-    // finishing the slice waits for nothing issued after them
+    // finishing the slice waits for nothing issued after them; they and r
+    // stream past the L2 (non-temporal), which keeps it for the gathers of x
     int i = t * kWave + lane;
-    double bi = i < n ? b[i] : 0.0, di = dinv && i < n ? dinv[i] : 1.0;
+    double bi = i < n ? __builtin_nontemporal_load(b + i) : 0.0;
+    double di = dinv && i < n ? __builtin_nontemporal_load(dinv + i) : 1.0;
     int cc[U];
     AV vv[U];
 #pragma unroll
@@ -1971,8 +1973,8 @@ __global__ __launch_bounds__(kRedThreads) void k_sell_residual(int n, int nslice
       const int in = tn * kWave + lane;
       double bn = 0.0, dn = 1.0;
       if (last && tn < t1 && in < n) {
-        bn = b[in];
-        if (dinv) dn = dinv[in];
+        bn = __builtin_nontemporal_load(b + in);
+        if (dinv) dn = __builtin_nontemporal_load(dinv + in);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -1981,7 +1983,7 @@ __global__ __launch_bounds__(kRedThreads) void k_sell_residual(int n, int nslice
         if (i < n) {
           const double d = __dadd_rn(bi, -s);
           const double ri = dinv ? __dmul_rn(di, d) : d;
-          r[i] = ri;
+          __builtin_nontemporal_store(ri, r + i);
           sq = sq + ri * ri;
         }
         s = 0.0;

@@ -276,7 +276,7 @@ hipError_t launch_csr_spmv(int n, const int* rp, const int* col, const double* v
 // r = PL (b - A x); partial sums of r^2 per block into part
 // (residual_blocks(n, lanes) <= kResidualPartials entries).  lanes per row
 // from residual_lanes(n, nnz).
-constexpr int kResidualPartials = 4096;
+constexpr int kResidualPartials = 16384;
 int residual_lanes(int64_t n, int64_t nnz);
 int residual_blocks(int n, int lanes);
 hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col, const void* val,
