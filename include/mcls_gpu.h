@@ -167,7 +167,8 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *   i64 "Walk Block Size"      (256), i64 "Walk Blocks Per CU" (0 = auto),
  *   i64 "Walk Batch"           (64) history ids per work-queue grab
  *   i64 "Tally Tile"           (0) LDS tally tile entries per wave (0 = off,
- *        else a power of two in [64, 2048]; collision estimator): tallies
+ *        else a power of two in [64, 2048]; adjoint collision and expected-
+ *        value estimators, each expected-value scatter entry through it): tallies
  *        merge in LDS and are flushed into an FP64 tally, also for FP32 walks
  *        (these four are what mcls_gpu_autotune sets)
  *   i64 "Walks Per Lane"       (1) 2: each lane carries two histories and
@@ -279,7 +280,7 @@ int mcls_gpu_convergence_criteria(mcls_gpu_context* ctx, int64_t max_iters, doub
  * of two pilots.  They time the block size (64,
  * 128, 256) at 8, 16 and 32 resident waves per CU, then the history ids a
  * wave takes per queue grab (64 .. 512) at the fastest shape, then
- * (collision estimator) the LDS tally tile: off, 128, 512 or 1024 entries
+ * (adjoint collision or expected-value estimator) the LDS tally tile: off, 128, 512 or 1024 entries
  * per wave (those that fit 64 KiB of LDS); last, the best shape of each
  * block size walks the whole source once, and the fastest of those sets
  * the block size.  `ms_best` is the window pilot time of the configuration

@@ -336,7 +336,8 @@ int ensure_vectors(mcls_gpu_context* c) {
 }
 
 bool tile_on(const mcls_gpu_context* c, bool record) {
-  return !record && c->prm.tally_tile > 0 && c->prm.estimator == mcls::EST_COLLISION && !c->prm.forward;
+  return !record && c->prm.tally_tile > 0 && !c->prm.forward &&
+         (c->prm.estimator == mcls::EST_COLLISION || c->prm.estimator == mcls::EST_EXPECTED_VALUE);
 }
 
 // The LDS tile kind of the walk (FP64 walks keep FP64 partial sums).
@@ -2319,7 +2320,8 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
     return pilot_win(blk, bpc, bat, tile, window, ms);
   };
   // a tile of `tile` entries per wave fits the 64 KiB of LDS of a block
-  const bool tiled_est = saved.estimator == mcls::EST_COLLISION && !saved.forward;
+  const bool tiled_est =
+      !saved.forward && (saved.estimator == mcls::EST_COLLISION || saved.estimator == mcls::EST_EXPECTED_VALUE);
   const size_t tile_elem = mcls::tile_entry_bytes(tile_kind_of(saved), saved.precision == 32 ? 4 : 8);
   auto tile_fits = [&](int64_t blk, int64_t tile) {
     return !tiled_est || tile <= 0 || (size_t)(blk / 64) * (size_t)tile * tile_elem <= 65536;
@@ -2378,7 +2380,7 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
       best.batch = batches[k];
     }
   }
-  // LDS tally tile (adjoint collision estimator): off, 128, 512 or 1024
+  // LDS tally tile (adjoint collision and expected-value estimators): off, 128, 512 or 1024
   // entries per wave, within 64 KiB of LDS per workgroup
   if (tiled_est) {
     const int64_t tiles[4] = {0, 128, 512, 1024};

@@ -516,19 +516,17 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     // with.  It is issued after the transition's first load, so that the
     // atomic's round trip overlaps that load's instead of preceding it (the
     // counter that tracks outstanding memory operations drains in order).
-    auto tally_step = [&]() {
-      if (EST == EST_FORWARD) {
-        fsum = fsum + (double)w * a.rhs[state];
-      } else if (EST == EST_EXPECTED_VALUE) {
-        for (uint32_t k = 0; k < len; ++k) tally_add(&tally[colp[beg + k] & 0x7FFFFFFFu], (Acc)(w * ht[beg + k]));
-      } else if constexpr (TILE != kTileOff) {
+    // One addition v at state st: into this wave's LDS tile, or the FP64
+    // tally when the tile is off or the state's set is full.
+    auto tally_to = [&](int st, Real v) {
+      if constexpr (TILE != kTileOff) {
         // kTileWays-way sets: the state may sit in any slot of its set, whose
         // keys are read together; empty ones are claimed by compare-and-swap
         // in slot order (lanes of one wave with the same state agree on the
         // slot, so a state never holds two).  Full sets send it to HBM.
         const uint32_t s0 = a.tile_direct
-                                  ? ((uint32_t)state & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
-                                  : (((uint32_t)state * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
+                                  ? ((uint32_t)st & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
+                                  : (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
         int keys[kTileWays];
         if constexpr (kTileWays == 4) {
           const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
@@ -540,20 +538,31 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         int slot = -1;
 #pragma unroll
         for (int j = 0; j < kTileWays; ++j)
-          if (slot < 0 && keys[j] == state) slot = (int)s0 + j;
+          if (slot < 0 && keys[j] == st) slot = (int)s0 + j;
 #pragma unroll
         for (int j = 0; j < kTileWays; ++j) {
           if (slot < 0 && keys[j] < 0) {
-            const int k = atomicCAS(&tkeys[s0 + j], -1, state);
-            if (k < 0 || k == state) slot = (int)s0 + j;
+            const int k = atomicCAS(&tkeys[s0 + j], -1, st);
+            if (k < 0 || k == st) slot = (int)s0 + j;
           }
         }
         if (slot >= 0)
-          tile_add<TILE>(tvals, tcomp, slot, (TAcc)w);
+          tile_add<TILE>(tvals, tcomp, slot, (TAcc)v);
         else
-          tally_add(&tally[state], (Acc)w);
+          tally_add(&tally[st], (Acc)v);
       } else {
-        tally_add(&tally[state], (Acc)w);
+        tally_add(&tally[st], (Acc)v);
+      }
+    };
+    auto tally_step = [&]() {
+      if (EST == EST_FORWARD) {
+        fsum = fsum + (double)w * a.rhs[state];
+      } else if (EST == EST_EXPECTED_VALUE) {
+        // the row's scatter, each entry through the tile (a state order keeps
+        // a row's columns, and the rows of the wave's walks, in few sets)
+        for (uint32_t k = 0; k < len; ++k) tally_to((int)(colp[beg + k] & 0x7FFFFFFFu), w * ht[beg + k]);
+      } else {
+        tally_to(state, w);
       }
     };
     bool end = !STRIDED && len == 0;  // absorbing row (all of h dropped)
@@ -898,7 +907,7 @@ hipError_t launch_walk_s(const WalkArgs& a, const WalkConfig& c, hipStream_t s) 
   }
   if (c.record) {
     k_walk<Real, EST, SRC, SAMP, true, kTileOff><<<c.grid, c.block, 0, s>>>(a);
-  } else if constexpr (EST == EST_COLLISION) {
+  } else if constexpr (EST == EST_COLLISION || EST == EST_EXPECTED_VALUE) {
     const size_t lds = (size_t)(c.block / 64) * (size_t)a.tile_slots * tile_entry_bytes(c.tile, sizeof(Real));
     if constexpr (sizeof(Real) == 4) {
       if (c.tile == kTileWalk) {

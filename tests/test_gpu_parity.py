@@ -193,17 +193,23 @@ def test_paths_bitexact_skewed_rows(ctx, prec):
     assert (gp[:, 5] >= 0).mean() > 0.5
 
 
+@pytest.mark.parametrize("tile", [0, 256])
 @pytest.mark.parametrize("sampler", ["CDF", "Alias"])
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("est", ["Collision", "Expected Value"])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
-def test_mc_tally_matches_oracle(ctx, prec, est, source, sampler):
+def test_mc_tally_matches_oracle(ctx, prec, est, source, sampler, tile):
+    """Tallies against the oracle's with the same paths, with and without
+    the LDS tile (the expected-value estimator scatters each row entry
+    through it; FP32 walks keep compensated tile sums)."""
     prob, f32 = _problem("diff3d")
     nh, L, cutoff = 20000, 50, 1e-2
     _setup(ctx, prob, f32, params={"Walk Precision": f"FP{prec}", "Estimator Type": est,
                                    "Source Sampling Type": source, "History Length": L,
                                    "Weight Cutoff": cutoff, "Number of Histories": nh,
-                                   "Random Seed": 99, "Transition Sampler": sampler})
+                                   "Random Seed": 99, "Transition Sampler": sampler, "Tally Tile": tile,
+                                   "Tally Tile Precision": "Compensated", "Walk Block Size": 256,
+                                   "Walk Blocks Per CU": 0})
     A = _csr(prob, f32)
     dinv = po.jacobi_dinv(A)
     dom = po.build_domain(A, dinv)
@@ -215,6 +221,7 @@ def test_mc_tally_matches_oracle(ctx, prec, est, source, sampler):
                       po.SRC_STRATIFIED if source == "Stratified" else po.SRC_RANDOM, seed=99,
                       sampler=po.SAMPLER_ALIAS if sampler == "Alias" else po.SAMPLER_CDF)
     xo, steps = po.mc_solve(dom, rhs, mp, solve_index=5)
+    ctx.set_params({"Tally Tile": 0})
     assert st.steps == steps
     tol = 1e-10 if prec == 64 else 2e-5
     assert np.linalg.norm(xg - xo) <= tol * np.linalg.norm(xo)
