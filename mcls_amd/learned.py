@@ -233,6 +233,16 @@ def family_eval(net, n_eval, n=256, per_unknown=4, reps=4, pilot_reps=3, seed0=2
     return rec
 
 
+def _direct_error(prob, x):
+    """Relative 2-norm error of x against an FP64 direct solve of A x = b."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    A = sp.csr_matrix((prob.values(), prob.col, prob.rp), shape=(prob.n, prob.n))
+    xd = sla.spsolve(A.tocsc(), prob.b)
+    return float(np.linalg.norm(x - xd) / np.linalg.norm(xd))
+
+
 def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smoother_steps=10, chunk=20,
                 log=print):
     """MCSA (damped Richardson smoothing, ||r|| <= 1e-8 ||PL b||) on the
@@ -281,6 +291,8 @@ def family_mcsa(net, seeds, n=256, per_unknown=32, iters=30, relaxation=0.7, smo
                     if st.converged or int(st.iterations) == 0:
                         break
                 rec["seconds"] = time.perf_counter() - t0
+                if rec.get("converged"):
+                    rec["error_vs_direct"] = _direct_error(prob, x)
                 rec["residual_history"] = hist
                 res[name] = rec
                 ctx.set_importance_model(None)
