@@ -347,3 +347,32 @@ def test_birth_magnitudes_keep_the_estimator_unbiased(source):
     expect = np.zeros(n)
     expect[k] = b[k]
     assert np.allclose(x, expect, rtol=1e-14, atol=0) and steps == 500
+
+
+@pytest.mark.parametrize("kind", ["stencil", "ragged"])
+def test_residual_equals_scipy_bit_for_bit(kind):
+    """orc_prec_residual (updatePrecResidual without PR): rows summed left to
+    right in FP64 without contraction, then PL (b - s).  scipy's CSR
+    product sums each row in stored order too, so the two agree bit for bit
+    (a dropped entry, a wrong column or a misplaced PL would not)."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(7)
+    if kind == "stencil":
+        from mcls_amd import capi
+        prob = capi.build_problem(capi.make_spec(capi.STENCIL3D_27PT, 9, seed=2, p0=0.6, p1=0.9, p2=0.8, p3=1.2))
+        rp, col, val, n = prob.rp, prob.col, prob.values(), prob.n
+    else:
+        n = 300
+        lens = rng.integers(0, 30, n)
+        rp = np.zeros(n + 1, np.int64)
+        rp[1:] = np.cumsum(lens)
+        col = np.concatenate([np.sort(rng.choice(n, size=k, replace=False)) for k in lens]).astype(np.int32)
+        val = rng.standard_normal(int(rp[-1]))
+    A = po.Csr(rp, col, val)
+    S = sp.csr_matrix((val, col, rp), shape=(n, n))
+    b = rng.standard_normal(n)
+    x = rng.standard_normal(n)
+    d = rng.random(n) + 0.5
+    assert np.array_equal(po.prec_residual(A, b, x, None), b - S @ x)
+    assert np.array_equal(po.prec_residual(A, b, x, d), d * (b - S @ x))
