@@ -315,6 +315,26 @@ def test_sliced_residual_ragged_rows(ctx, f32, n):
     assert np.allclose(rc, r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
 
 
+@pytest.mark.parametrize("fmt", ["Sliced ELL", "CSR"])
+def test_residual_of_matrix_without_entries(ctx, fmt):
+    """A = 0 (70 rows, no entries; slices of width 0): r = b exactly, its
+    norm the fixed-order sum of squares; and n = 0 is refused."""
+    n = 70
+    ctx.set_matrix(np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    ctx.clear_left_prec()
+    ctx.set_params({"Residual Format": fmt})
+    try:
+        b = np.random.default_rng(5).standard_normal(n)
+        r, nrm = ctx.prec_residual(b, np.ones(n))
+    finally:
+        ctx.set_params({"Residual Format": "Sliced ELL"})
+    assert np.array_equal(r, b)
+    assert abs(nrm - np.linalg.norm(b)) <= 1e-14 * np.linalg.norm(b)
+    with pytest.raises(capi.GpuError) as e:
+        ctx.set_matrix(np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    assert e.value.code == capi.MCLS_ERR_INVALID_ARG
+
+
 @pytest.mark.parametrize("bad", [np.nan, np.inf])
 def test_nonfinite_right_hand_side_is_refused(ctx, bad):
     """A NaN or Inf in b: MCSA stops at its initial residual and the MC
