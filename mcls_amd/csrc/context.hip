@@ -131,6 +131,8 @@ struct mcls_gpu_context {
   bool sell_on = false;
   bool has_prec = false;
   DBuf<double> dinv;
+  DBuf<uint32_t> guide;  // random births' guide table, 2^guide_bits + 1 entries
+  int guide_bits = 0;
   // general (sparse) left / right preconditioners: FP64 CSR, ascending rows
   struct PrecCsr {
     DBuf<int> rp, col;
@@ -319,6 +321,8 @@ int ensure_vectors(mcls_gpu_context* c) {
   MCLS_TRY(c->ncnt.ensure(n, "ncnt"));
   MCLS_TRY(c->ncdf.ensure(n, "ncdf"));
   MCLS_TRY(c->scan_tmp.ensure(mcls::scan_tmp_elems((int64_t)n) + 1, "scan tmp"));
+  c->guide_bits = mcls::guide_bits_for((int64_t)n);
+  MCLS_TRY(c->guide.ensure(((size_t)1 << c->guide_bits) + 1, "birth guide"));
   MCLS_TRY(c->bmax.ensure(1, "bmax"));
   MCLS_TRY(c->steps.ensure(1, "steps"));
   MCLS_TRY(c->queue.ensure(mcls::kQueueWords, "queue"));
@@ -352,11 +356,13 @@ struct SolveBufs {
   uint64_t *q, *qcdf, *ncnt, *ncdf, *scan_tmp;
   unsigned long long *bmax, *steps, *queue;
   double* t64;  // the tally (FP64 for every walk precision)
+  uint32_t* guide = nullptr;  // random births' guide table (nullable)
+  int guide_bits = 0;
 };
 
 SolveBufs ctx_bufs(mcls_gpu_context* c) {
   return SolveBufs{c->q.p,    c->qcdf.p,  c->ncnt.p,  c->ncdf.p, c->scan_tmp.p,
-                   c->bmax.p, c->steps.p, c->queue.p, c->t64.p};
+                   c->bmax.p, c->steps.p, c->queue.p, c->t64.p, c->guide.p, c->guide_bits};
 }
 
 void invalidate_graph(mcls_gpu_context* c) {
@@ -912,6 +918,8 @@ WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, 
   a.steps = sb.steps;
   a.batch = (int)c->prm.walk_batch;
   a.per_state = forward_per_state(c);
+  a.guide = c->prm.source == mcls::SRC_RANDOM ? sb.guide : nullptr;
+  a.guide_bits = sb.guide_bits;
   return a;
 }
 
@@ -929,6 +937,8 @@ int build_source(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, in
   if (c->prm.source == mcls::SRC_STRATIFIED) {
     HIP_TRY(mcls::launch_strat_counts(sb.q, n, sb.qcdf + (n - 1), (uint64_t)nh, sb.ncnt, s));
     HIP_TRY(mcls::launch_scan_u64(sb.ncnt, sb.ncdf, n, sb.scan_tmp, s));
+  } else if (sb.guide) {
+    HIP_TRY(mcls::launch_build_guide(n, sb.qcdf, sb.guide_bits, sb.guide, s));
   }
   return MCLS_OK;
 }

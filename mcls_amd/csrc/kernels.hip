@@ -77,6 +77,22 @@ __device__ __forceinline__ int upper_bound_u64(const uint64_t* __restrict__ cdf,
   return lo;
 }
 
+// The state of a random birth: the first k with qcdf[k] > t, for a draw
+// t in [0, total).  With a guide table the search covers only
+// [guide[j], guide[j + 1]], j = t's top guide_bits bits of total's range,
+// instead of all n entries: the same k, a few probes instead of log2(n).
+__device__ __forceinline__ int birth_state(const uint64_t* __restrict__ qcdf, const uint32_t* __restrict__ guide,
+                                           int guide_bits, int n, uint64_t total, uint64_t t) {
+  if (guide) {
+    const int bl = 64 - __clzll((long long)total);
+    const int sh = bl > guide_bits ? bl - guide_bits : 0;
+    const uint64_t j = t >> sh;
+    const int lo = (int)guide[j], hi = (int)guide[j + 1];
+    return lo + upper_bound_u64(qcdf + lo, hi - lo + 1, t);
+  }
+  return upper_bound_u64(qcdf, n, t);
+}
+
 // First k in [lo, n) with cdf[k] > t (n when none), found by the whole wave
 // in a few dependent round trips instead of log2(n): each round the 64 lanes
 // probe 64 evenly spaced entries of the range left and the ballot of
@@ -470,7 +486,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             w0 = 1.0;
           } else if (SRC == SRC_RANDOM) {
             const uint64_t t = mul_u53(xorshift64(rng), total);
-            state = upper_bound_u64(a.qcdf, n, t);
+            state = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
           } else if (window) {
             state = slo + bpos;
@@ -819,7 +835,7 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
             w0 = 1.0;
           } else if (SRC == SRC_RANDOM) {
             const uint64_t t = mul_u53(xorshift64(rng[j]), total);
-            state[j] = upper_bound_u64(a.qcdf, n, t);
+            state[j] = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state[j]] * (totald / (double)a.q[state[j]]);
           } else {
             state[j] = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
@@ -2673,6 +2689,32 @@ hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col,
                                 hipStream_t s) {
   return f32 ? prec_residual_t<float>(n, lanes, rp, col, val, dinv, b, x, r, part, s)
              : prec_residual_t<double>(n, lanes, rp, col, val, dinv, b, x, r, part, s);
+}
+
+// guide[j] for j = 0 .. 2^bits (see birth_state); Q = 0: all zero
+__global__ void k_build_guide(int n, const uint64_t* __restrict__ qcdf, int bits, uint32_t* __restrict__ guide) {
+  const uint64_t total = qcdf[n - 1];
+  const int m = 1 << bits;
+  const int bl = total ? 64 - __clzll((long long)total) : 0;
+  const int sh = bl > bits ? bl - bits : 0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j <= m; j += gridDim.x * blockDim.x) {
+    const int k = total ? upper_bound_u64(qcdf, n, (uint64_t)j << sh) : 0;
+    guide[j] = (uint32_t)(k < n - 1 ? k : n - 1);
+  }
+}
+
+int guide_bits_for(int64_t n) {
+  int b = 0;
+  while (b < 62 && ((int64_t)1 << b) < n) ++b;  // ceil(log2 n)
+  b -= 2;                                        // about four states per entry
+  return b < 4 ? 4 : (b > 22 ? 22 : b);
+}
+
+hipError_t launch_build_guide(int n, const uint64_t* qcdf, int bits, uint32_t* guide, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int m = 1 << bits;
+  k_build_guide<<<grid_for(m + 1, 256), 256, 0, s>>>(n, qcdf, bits, guide);
+  return hipGetLastError();
 }
 
 constexpr int kSellRound = 16;  // entries per lane per round of loads

@@ -148,6 +148,34 @@ def test_paths_bitexact(ctx, prec, source, kind, sampler):
     assert (gp[:, 1] >= 0).mean() > 0.9  # walks actually move
 
 
+@pytest.mark.parametrize("n", [40, 3000])
+def test_random_births_skewed_source_bitexact(ctx, n):
+    """Random births from a source whose magnitudes span twelve decades, most
+    of them zero and a few huge: the birth search (guide table, then the
+    bracketed search) lands where the oracle's full lower bound does, for
+    every history, on operators small enough that the guide covers few
+    states per entry and large enough that it covers many."""
+    nx = int(round(n ** 0.5))
+    prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, nx, nx, seed=3))
+    L, nh = 20, 20000
+    _setup(ctx, prob, False, params={"Walk Precision": "FP64", "Source Sampling Type": "Random", "History Length": L,
+                                     "Weight Cutoff": 1e-6, "Number of Histories": nh, "Random Seed": 4321,
+                                     "Transition Sampler": "CDF"})
+    A = _csr(prob)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rng = np.random.default_rng(n)
+    rhs = np.where(rng.random(prob.n) < 0.8, 0.0, 10.0 ** rng.uniform(-6, 6, prob.n))
+    rhs[rng.integers(0, prob.n, 3)] = 1e6
+    rhs *= np.where(rng.random(prob.n) < 0.5, -1.0, 1.0)
+    gp, gw = ctx.record_paths(rhs, 3, 0, 4000, L)
+    mp = po.mc_params(nh, L, 1e-6, 64, po.EST_COLLISION, po.SRC_RANDOM, seed=4321, sampler=po.SAMPLER_CDF)
+    op, ow = po.record_paths(dom, rhs, mp, 0, 4000, L, solve_index=3)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    assert np.all(rhs[gp[:, 0]] != 0.0)  # no birth on a zero entry
+    assert len(np.unique(gp[:, 0])) >= 3
+
+
 def _skewed_problem(n=400, per_row=40, seed=21):
     """Rows of ~40 entries whose magnitudes span six decades, so that CDF
     inversion lands far from the uniform guess floor(u len) on both sides."""
