@@ -131,7 +131,7 @@ struct mcls_gpu_context {
   bool sell_on = false;
   bool has_prec = false;
   DBuf<double> dinv;
-  DBuf<uint32_t> guide;  // random births' guide table, 2^guide_bits + 1 entries
+  DBuf<uint32_t> guide;  // births' guide table, 2^guide_bits + 1 entries
   int guide_bits = 0;
   // general (sparse) left / right preconditioners: FP64 CSR, ascending rows
   struct PrecCsr {
@@ -356,7 +356,7 @@ struct SolveBufs {
   uint64_t *q, *qcdf, *ncnt, *ncdf, *scan_tmp;
   unsigned long long *bmax, *steps, *queue;
   double* t64;  // the tally (FP64 for every walk precision)
-  uint32_t* guide = nullptr;  // random births' guide table (nullable)
+  uint32_t* guide = nullptr;  // births' guide table (nullable)
   int guide_bits = 0;
 };
 
@@ -918,7 +918,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, 
   a.steps = sb.steps;
   a.batch = (int)c->prm.walk_batch;
   a.per_state = forward_per_state(c);
-  a.guide = c->prm.source == mcls::SRC_RANDOM ? sb.guide : nullptr;
+  a.guide = sb.guide;  // over qcdf (random) or ncdf (stratified), as the source built it
   a.guide_bits = sb.guide_bits;
   return a;
 }
@@ -937,6 +937,7 @@ int build_source(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, in
   if (c->prm.source == mcls::SRC_STRATIFIED) {
     HIP_TRY(mcls::launch_strat_counts(sb.q, n, sb.qcdf + (n - 1), (uint64_t)nh, sb.ncnt, s));
     HIP_TRY(mcls::launch_scan_u64(sb.ncnt, sb.ncdf, n, sb.scan_tmp, s));
+    if (sb.guide) HIP_TRY(mcls::launch_build_guide(n, sb.ncdf, sb.guide_bits, sb.guide, s));
   } else if (sb.guide) {
     HIP_TRY(mcls::launch_build_guide(n, sb.qcdf, sb.guide_bits, sb.guide, s));
   }

@@ -299,6 +299,195 @@ __device__ __forceinline__ void tile_add(TAcc* tvals, float* tcomp, int slot, TA
   }
 }
 
+// One wave's LDS tally tile (TILE != kTileOff): a.tile_slots entries of
+// {state, partial sum (, compensation)} in kTileWays-way sets, carved out of
+// the block's dynamic LDS (keys of every wave, then sums, then compensations).
+template <int TILE, typename TAcc>
+struct WaveTile {
+  int* keys = nullptr;    // state held by each slot, -1 = empty
+  TAcc* vals = nullptr;   // partial sums
+  float* comp = nullptr;  // kTileComp: compensation terms
+  int slots = 0;
+  int shift = 0;   // 32 - log2(slots)
+  int direct = 0;  // set = state mod sets (else a Fibonacci hash)
+
+  // wave-uniform
+  __device__ void init(unsigned char* lds, const WalkArgs& a, int lane) {
+    const int wv = threadIdx.x >> 6;
+    const size_t nw = (size_t)(blockDim.x >> 6);
+    slots = a.tile_slots;
+    shift = a.tile_shift;
+    direct = a.tile_direct;
+    keys = (int*)lds + wv * slots;
+    vals = (TAcc*)(lds + nw * slots * sizeof(int)) + wv * slots;
+    if constexpr (TILE == kTileComp) comp = (float*)(lds + nw * slots * (sizeof(int) + sizeof(TAcc))) + wv * slots;
+    for (int k = lane; k < slots; k += 64) {
+      keys[k] = -1;
+      vals[k] = 0;
+      if constexpr (TILE == kTileComp) comp[k] = 0.0f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  // One tally v at state st.  The state may sit in any slot of its set, whose
+  // keys are read together; empty ones are claimed by compare-and-swap in slot
+  // order (lanes of one wave with the same state agree on the slot, so a
+  // state never holds two).  A full set sends the tally to the FP64 tally.
+  template <typename V>
+  __device__ void add(double* __restrict__ tally, int st, V v) {
+    const uint32_t s0 = direct ? ((uint32_t)st & ((uint32_t)(slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
+                               : (((uint32_t)st * 2654435761u) >> (shift + kTileWaysLog2)) << kTileWaysLog2;
+    int kq[kTileWays];
+    if constexpr (kTileWays == 4) {
+      const uint4 kk = *reinterpret_cast<const uint4*>(keys + s0);
+      kq[0] = (int)kk.x; kq[1] = (int)kk.y; kq[2] = (int)kk.z; kq[3] = (int)kk.w;
+    } else {
+      const uint2 kk = *reinterpret_cast<const uint2*>(keys + s0);
+      kq[0] = (int)kk.x; kq[1] = (int)kk.y;
+    }
+    int slot = -1;
+#pragma unroll
+    for (int j = 0; j < kTileWays; ++j)
+      if (slot < 0 && kq[j] == st) slot = (int)s0 + j;
+#pragma unroll
+    for (int j = 0; j < kTileWays; ++j) {
+      if (slot < 0 && kq[j] < 0) {
+        const int k = atomicCAS(&keys[s0 + j], -1, st);
+        if (k < 0 || k == st) slot = (int)s0 + j;
+      }
+    }
+    if (slot >= 0)
+      tile_add<TILE>(vals, comp, slot, (TAcc)v);
+    else
+      tally_add(&tally[st], (double)v);
+  }
+
+  // Every held slot's partial sum (and compensation, met in FP64) into the
+  // FP64 tally, and the tile cleared.  Wave-uniform: every lane of the wave
+  // runs it together; four slots (one set) per lane and pass, keys and sums
+  // in vector reads, a set cleared by vector writes when any slot was held.
+  __device__ void flush(double* __restrict__ tally, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane * 4; k < slots; k += 256) {
+      const uint4 kk = *reinterpret_cast<const uint4*>(keys + k);
+      const int kq[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
+      // the AND of the keys is negative only when all four slots are empty
+      if ((kq[0] & kq[1] & kq[2] & kq[3]) >= 0) {
+        const TileSet<TAcc> v = *reinterpret_cast<const TileSet<TAcc>*>(vals + k);
+        if constexpr (TILE == kTileComp) {
+          const TileSet<float> cv = *reinterpret_cast<const TileSet<float>*>(comp + k);
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (kq[q] >= 0) tally_add(&tally[kq[q]], (double)v.s[q] + (double)cv.s[q]);
+          *reinterpret_cast<TileSet<float>*>(comp + k) = TileSet<float>{};
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (kq[q] >= 0) tally_add(&tally[kq[q]], (double)v.s[q]);
+        }
+        *reinterpret_cast<uint4*>(keys + k) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        *reinterpret_cast<TileSet<TAcc>*>(vals + k) = TileSet<TAcc>{};
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+};
+
+// Stratified births of one queue grab, history ids [qnext, qend)
+// (consecutive): the grab's first and last birth states slo and shi, and,
+// when they span at most 64 states (`window`), lane L holding state slo + L's
+// last id relative to the grab's start (win_c) and its birth weight
+// (win_w0), so that a birth is a shuffle search with no memory access.
+struct StratWindow {
+  int slo = 0, shi = 0;
+  bool window = false;
+  int64_t qbase = 0;
+  uint32_t win_c = 0xFFFFFFFFu;
+  double win_w0 = 0.0;
+};
+
+// The grab's window.  The first state (first k with ncdf[k] > qnext) lies in
+// [guide[j], guide[j + 1]], j = qnext >> gsh (a.guide built over ncdf,
+// nullable): one guide probe and one coalesced load of
+// V[m] = ncdf[lo - 1 + m] settle it, and V also holds the window's counts
+// unless the window runs past it.  Wave-uniform: every lane of the wave calls
+// it with the same grab.
+__device__ __forceinline__ void grab_stratified(const WalkArgs& a, int n, int gsh, double totald, int64_t qnext,
+                                                int64_t qend, int lane, StratWindow& g) {
+  const uint64_t t0 = (uint64_t)qnext, t1 = (uint64_t)(qend - 1);
+  int lo = 0, hi = n - 1;  // the first state lies in [lo, hi]
+  if (a.guide) {
+    const uint64_t j = t0 >> gsh;
+    lo = (int)a.guide[j];
+    hi = (int)a.guide[j + 1];
+  }
+  if (hi - lo > 62) lo = wave_upper_bound_u64(a.ncdf, lo, hi + 1, t0, lane);  // no guide, or a long birthless run
+  const int km = lo - 1 + lane;
+  const uint64_t V = km < 0 ? 0ULL : (km < n ? a.ncdf[km] : ~0ULL);
+  // lanes 1..63 hold ncdf[lo..lo + 62], non-decreasing: those <= t0 are a
+  // prefix, and ncdf[hi] > t0 with hi <= lo + 62
+  g.slo = lo + __popcll(__ballot(lane >= 1 && V <= t0));
+  g.qbase = qnext;
+  const int dd = g.slo - lo;  // 0 <= dd <= 62
+  const int k = g.slo + lane;
+  const int s1 = lane + dd + 1;  // V's index of ncdf[k]
+  uint64_t ck = __shfl(V, s1 & 63);
+  uint64_t pk = __shfl(V, (lane + dd) & 63);  // ncdf[k - 1] (0 for k = 0)
+  if (s1 > 63) {  // past V: k >= 63
+    ck = k < n ? a.ncdf[k] : ~0ULL;
+    pk = k - 1 < n ? a.ncdf[k - 1] : ~0ULL;
+  }
+  const int c = __popcll(__ballot(ck <= t1));
+  g.window = c < 64;
+  if (g.window) {
+    g.shi = g.slo + c;
+    g.win_c = 0xFFFFFFFFu;
+    g.win_w0 = 0.0;
+    if (lane <= c) {
+      const uint64_t nk = ck - pk;
+      const uint64_t rel = ck - (uint64_t)g.qbase;
+      g.win_c = rel < 0xFFFFFFFEULL ? (uint32_t)rel : 0xFFFFFFFEu;
+      g.win_w0 = nk ? a.rhs[k] * (totald / (double)nk) : 0.0;
+    }
+  } else {
+    g.shi = wave_upper_bound_u64(a.ncdf, g.slo + 64, n, t1, lane);
+  }
+}
+
+// Window birth of history id qnext + rank: its state's offset from g.slo
+// (the first window state whose last id passes it, by binary lifting over the
+// lanes) and its weight.  Wave-uniform: every lane shuffles, the lanes not
+// being born ignore the result.
+__device__ __forceinline__ void window_birth(const StratWindow& g, int64_t qnext, int rank, int& bpos, double& bw0) {
+  const uint32_t keyb = (uint32_t)(qnext - g.qbase) + (uint32_t)rank;
+  bpos = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1) {
+    const uint32_t cv = __shfl(g.win_c, bpos + step - 1);
+    if (cv <= keyb) bpos += step;
+  }
+  bw0 = __shfl(g.win_w0, bpos);
+}
+
+// Stratified birth outside a window: the first k in [g.slo, g.shi] with
+// ncdf[k] > hid, and its weight.
+__device__ __forceinline__ int strat_birth(const WalkArgs& a, const StratWindow& g, double totald, int64_t hid,
+                                           double& w0) {
+  const int st = g.slo + upper_bound_u64(a.ncdf + g.slo, g.shi - g.slo + 1, (uint64_t)hid);
+  const uint64_t nk = a.ncdf[st] - (st ? a.ncdf[st - 1] : 0);
+  w0 = a.rhs[st] * (totald / (double)nk);
+  return st;
+}
+
+// The shift of a history id to its guide bucket (as k_build_guide)
+__device__ __forceinline__ int guide_shift(uint64_t total, int bits) {
+  const int bl = total ? 64 - __clzll((long long)total) : 0;
+  return bl > bits ? bl - bits : 0;
+}
+
 template <typename Real, int EST, int SRC, int SAMP, bool REC, int TILE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   using Acc = double;  // the HBM tally
@@ -331,57 +520,14 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   unsigned long long* ctr = a.queue + qs * kQueueStride;
   const int lane = __lane_id();
 
-  int* tkeys = nullptr;  // TILE: this wave's slots
-  TAcc* tvals = nullptr;  // partial sums
-  float* tcomp = nullptr;  // kTileComp: the compensation terms
+  WaveTile<TILE, TAcc> tile;  // TILE: this wave's LDS tally tile
   if constexpr (TILE != kTileOff) {
     extern __shared__ __align__(16) unsigned char walk_lds[];
-    const int wv = threadIdx.x >> 6;
-    const size_t nw = (size_t)(blockDim.x >> 6);
-    tkeys = (int*)walk_lds + wv * a.tile_slots;
-    tvals = (TAcc*)(walk_lds + nw * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
-    if constexpr (TILE == kTileComp)
-      tcomp = (float*)(walk_lds + nw * a.tile_slots * (sizeof(int) + sizeof(TAcc))) + wv * a.tile_slots;
-    for (int k = lane; k < a.tile_slots; k += 64) {
-      tkeys[k] = -1;
-      tvals[k] = 0;
-      if constexpr (TILE == kTileComp) tcomp[k] = 0.0f;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    tile.init(walk_lds, a, lane);
   }
   // wave-uniform: every lane of the wave runs it together
   auto flush_tile = [&]() {
-    if constexpr (TILE != kTileOff) {
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      // four slots (one set) per lane and pass: keys and sums in vector
-      // reads, the set cleared by vector writes when any slot was held
-      for (int k = lane * 4; k < a.tile_slots; k += 256) {
-        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
-        const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
-        // the AND of the keys is negative only when all four slots are empty
-        if ((keys[0] & keys[1] & keys[2] & keys[3]) >= 0) {
-          const TileSet<TAcc> v = *reinterpret_cast<const TileSet<TAcc>*>(tvals + k);
-          if constexpr (TILE == kTileComp) {
-            // the partial sum and its compensation meet in FP64
-            const TileSet<float> cv = *reinterpret_cast<const TileSet<float>*>(tcomp + k);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (keys[q] >= 0) tally_add(&tally[keys[q]], (double)v.s[q] + (double)cv.s[q]);
-            *reinterpret_cast<TileSet<float>*>(tcomp + k) = TileSet<float>{};
-          } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (keys[q] >= 0) tally_add(&tally[keys[q]], (Acc)v.s[q]);
-          }
-          *reinterpret_cast<uint4*>(tkeys + k) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-          *reinterpret_cast<TileSet<TAcc>*>(tvals + k) = TileSet<TAcc>{};
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
+    if constexpr (TILE != kTileOff) tile.flush(tally, lane);
   };
 
   // The current row's header is carried from the record that chose the
@@ -392,16 +538,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   constexpr bool STRIDED = SAMP == SAMPLER_ALIAS_STRIDED;
   constexpr bool CARRY = (SAMP != SAMPLER_ALIAS || FUSED) && !STRIDED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
-  int slo = 0, shi = 0;         // stratified: birth states of the current batch
-  // Stratified births of a batch whose states fit one wave (the window
-  // [slo, shi] of at most 64 states, wave-uniform `window`): lane L holds
-  // state slo + L's last history id relative to the batch start (win_c) and
-  // its birth weight (win_w0), so a birth is a shuffle search, not a chain of
-  // dependent loads and a division.
-  bool window = false;
-  int64_t qbase = 0;
-  uint32_t win_c = 0xFFFFFFFFu;
-  double win_w0 = 0.0;
+  StratWindow grab;             // stratified: birth states of the current batch
+  const int gsh = SRC == SRC_STRATIFIED && EST != EST_FORWARD ? guide_shift(total, a.guide_bits) : 0;
   bool exhausted = false;       // wave-uniform
   bool alive = false;
   int64_t hid = 0;
@@ -432,49 +570,19 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             exhausted = true;
             break;
           }
-          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD) {
-            // the batch's ids are consecutive: bound its birth states once
-            // (every lane of the wave is here), so each birth searches only
-            // that window of the count table
-            slo = wave_upper_bound_u64(a.ncdf, 0, n, (uint64_t)qnext, lane);
-            qbase = qnext;
-            const int k = slo + lane;
-            const uint64_t ck = k < n ? a.ncdf[k] : ~0ULL;
-            const int c = __popcll(__ballot(ck <= (uint64_t)(qend - 1)));
-            window = c < 64;
-            if (window) {
-              shi = slo + c;
-              win_c = 0xFFFFFFFFu;
-              win_w0 = 0.0;
-              if (lane <= c) {
-                const uint64_t nk = ck - (k ? a.ncdf[k - 1] : 0);
-                const uint64_t rel = ck - (uint64_t)qbase;
-                win_c = rel < 0xFFFFFFFEULL ? (uint32_t)rel : 0xFFFFFFFEu;
-                win_w0 = nk ? a.rhs[k] * (totald / (double)nk) : 0.0;
-              }
-            } else {
-              shi = wave_upper_bound_u64(a.ncdf, slo + 64, n, (uint64_t)(qend - 1), lane);
-            }
-          }
+          // the batch's ids are consecutive: bound its birth states once
+          // (every lane of the wave is here), so each birth searches only
+          // that window of the count table, or none
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD)
+            grab_stratified(a, n, gsh, totald, qnext, qend, lane, grab);
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
         const int take = (int64_t)cnt < avail ? cnt : (int)avail;
         const int rank = __popcll(need & __lanemask_lt());
-        // window births: the first window state whose last id passes this
-        // lane's id, by binary lifting over the lanes (wave-uniform: every
-        // lane shuffles, the lanes not being born ignore the result)
         int bpos = 0;
         double bw0 = 0.0;
-        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && window) {
-          const uint32_t keyb = (uint32_t)(qnext - qbase) + (uint32_t)rank;
-#pragma unroll
-          for (int step = 32; step >= 1; step >>= 1) {
-            const uint32_t cv = __shfl(win_c, bpos + step - 1);
-            if (cv <= keyb) bpos += step;
-          }
-          bw0 = __shfl(win_w0, bpos);
-        }
+        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
         if (!alive && rank < take) {
           hid = qnext + rank;
           rng = history_state(key, (uint64_t)hid);
@@ -488,14 +596,11 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             const uint64_t t = mul_u53(xorshift64(rng), total);
             state = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
-          } else if (window) {
-            state = slo + bpos;
+          } else if (grab.window) {
+            state = grab.slo + bpos;
             w0 = bw0;
           } else {
-            // first k with ncdf[k] > hid lies in [slo, shi] for this batch
-            state = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
-            const uint64_t nk = a.ncdf[state] - (state ? a.ncdf[state - 1] : 0);
-            w0 = a.rhs[state] * (totald / (double)nk);
+            state = strat_birth(a, grab, totald, hid, w0);
           }
           w = (Real)w0;
           wc = (Real)a.cutoff * (w < 0 ? -w : w);
@@ -535,40 +640,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
     // One addition v at state st: into this wave's LDS tile, or the FP64
     // tally when the tile is off or the state's set is full.
     auto tally_to = [&](int st, Real v) {
-      if constexpr (TILE != kTileOff) {
-        // kTileWays-way sets: the state may sit in any slot of its set, whose
-        // keys are read together; empty ones are claimed by compare-and-swap
-        // in slot order (lanes of one wave with the same state agree on the
-        // slot, so a state never holds two).  Full sets send it to HBM.
-        const uint32_t s0 = a.tile_direct
-                                  ? ((uint32_t)st & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
-                                  : (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
-        int keys[kTileWays];
-        if constexpr (kTileWays == 4) {
-          const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
-          keys[0] = (int)kk.x; keys[1] = (int)kk.y; keys[2] = (int)kk.z; keys[3] = (int)kk.w;
-        } else {
-          const uint2 kk = *reinterpret_cast<const uint2*>(tkeys + s0);
-          keys[0] = (int)kk.x; keys[1] = (int)kk.y;
-        }
-        int slot = -1;
-#pragma unroll
-        for (int j = 0; j < kTileWays; ++j)
-          if (slot < 0 && keys[j] == st) slot = (int)s0 + j;
-#pragma unroll
-        for (int j = 0; j < kTileWays; ++j) {
-          if (slot < 0 && keys[j] < 0) {
-            const int k = atomicCAS(&tkeys[s0 + j], -1, st);
-            if (k < 0 || k == st) slot = (int)s0 + j;
-          }
-        }
-        if (slot >= 0)
-          tile_add<TILE>(tvals, tcomp, slot, (TAcc)v);
-        else
-          tally_add(&tally[st], (Acc)v);
-      } else {
+      if constexpr (TILE != kTileOff)
+        tile.add(tally, st, v);
+      else
         tally_add(&tally[st], (Acc)v);
-      }
     };
     auto tally_step = [&]() {
       if (EST == EST_FORWARD) {
@@ -662,9 +737,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
 // Two walks per lane on the strided alias tables (FP32, collision or
 // forward estimator, no path recording): each lane carries two histories
 // and issues both of their record loads before it waits on either, doubling
-// the loads in flight per resident wave.  Births, transitions, tallies and
-// the LDS tile are those of k_walk; only the order of the tally additions
-// differs.
+// the loads in flight per resident wave, so that half the waves keep as many
+// walks in flight with half the history ids grabbed and not yet walked.
+// Births (the same grab windows), transitions, tallies and the LDS tile are
+// those of k_walk; only the order of the tally additions differs.
 template <int EST, int SRC, int TILE>
 __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
   using Real = float;
@@ -696,92 +772,24 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
   const int lane = __lane_id();
   const uint32_t stride = d.stride;
 
-  int* tkeys = nullptr;
-  TAcc* tvals = nullptr;
-  float* tcomp = nullptr;
+  WaveTile<TILE, TAcc> tile;
   if constexpr (TILE != kTileOff) {
     extern __shared__ __align__(16) unsigned char walk_lds[];
-    const int wv = threadIdx.x >> 6;
-    const size_t nw = (size_t)(blockDim.x >> 6);
-    tkeys = (int*)walk_lds + wv * a.tile_slots;
-    tvals = (TAcc*)(walk_lds + nw * a.tile_slots * sizeof(int)) + wv * a.tile_slots;
-    if constexpr (TILE == kTileComp)
-      tcomp = (float*)(walk_lds + nw * a.tile_slots * (sizeof(int) + sizeof(TAcc))) + wv * a.tile_slots;
-    for (int k = lane; k < a.tile_slots; k += 64) {
-      tkeys[k] = -1;
-      tvals[k] = 0;
-      if constexpr (TILE == kTileComp) tcomp[k] = 0.0f;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    tile.init(walk_lds, a, lane);
   }
   auto flush_tile = [&]() {
-    if constexpr (TILE != kTileOff) {
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      // four slots (one set) per lane and pass: keys and sums in vector
-      // reads, the set cleared by vector writes when any slot was held
-      for (int k = lane * 4; k < a.tile_slots; k += 256) {
-        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + k);
-        const int keys[4] = {(int)kk.x, (int)kk.y, (int)kk.z, (int)kk.w};
-        // the AND of the keys is negative only when all four slots are empty
-        if ((keys[0] & keys[1] & keys[2] & keys[3]) >= 0) {
-          const TileSet<TAcc> v = *reinterpret_cast<const TileSet<TAcc>*>(tvals + k);
-          if constexpr (TILE == kTileComp) {
-            // the partial sum and its compensation meet in FP64
-            const TileSet<float> cv = *reinterpret_cast<const TileSet<float>*>(tcomp + k);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (keys[q] >= 0) tally_add(&tally[keys[q]], (double)v.s[q] + (double)cv.s[q]);
-            *reinterpret_cast<TileSet<float>*>(tcomp + k) = TileSet<float>{};
-          } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (keys[q] >= 0) tally_add(&tally[keys[q]], (Acc)v.s[q]);
-          }
-          *reinterpret_cast<uint4*>(tkeys + k) = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-          *reinterpret_cast<TileSet<TAcc>*>(tvals + k) = TileSet<TAcc>{};
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-    }
+    if constexpr (TILE != kTileOff) tile.flush(tally, lane);
   };
   auto tally_at = [&](int st, Real wt) {
-    if constexpr (TILE != kTileOff) {
-      const uint32_t s0 = a.tile_direct
-                                  ? ((uint32_t)st & ((uint32_t)(a.tile_slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
-                                  : (((uint32_t)st * 2654435761u) >> (a.tile_shift + kTileWaysLog2)) << kTileWaysLog2;
-      int keys[kTileWays];
-      if constexpr (kTileWays == 4) {
-        const uint4 kk = *reinterpret_cast<const uint4*>(tkeys + s0);
-        keys[0] = (int)kk.x; keys[1] = (int)kk.y; keys[2] = (int)kk.z; keys[3] = (int)kk.w;
-      } else {
-        const uint2 kk = *reinterpret_cast<const uint2*>(tkeys + s0);
-        keys[0] = (int)kk.x; keys[1] = (int)kk.y;
-      }
-      int slot = -1;
-#pragma unroll
-      for (int q = 0; q < kTileWays; ++q)
-        if (slot < 0 && keys[q] == st) slot = (int)s0 + q;
-#pragma unroll
-      for (int q = 0; q < kTileWays; ++q) {
-        if (slot < 0 && keys[q] < 0) {
-          const int k = atomicCAS(&tkeys[s0 + q], -1, st);
-          if (k < 0 || k == st) slot = (int)s0 + q;
-        }
-      }
-      if (slot >= 0)
-        tile_add<TILE>(tvals, tcomp, slot, (TAcc)wt);
-      else
-        tally_add(&tally[st], (Acc)wt);
-    } else {
+    if constexpr (TILE != kTileOff)
+      tile.add(tally, st, wt);
+    else
       tally_add(&tally[st], (Acc)wt);
-    }
   };
 
   int64_t qnext = 0, qend = 0;
-  int slo = 0, shi = 0;
+  StratWindow grab;
+  const int gsh = SRC == SRC_STRATIFIED && EST != EST_FORWARD ? guide_shift(total, a.guide_bits) : 0;
   bool exhausted = false;
   bool alive[J] = {false, false};
   int state[J] = {0, 0};
@@ -810,20 +818,16 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
             exhausted = true;
             break;
           }
-          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD) {
-            int lo = 0, hi = 0;
-            if (lane == leader) {
-              lo = upper_bound_u64(a.ncdf, n, (uint64_t)qnext);
-              hi = upper_bound_u64(a.ncdf, n, (uint64_t)(qend - 1));
-            }
-            slo = __shfl(lo, leader);
-            shi = __shfl(hi, leader);
-          }
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD)
+            grab_stratified(a, n, gsh, totald, qnext, qend, lane, grab);
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
         const int take = (int64_t)cnt < avail ? cnt : (int)avail;
         const int rank = __popcll(need & __lanemask_lt());
+        int bpos = 0;
+        double bw0 = 0.0;
+        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
         if (!alive[j] && rank < take) {
           const int64_t hid = qnext + rank;
           rng[j] = history_state(key, (uint64_t)hid);
@@ -837,10 +841,11 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
             const uint64_t t = mul_u53(xorshift64(rng[j]), total);
             state[j] = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state[j]] * (totald / (double)a.q[state[j]]);
+          } else if (grab.window) {
+            state[j] = grab.slo + bpos;
+            w0 = bw0;
           } else {
-            state[j] = slo + upper_bound_u64(a.ncdf + slo, shi - slo + 1, (uint64_t)hid);
-            const uint64_t nk = a.ncdf[state[j]] - (state[j] ? a.ncdf[state[j] - 1] : 0);
-            w0 = a.rhs[state[j]] * (totald / (double)nk);
+            state[j] = strat_birth(a, grab, totald, hid, w0);
           }
           w[j] = (Real)w0;
           wc[j] = (Real)a.cutoff * (w[j] < 0 ? -w[j] : w[j]);

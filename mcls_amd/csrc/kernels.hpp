@@ -117,8 +117,10 @@ struct WalkArgs {
   int tile_shift;          // 32 - log2(tile_slots)
   int tile_direct;         // 1: tile set = state mod sets (locality-ordered
                            // states); 0: Fibonacci hash of the state
-  // random source: guide table of the birth search (nullable), over the top
-  // guide_bits bits of the draw's range (launch_build_guide)
+  // guide table of the birth search (nullable), over the top guide_bits
+  // bits of the source's range (launch_build_guide): built over qcdf for the
+  // random source (a birth's draw), over ncdf for the stratified one (a
+  // queue grab's first history id)
   const uint32_t* guide;
   int guide_bits;
 };
@@ -286,10 +288,12 @@ int residual_blocks(int n, int lanes);
 hipError_t launch_prec_residual(int n, int lanes, const int* rp, const int* col, const void* val,
                                 bool f32, const double* dinv, const double* b, const double* x,
                                 double* r, double* part, hipStream_t s);
-// Guide table of the random source's birth search: for j = 0 .. 2^bits,
-// guide[j] = the first k with qcdf[k] > j << sh (at most n - 1), where
-// sh = max(0, bitlength(Q) - bits), Q = qcdf[n - 1] read on the device.  A
-// draw t in [0, Q) then lies between guide[t >> sh] and guide[(t >> sh) + 1].
+// Guide table of a birth search over a cumulative table (qcdf for a random
+// source's draws, ncdf for a stratified source's queue grabs): for
+// j = 0 .. 2^bits, guide[j] = the first k with qcdf[k] > j << sh (at most
+// n - 1), where sh = max(0, bitlength(Q) - bits), Q = qcdf[n - 1] read on the
+// device.  The first k with qcdf[k] > t, for t in [0, Q), then lies between
+// guide[t >> sh] and guide[(t >> sh) + 1].
 int guide_bits_for(int64_t n);
 hipError_t launch_build_guide(int n, const uint64_t* qcdf, int bits, uint32_t* guide, hipStream_t s);
 // Sliced-ELL copy of A for the residual (slices of 64 rows, entry-major,
