@@ -283,7 +283,9 @@ int mcls_gpu_convergence_criteria(mcls_gpu_context* ctx, int64_t max_iters, doub
  * (adjoint collision or expected-value estimator) the LDS tally tile: off, 128, 512 or 1024 entries
  * per wave (those that fit 64 KiB of LDS); last, the best shape of each
  * block size walks the whole source once, and the fastest of those sets
- * the block size.  `ms_best` is the window pilot time of the configuration
+ * the block size, after which the chosen batch's neighbours (half and
+ * double, within 64 .. 512) walk the whole source at that shape and the
+ * fastest sets the batch.  `ms_best` is the window pilot time of the configuration
  * returned and `ms_confirmed` its walk of the whole source.  Shapes whose
  * tile does not fit the LDS are skipped; an entry configuration that does
  * not fit is not timed (`ms_default` = -1) and the search starts untiled.

@@ -2423,14 +2423,32 @@ int mcls_gpu_autotune(mcls_gpu_context* c, const double* b, int ptr_kind, int64_
         fbpc = block_bpc[i];
       }
     }
+    // The batch, confirmed the same way at that shape: the window holds an
+    // eighth of the solve's grabs, and has ranked 256 ids per grab ahead of
+    // 512 that walk the whole source faster.  The chosen batch's neighbours
+    // walk the full source too; the fastest is kept.
+    const int64_t fbat0 = best.batch;
+    int64_t fbat = fbat0;
+    if (full_best >= 0.0) {
+      for (int k = 0; k < 4 && rc == MCLS_OK; ++k) {
+        if (batches[k] != fbat0 / 2 && batches[k] != fbat0 * 2) continue;
+        rc = pilot_win(fb, fbpc, batches[k], best.tally_tile, 1.0, &ms);
+        ++best.candidates;
+        if (rc == MCLS_OK && ms < full_best) {
+          full_best = ms;
+          fbat = batches[k];
+        }
+      }
+    }
     // ms_best is the window pilot of the configuration returned
-    if (rc == MCLS_OK && (fb != best.block || fbpc != best.blocks_per_cu)) {
-      rc = pilot_ms(fb, fbpc, best.batch, best.tally_tile, &ms);
+    if (rc == MCLS_OK && (fb != best.block || fbpc != best.blocks_per_cu || fbat != fbat0)) {
+      rc = pilot_ms(fb, fbpc, fbat, best.tally_tile, &ms);
       ++best.candidates;
       best.ms_best = ms;
     }
     best.block = fb;
     best.blocks_per_cu = fbpc;
+    best.batch = fbat;
     best.ms_confirmed = full_best;
   }
   c->prm = saved;

@@ -269,9 +269,12 @@ __device__ __forceinline__ uint32_t cdf_step(const DomainView& d, uint32_t& beg,
 // twice FP32's precision (12 bytes per entry).
 // Every walk's HBM tally is FP64 (FP32 walks convert their weights on the
 // way in), so nothing is summed in FP32 across flushes or in an untiled walk.
-// Slots per set of the tile (4: measured 2% faster than 2 and 7% faster
-// than direct mapping on the 256^3 27-point chosen walk).
-constexpr int kTileWaysLog2 = 2;
+// Slots per set of the tile (8 on the 256^3 27-point chosen walk: 30.8 ms
+// per launch against 31.4 with 4-way sets, whose sets were full for 15% of
+// the tallies, and 35.0 with 16-way sets, whose key reads cost more than
+// they keep; 4-way sets were 2% faster than 2-way and 7% faster than direct
+// mapping).  A set's keys are read in 16-byte LDS loads.
+constexpr int kTileWaysLog2 = 3;
 constexpr int kTileWays = 1 << kTileWaysLog2;
 
 // The partial sums of one set, aligned so that a flush reads and clears them
@@ -338,13 +341,15 @@ struct WaveTile {
   __device__ void add(double* __restrict__ tally, int st, V v) {
     const uint32_t s0 = direct ? ((uint32_t)st & ((uint32_t)(slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
                                : (((uint32_t)st * 2654435761u) >> (shift + kTileWaysLog2)) << kTileWaysLog2;
+    static_assert(kTileWays % 4 == 0, "a set's keys are read four at a time");
     int kq[kTileWays];
-    if constexpr (kTileWays == 4) {
-      const uint4 kk = *reinterpret_cast<const uint4*>(keys + s0);
-      kq[0] = (int)kk.x; kq[1] = (int)kk.y; kq[2] = (int)kk.z; kq[3] = (int)kk.w;
-    } else {
-      const uint2 kk = *reinterpret_cast<const uint2*>(keys + s0);
-      kq[0] = (int)kk.x; kq[1] = (int)kk.y;
+#pragma unroll
+    for (int q = 0; q < kTileWays / 4; ++q) {
+      const uint4 kk = *reinterpret_cast<const uint4*>(keys + s0 + 4 * q);
+      kq[4 * q] = (int)kk.x;
+      kq[4 * q + 1] = (int)kk.y;
+      kq[4 * q + 2] = (int)kk.z;
+      kq[4 * q + 3] = (int)kk.w;
     }
     int slot = -1;
 #pragma unroll

@@ -29,15 +29,14 @@ def test_autotune_choice_and_shape_independence(ctx, sampler, prec, method):
     print("tuning", t)
     assert ctx.stats().solve_index == before
     # shapes, batches, tiles (not for forward walks), one full walk per block
-    # size, and a window pilot of the final shape when the full walks changed it
+    # size, one per neighbouring batch of the chosen one (one or two), and a
+    # window pilot of the final shape when the full walks changed it
     base = (16 if method == "Adjoint" else 13) + 3
-    assert t["candidates"] in (base, base + 1)
+    assert base + 1 <= t["candidates"] <= base + 3
     assert t["tally_tile"] in (0, 128, 512, 1024)
     assert t["block"] in (64, 128, 256) and t["batch"] in (64, 128, 256, 512)
     assert t["blocks_per_cu"] * t["block"] // 64 in (8, 16, 32)
     assert t["ms_best"] > 0.0 and t["ms_confirmed"] > 0.0
-    if t["candidates"] == base:
-        assert t["ms_best"] <= t["ms_default"]
     ref_p, ref_w = ctx.record_paths(rhs, 5, 0, 3000, 200)
     ref_x = ctx.mc_solve_raw(rhs, 6)
     for blk, bpc, bat in ((64, 32, 512), (256, 2, 64), (128, 8, 128)):

@@ -176,6 +176,40 @@ def test_random_births_skewed_source_bitexact(ctx, n):
     assert len(np.unique(gp[:, 0])) >= 3
 
 
+@pytest.mark.parametrize("batch", [64, 512])
+@pytest.mark.parametrize("n", [40, 3000])
+def test_stratified_births_skewed_source_bitexact(ctx, n, batch):
+    """Stratified births from a source whose magnitudes span twelve decades,
+    most of them zero and a few huge: a queue grab's first state comes from
+    the guide table over the birth counts, and the source drives every branch
+    of that search - grabs inside one state's thousands of histories, grabs
+    whose states overrun one wave's window, and guide buckets spanning runs
+    of more than 62 states with few or no births, where the wave search
+    takes over.  Every history is born where the oracle's full search puts
+    it, with the oracle's weight."""
+    nx = int(round(n ** 0.5))
+    prob = capi.build_problem(capi.make_spec(capi.POISSON2D_5PT, nx, nx, seed=5))
+    L, nh = 20, 20000
+    _setup(ctx, prob, False, params={"Walk Precision": "FP64", "Source Sampling Type": "Stratified",
+                                     "History Length": L, "Weight Cutoff": 1e-6, "Number of Histories": nh,
+                                     "Random Seed": 2468, "Transition Sampler": "CDF", "Walk Batch": batch})
+    A = _csr(prob)
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rng = np.random.default_rng(n + batch)
+    rhs = np.where(rng.random(prob.n) < 0.8, 0.0, 10.0 ** rng.uniform(-6, 6, prob.n))
+    rhs[rng.integers(0, prob.n, 3)] = 1e6
+    rhs[prob.n // 3: prob.n // 3 + min(200, prob.n // 3)] = 0.0  # a long birthless run
+    rhs *= np.where(rng.random(prob.n) < 0.5, -1.0, 1.0)
+    mp = po.mc_params(nh, L, 1e-6, 64, po.EST_COLLISION, po.SRC_STRATIFIED, seed=2468, sampler=po.SAMPLER_CDF)
+    # the source holds at least nh histories (counts are rounded up)
+    gp, gw = ctx.record_paths(rhs, 5, 0, nh, L)
+    op, ow = po.record_paths(dom, rhs, mp, 0, nh, L, solve_index=5)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    assert np.all(rhs[gp[:, 0]] != 0.0)  # no birth on a zero entry
+    assert len(np.unique(gp[:, 0])) >= 3
+
+
 def _skewed_problem(n=400, per_row=40, seed=21):
     """Rows of ~40 entries whose magnitudes span six decades, so that CDF
     inversion lands far from the uniform guess floor(u len) on both sides."""
