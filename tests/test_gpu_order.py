@@ -114,14 +114,15 @@ def test_mc_tally_and_residual_match_permuted_problem(form):
         assert np.array_equal(co.get_left_prec_diag()[order], cp.get_left_prec_diag())
 
 
-@pytest.mark.parametrize("tile_prec", ["Walk", "Compensated"])
+@pytest.mark.parametrize("tile_prec", ["Walk", "Compensated", "FP64"])
 @pytest.mark.parametrize("walks", [1, 2])
 @pytest.mark.parametrize("tile", [256, 1024])
 def test_ordered_tile_tally_matches_oracle(tile, walks, tile_prec):
     """The strided FP32 walk with a state order, whose LDS tile maps a state
     to set (state mod sets) instead of a hash, against the oracle on
     P A P^T: same step count, tallies to the FP32 partial sums' rounding
-    ("Walk") or to the FP64 summation order ("Compensated")."""
+    ("Walk") or to the FP64 summation order ("Compensated", and "FP64", the
+    256^3 workload's chosen form)."""
     prob, pprob, order = _system(10)
     A = po.Csr(pprob.rp, pprob.col, pprob.val32.astype(np.float64))
     dinv = po.jacobi_dinv(A)
