@@ -420,6 +420,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, args.form, steps_per_launch,
                                                  ctx_tile, order_label),
+                     # not counted in this run (counters need their own profiler
+                     # passes): the committed rocprofv3 FETCH_SIZE + WRITE_SIZE record
+                     # of this form, calibrated on a copy kernel, when it matches
+                     "traffic_from": f"profiles/walk_{args.workload}_traffic.json",
                      "kernel": f"k_walk<{'float' if prec == 'FP32' else 'double'}>",
                      "bytes_per_step": bstep, "steps_per_launch": steps_per_launch,
                      "avg_launch_ms": avg_launch_s * 1e3},

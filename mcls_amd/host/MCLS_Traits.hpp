@@ -192,9 +192,9 @@ class VectorTraits<Vector> {
     for (int64_t i = 0; i < A.size(); ++i)
       A[i] = (alpha == 0.0 ? 0.0 : alpha * A[i]) + beta * B[i] + gamma * C[i];
   }
-  // A = alpha B .* C + beta A
+  // A(i) = alpha A(i) + beta B(i) C(i) (src/MCLS_VectorTraits.hpp:345-352)
   static void elementWiseMultiply(Vector& A, double alpha, const Vector& B, const Vector& C, double beta) {
-    for (int64_t i = 0; i < A.size(); ++i) A[i] = alpha * B[i] * C[i] + beta * A[i];
+    for (int64_t i = 0; i < A.size(); ++i) A[i] = alpha * A[i] + beta * B[i] * C[i];
   }
 };
 

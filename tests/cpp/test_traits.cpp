@@ -128,7 +128,14 @@ static void test_vector_traits() {
   for (int i = 0; i < n; ++i) CHECK(std::fabs(f[i] - (2.0 * a[i] - 3.0 * b[i] + 0.5 * c[i])) < 1e-15);
   Vector g = a;
   VT::elementWiseMultiply(g, 2.0, b, c, 0.25);
-  for (int i = 0; i < n; ++i) CHECK(std::fabs(g[i] - (2.0 * b[i] * c[i] + 0.25 * a[i])) < 1e-15);
+  for (int i = 0; i < n; ++i) CHECK(std::fabs(g[i] - (2.0 * a[i] + 0.25 * b[i] * c[i])) < 1e-15);
+  {
+    // the reference's known answer (tpetra/test/tstTpetraVector.cpp:636-657):
+    // A = 1, B = 2, C = 3, alpha = 4, beta = 5 -> 1*4 + 5*2*3 = 34
+    Vector A1(10, 1.0), B1(10, 2.0), C1(10, 3.0);
+    VT::elementWiseMultiply(A1, 4.0, B1, C1, 5.0);
+    for (int i = 0; i < 10; ++i) CHECK(A1[i] == 34.0);
+  }
   Vector h(n, std::nan(""));
   VT::update(h, 0.0, a, 1.0);  // alpha = 0 does not read A
   for (int i = 0; i < n; ++i) CHECK(h[i] == a[i]);
