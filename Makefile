@@ -16,7 +16,7 @@ GPU_SRCS := $(SRC)/kernels.hip $(SRC)/context.hip $(SRC)/mlp.hip
 GPU_HDRS := $(SRC)/kernels.hpp $(SRC)/mlp.hpp include/mcls_gpu.h include/mcls_mlp.h
 
 all: $(LIBDIR)/libmcls_gpu.so $(LIBDIR)/libmcls_problems.so tests/cpp/bin/test_facade tests/cpp/bin/test_mm \
-     tests/cpp/bin/test_traits oracle
+     tests/cpp/bin/test_traits tests/cpp/bin/test_tpetra_adapter oracle
 
 $(LIBDIR)/libmcls_gpu.so: $(GPU_SRCS) $(GPU_HDRS)
 	mkdir -p $(LIBDIR)
@@ -29,6 +29,14 @@ $(LIBDIR)/libmcls_problems.so: $(SRC)/problems.cpp include/mcls_problems.h
 tests/cpp/bin/test_facade: tests/cpp/test_facade.cpp mcls_amd/host/*.hpp include/mcls_gpu.h $(LIBDIR)/libmcls_gpu.so
 	mkdir -p tests/cpp/bin
 	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_facade.cpp -L$(LIBDIR) -lmcls_gpu \
+	    -L/opt/rocm/lib -Wl,-rpath,'$$ORIGIN/../../../$(LIBDIR)' -Wl,-rpath,/opt/rocm/lib
+
+# the Tpetra binding against a double of the Tpetra API it calls (Trilinos is
+# not installed here); host checks without a GPU, the managers with --gpu
+tests/cpp/bin/test_tpetra_adapter: tests/cpp/test_tpetra_adapter.cpp tests/cpp/tpetra_double.hpp mcls_amd/host/*.hpp \
+    include/mcls_gpu.h $(LIBDIR)/libmcls_gpu.so
+	mkdir -p tests/cpp/bin
+	$(CXX) $(CXXFLAGS) -Imcls_amd/host -o $@ tests/cpp/test_tpetra_adapter.cpp -L$(LIBDIR) -lmcls_gpu \
 	    -L/opt/rocm/lib -Wl,-rpath,'$$ORIGIN/../../../$(LIBDIR)' -Wl,-rpath,/opt/rocm/lib
 
 # MatrixMarket reader driver: host-only, no GPU library
