@@ -50,7 +50,7 @@ def log(msg):
 
 WORKLOADS = {
     "cfg4": dict(kind=capi.STENCIL3D_27PT, nx=256, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
-                 histories=1 << 28, bricks=((32, 32, 32), (4, 4, 4)), label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
+                 histories=1 << 28, bricks=((32, 32, 32), (8, 8, 8), (4, 4, 4)), label="3-D 27-point stencil 256^3 (16.8M unknowns, 449M nnz, "
                  "FP32 CSR), point Jacobi, 2^28 histories/iteration, tol 1e-8"),
     "cfg4-64": dict(kind=capi.STENCIL3D_27PT, nx=64, seed=4, p=(0.6, 0.9, 0.8, 1.2), f32=True,
                     histories=16 * 64 ** 3, bricks=((32, 32, 32), (4, 4, 4)), label="3-D 27-point stencil 64^3, FP32 CSR, point Jacobi, "

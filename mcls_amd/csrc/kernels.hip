@@ -1107,23 +1107,23 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(const T* __rest
   if (threadIdx.x == 0) sums[blockIdx.x] = sm[0];
 }
 
-// exclusive scan of sums[0..m) in place, single block
+// exclusive scan of sums[0..m) in place, single block: each thread sums a
+// contiguous chunk, the chunk totals are scanned across the block once, and
+// each thread writes its chunk's running prefix (integers: exact in any order)
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_sums(T* sums, int64_t m) {
   __shared__ T sm[kScanThreads];
-  __shared__ T carry_s;
-  if (threadIdx.x == 0) carry_s = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < m; base += kScanThreads) {
-    int64_t i = base + threadIdx.x;
-    T v = i < m ? sums[i] : T(0);
-    T tot;
-    T ex = block_excl_scan<T>(v, sm, &tot);
-    T carry = carry_s;
-    if (i < m) sums[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) carry_s = carry + tot;
-    __syncthreads();
+  const int64_t per = (m + kScanThreads - 1) / kScanThreads;
+  const int64_t b = (int64_t)threadIdx.x * per < m ? (int64_t)threadIdx.x * per : m;
+  const int64_t e = b + per < m ? b + per : m;
+  T s = T(0);
+  for (int64_t i = b; i < e; ++i) s += sums[i];
+  T tot;
+  T ex = block_excl_scan<T>(s, sm, &tot);
+  for (int64_t i = b; i < e; ++i) {
+    const T v = sums[i];
+    sums[i] = ex;
+    ex += v;
   }
 }
 
