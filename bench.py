@@ -58,8 +58,11 @@ WORKLOADS = {
 }
 
 FORMS = {
-    # weight cutoff 0.1: measured fastest to 1e-8 at 2^28 histories per
-    # iteration (8 iterations of 8.5 steps per history; 1e-2 takes 6 of 16.5)
+    # weight cutoff 0.1: 8 iterations of 8.5 steps per history at 2^28
+    # histories per iteration (1e-2 takes 6 of 16.5); larger cutoffs run
+    # faster only as the walks shrink towards a one-step stochastic Jacobi
+    # sweep (0.6: 2.1 steps, 21 iterations), so the MC correction stays a
+    # multi-term Neumann estimate here
     "chosen": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
                # alias tables padded to the row stride: one 16-byte record per step
