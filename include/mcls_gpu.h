@@ -182,6 +182,14 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        the exact rounding error of every addition (FP32 walks, 12
  *        bytes); each is flushed into the FP64 tally, which every walk
  *        precision uses (FP32 walks included, tiled or not)
+ *   str "Birth Order"          ("Id" | "State"): with the "Random" source,
+ *        walk the histories in the order of their birth states instead of
+ *        their ids - each history's birth is drawn ahead of the walk from
+ *        its own stream and the (state, id) pairs are radix-sorted, so
+ *        every history keeps its id, stream and path and only the walking
+ *        order (the locality of the walks in flight) and the tally
+ *        summation order change; not with "Use Graph" (refused), and the
+ *        allocation-free solve walks in id order
  *   str "Residual Format"      ("Sliced ELL" | "CSR"): the residual
  *        PL (b - A PR y) reads a sliced-ELL copy of A (slices of 64 rows
  *        padded to their longest row, built with the operator when padding

@@ -97,6 +97,7 @@ struct Params {
   int tile_kind = mcls::kTileF64;  // tile partial sums: FP64, the walk's precision, or FP32 compensated
   bool residual_csr = false;        // "Residual Format" = "CSR": vector-CSR residual kernel
   int64_t walks_per_lane = 1;  // 2: two histories per lane (strided alias walks)
+  int birth_order = 0;           // "Birth Order" = "State": random births walked in state order
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
   int sampler = mcls::SAMPLER_CDF;
@@ -184,6 +185,10 @@ struct mcls_gpu_context {
   DBuf<double> x, b, r, rhs, part, scal;
   DBuf<double> t64;  // the tally (FP64 for every walk precision)
   DBuf<uint64_t> q, qcdf, ncnt, ncdf, scan_tmp;
+  // "Birth Order" = "State": the random births drawn ahead, before and after
+  // sorting by state, and the sort's scratch
+  DBuf<uint64_t> births, births_sorted;
+  DBuf<unsigned char> births_tmp;
   DBuf<unsigned long long> bmax, steps, queue;
   DBuf<int> flags;
   double* host_scal = nullptr;  // pinned, 16 doubles (see kHostM / kHostUnconv)
@@ -358,6 +363,7 @@ struct SolveBufs {
   double* t64;  // the tally (FP64 for every walk precision)
   uint32_t* guide = nullptr;  // births' guide table (nullable)
   int guide_bits = 0;
+  bool may_allocate = true;   // false: the allocation-free entry point's buffers
 };
 
 SolveBufs ctx_bufs(mcls_gpu_context* c) {
@@ -947,6 +953,36 @@ int build_source(mcls_gpu_context* c, const double* rhs, int64_t nh, const doubl
   return build_source(c, ctx_bufs(c), rhs, nh, mag);
 }
 
+// "Birth Order" = "State" (random source, adjoint walks): draw the births of
+// history ids [s0, s1) ahead of the walk and sort them by state, so that the
+// walk takes its histories in birth-state order - the locality a stratified
+// source has by construction - with every history's id, stream and path
+// unchanged.  Sets a.order / a.order_base; leaves them null otherwise.
+int prepare_birth_order(mcls_gpu_context* c, const SolveBufs& sb, int64_t s0, int64_t s1, WalkArgs& a) {
+  a.order = nullptr;
+  a.order_base = 0;
+  if (!c->prm.birth_order || c->prm.source != mcls::SRC_RANDOM || c->prm.forward || s1 <= s0) return MCLS_OK;
+  if (!sb.may_allocate) return MCLS_OK;  // the allocation-free solve walks in id order
+  if (c->prm.use_graph)
+    return fail(MCLS_ERR_UNSUPPORTED, "Birth Order 'State' is not available with Use Graph");
+  const int64_t count = s1 - s0;
+  if (count >= ((int64_t)1 << 32)) return fail(MCLS_ERR_UNSUPPORTED, "Birth Order 'State' needs fewer than 2^32 histories");
+  int bits = 1;
+  while (bits < 31 && ((int64_t)1 << bits) < (int64_t)c->n) ++bits;
+  hipStream_t s = c->stream;
+  MCLS_TRY(c->births.ensure((size_t)count, "birth order"));
+  MCLS_TRY(c->births_sorted.ensure((size_t)count, "birth order"));
+  size_t tb = 0;
+  HIP_TRY(mcls::launch_sort_births(c->births.p, c->births_sorted.p, count, bits, nullptr, &tb, s));
+  MCLS_TRY(c->births_tmp.ensure(tb > 0 ? tb : 1, "birth order sort"));
+  HIP_TRY(mcls::launch_random_births(s0, count, a.key, a.key_dev, sb.qcdf, sb.guide, sb.guide_bits, c->n,
+                                     c->births.p, s));
+  HIP_TRY(mcls::launch_sort_births(c->births.p, c->births_sorted.p, count, bits, c->births_tmp.p, &tb, s));
+  a.order = c->births_sorted.p;
+  a.order_base = s0;
+  return MCLS_OK;
+}
+
 // One MC solve of source rhs (device) into the tally; then x (+)= tally/nh.
 // accumulate: x += ..., else x = ...  `timed` brackets the source build and
 // the walk with events (not inside a graph capture); `key_dev` (nullable)
@@ -990,6 +1026,13 @@ int mc_solve_device(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs,
   }
   WalkArgs a = walk_args(c, sb, rhs, solve_key(c->prm.seed, sidx), first, end);
   a.key_dev = key_dev;
+  // ordered births: the whole source's, walked from position first (a pilot
+  // window then sees all of its states' births), or only the ids of a
+  // history-id window
+  if (wcount >= 0)
+    MCLS_TRY(prepare_birth_order(c, sb, first, end, a));
+  else
+    MCLS_TRY(prepare_birth_order(c, sb, 0, last, a));
   WalkConfig wc = walk_config(c, false, end - first);
   if (wc.tile != mcls::kTileOff) {
     const size_t elem = mcls::tile_entry_bytes(wc.tile, c->prm.precision == 32 ? 4 : 8);  // key + partial sum(s)
@@ -1798,6 +1841,10 @@ int mcls_gpu_set_param_str(mcls_gpu_context* c, const char* key, const char* v) 
     if (key_is(v, "Random")) p.source = mcls::SRC_RANDOM;
     else if (key_is(v, "Stratified")) p.source = mcls::SRC_STRATIFIED;
     else return fail(MCLS_ERR_INVALID_ARG, std::string("Source Sampling Type '") + v + "'");
+  } else if (key_is(key, "Birth Order")) {
+    if (key_is(v, "Id")) p.birth_order = 0;
+    else if (key_is(v, "State")) p.birth_order = 1;
+    else return fail(MCLS_ERR_INVALID_ARG, std::string("Birth Order '") + v + "'");
   } else if (key_is(key, "Estimator Type")) {
     if (key_is(v, "Collision")) p.estimator = mcls::EST_COLLISION;
     else if (key_is(v, "Expected Value")) p.estimator = mcls::EST_EXPECTED_VALUE;
@@ -1965,6 +2012,7 @@ int mcls_gpu_record_paths(mcls_gpu_context* c, const double* rhs, uint64_t solve
   if (c->prm.sampler == mcls::SAMPLER_ALIAS_STRIDED && !c->strided_valid)
     return fail(MCLS_ERR_STATE, "strided alias records not built");
   WalkArgs a = walk_args(c, ctx_bufs(c), c->rhs.p, solve_key(c->prm.seed, solve_index), first, first + count);
+  MCLS_TRY(prepare_birth_order(c, ctx_bufs(c), first, first + count, a));
   a.paths = dpaths.p;
   a.wfinal = dw.p;
   a.max_len = max_len;
@@ -2051,6 +2099,7 @@ int mcls_gpu_mc_solve_noalloc(mcls_gpu_context* c, const double* b, double* x) {
     return p;
   };
   SolveBufs sb{};
+  sb.may_allocate = false;  // nothing below may allocate (ordered births walk in id order)
   sb.queue = (unsigned long long*)take(mcls::kQueueWords * sizeof(unsigned long long));
   sb.steps = (unsigned long long*)take(sizeof(unsigned long long));
   sb.bmax = (unsigned long long*)take(sizeof(unsigned long long));

@@ -14,6 +14,7 @@
 //   src/MCLS_LinearProblem_impl.hpp:411-442        rp = PL (b - A x)
 //   tpetra/src/MCLS_TpetraHelpers.hpp:196-263      composite threshold filter
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cfloat>
 #include <climits>
@@ -590,6 +591,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
         if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
         if (!alive && rank < take) {
           hid = qnext + rank;
+          if (SRC == SRC_RANDOM && EST != EST_FORWARD && a.order) {
+            // births drawn ahead and sorted by state: position hid holds
+            // this history's id and the state its first draw selected
+            const uint64_t kv = a.order[hid - a.order_base];
+            hid = a.order_base + (int64_t)(kv & 0xFFFFFFFFull);
+            state = (int)(kv >> 32);
+          }
           rng = history_state(key, (uint64_t)hid);
           double w0;
           if (EST == EST_FORWARD) {
@@ -598,8 +606,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             fsum = 0.0;
             w0 = 1.0;
           } else if (SRC == SRC_RANDOM) {
-            const uint64_t t = mul_u53(xorshift64(rng), total);
-            state = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
+            const uint64_t t = mul_u53(xorshift64(rng), total);  // the birth draw
+            if (!a.order) state = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state] * (totald / (double)a.q[state]);
           } else if (grab.window) {
             state = grab.slo + bpos;
@@ -834,7 +842,12 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
         double bw0 = 0.0;
         if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
         if (!alive[j] && rank < take) {
-          const int64_t hid = qnext + rank;
+          int64_t hid = qnext + rank;
+          if (SRC == SRC_RANDOM && EST != EST_FORWARD && a.order) {
+            const uint64_t kv = a.order[hid - a.order_base];
+            hid = a.order_base + (int64_t)(kv & 0xFFFFFFFFull);
+            state[j] = (int)(kv >> 32);
+          }
           rng[j] = history_state(key, (uint64_t)hid);
           double w0;
           if (EST == EST_FORWARD) {
@@ -843,8 +856,8 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
             fsum[j] = 0.0;
             w0 = 1.0;
           } else if (SRC == SRC_RANDOM) {
-            const uint64_t t = mul_u53(xorshift64(rng[j]), total);
-            state[j] = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
+            const uint64_t t = mul_u53(xorshift64(rng[j]), total);  // the birth draw
+            if (!a.order) state[j] = birth_state(a.qcdf, a.guide, a.guide_bits, n, total, t);
             w0 = a.rhs[state[j]] * (totald / (double)a.q[state[j]]);
           } else if (grab.window) {
             state[j] = grab.slo + bpos;
@@ -2725,6 +2738,41 @@ hipError_t launch_build_guide(int n, const uint64_t* qcdf, int bits, uint32_t* g
   const int m = 1 << bits;
   k_build_guide<<<grid_for(m + 1, 256), 256, 0, s>>>(n, qcdf, bits, guide);
   return hipGetLastError();
+}
+
+// Random births drawn ahead of the walk (see launch_random_births): the
+// history's first draw, exactly as the walk makes it, and the state it
+// selects.
+__global__ void k_random_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* __restrict__ key_dev,
+                                const uint64_t* __restrict__ qcdf, const uint32_t* __restrict__ guide, int guide_bits,
+                                int n, uint64_t* __restrict__ out) {
+  const uint64_t total = qcdf[n - 1];
+  const uint64_t kk = key_dev ? *key_dev : key;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < count; p += (int64_t)gridDim.x * blockDim.x) {
+    int st = 0;
+    if (total) {
+      uint64_t rng = history_state(kk, (uint64_t)(s0 + p));
+      st = birth_state(qcdf, guide, guide_bits, n, total, mul_u53(xorshift64(rng), total));
+    }
+    out[p] = ((uint64_t)(uint32_t)st << 32) | (uint64_t)p;
+  }
+}
+
+hipError_t launch_random_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* key_dev,
+                                const uint64_t* qcdf, const uint32_t* guide, int guide_bits, int n, uint64_t* out,
+                                hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count >= ((int64_t)1 << 32) || n <= 0) return hipErrorInvalidValue;
+  k_random_births<<<grid_for(count, 256), 256, 0, s>>>(s0, count, key, key_dev, qcdf, guide, guide_bits, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int state_bits, void* temp,
+                              size_t* temp_bytes, hipStream_t s) {
+  if (count < 0 || count >= ((int64_t)1 << 32) || state_bits < 1 || state_bits > 31) return hipErrorInvalidValue;
+  // stable LSD passes over the state bits only: within a state the entries
+  // keep their id order
+  return rocprim::radix_sort_keys(temp, *temp_bytes, in, out, (uint32_t)count, 32u, 32u + (unsigned)state_bits, s);
 }
 
 constexpr int kSellRound = 16;  // entries per lane per round of loads

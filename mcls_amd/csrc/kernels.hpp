@@ -123,6 +123,12 @@ struct WalkArgs {
   // queue grab's first history id)
   const uint32_t* guide;
   int guide_bits;
+  // random source walked in birth-state order (nullable): entry p - order_base
+  // of the walk's positions holds (birth state << 32) | (id - order_base), as
+  // launch_random_births drew and launch_sort_births sorted them; the walk
+  // takes position p's history instead of history p (same id, same stream)
+  const uint64_t* order;
+  int64_t order_base;
 };
 
 // LDS tally tile: off, FP64 partial sums, partial sums in the walk's
@@ -149,6 +155,18 @@ struct WalkConfig {
 
 // Zeroes the queue counters, then launches the walk.
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s);
+
+// Births of a random source drawn ahead of its walk: out[p] = (state << 32) | p
+// for history id s0 + p, p in [0, count) (count < 2^32), the state the
+// history's first draw selects from qcdf (with the guide table when given),
+// its stream keyed by *key_dev when set, else key.
+hipError_t launch_random_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* key_dev,
+                                const uint64_t* qcdf, const uint32_t* guide, int guide_bits, int n, uint64_t* out,
+                                hipStream_t s);
+// Stable radix sort of those entries by state (bits 32 .. 32 + state_bits):
+// with temp == nullptr only *temp_bytes is set (the storage needed).
+hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int state_bits, void* temp,
+                              size_t* temp_bytes, hipStream_t s);
 
 // Importance features: sums[0] = sum |a_jj|, sums[1] = sum |v_j| (part holds
 // 2 * kResidualPartials doubles), then X (n x 8) from A's rows and v.
