@@ -185,7 +185,7 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *   str "Birth Order"          ("Id" | "State"): with the "Random" source,
  *        walk the histories in the order of their birth states instead of
  *        their ids - each history's birth is drawn ahead of the walk from
- *        its own stream and the (state, id) pairs are radix-sorted, so
+ *        its own stream and the histories are radix-sorted by it, so
  *        every history keeps its id, stream and path and only the walking
  *        order (the locality of the walks in flight) and the tally
  *        summation order change; not with "Use Graph" (refused), and the

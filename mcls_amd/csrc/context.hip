@@ -967,8 +967,13 @@ int prepare_birth_order(mcls_gpu_context* c, const SolveBufs& sb, int64_t s0, in
     return fail(MCLS_ERR_UNSUPPORTED, "Birth Order 'State' is not available with Use Graph");
   const int64_t count = s1 - s0;
   if (count >= ((int64_t)1 << 32)) return fail(MCLS_ERR_UNSUPPORTED, "Birth Order 'State' needs fewer than 2^32 histories");
+  // with a guide table the sort key is the draw's guide bucket (about four
+  // states each, buckets in state order) and the states are resolved after
+  // the sort; else the key is the state itself
   int bits = 1;
-  while (bits < 31 && ((int64_t)1 << bits) < (int64_t)c->n) ++bits;
+  if (sb.guide) bits = sb.guide_bits;
+  else
+    while (bits < 31 && ((int64_t)1 << bits) < (int64_t)c->n) ++bits;
   hipStream_t s = c->stream;
   MCLS_TRY(c->births.ensure((size_t)count, "birth order"));
   MCLS_TRY(c->births_sorted.ensure((size_t)count, "birth order"));
@@ -978,6 +983,9 @@ int prepare_birth_order(mcls_gpu_context* c, const SolveBufs& sb, int64_t s0, in
   HIP_TRY(mcls::launch_random_births(s0, count, a.key, a.key_dev, sb.qcdf, sb.guide, sb.guide_bits, c->n,
                                      c->births.p, s));
   HIP_TRY(mcls::launch_sort_births(c->births.p, c->births_sorted.p, count, bits, c->births_tmp.p, &tb, s));
+  if (sb.guide)
+    HIP_TRY(mcls::launch_resolve_births(s0, count, a.key, a.key_dev, sb.qcdf, sb.guide, sb.guide_bits, c->n,
+                                        c->births_sorted.p, s));
   a.order = c->births_sorted.p;
   a.order_base = s0;
   return MCLS_OK;

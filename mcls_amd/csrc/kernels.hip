@@ -2740,21 +2740,45 @@ hipError_t launch_build_guide(int n, const uint64_t* qcdf, int bits, uint32_t* g
   return hipGetLastError();
 }
 
-// Random births drawn ahead of the walk (see launch_random_births): the
-// history's first draw, exactly as the walk makes it, and the state it
-// selects.
+// Random births drawn ahead of the walk (see launch_random_births).  With a
+// guide table, pass 1 keys each history by its draw's guide bucket - pure
+// arithmetic on its stream, no memory read - so that after sorting by bucket
+// pass 2 resolves the birth states in bucket order, where the guide and
+// cumulative-weight reads of neighbouring histories share lines.  Without a
+// guide the state is drawn and searched directly.
+__device__ __forceinline__ uint64_t birth_draw(uint64_t kk, int64_t hid, uint64_t total) {
+  uint64_t rng = history_state(kk, (uint64_t)hid);
+  return mul_u53(xorshift64(rng), total);  // the walk's birth draw
+}
+
 __global__ void k_random_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* __restrict__ key_dev,
                                 const uint64_t* __restrict__ qcdf, const uint32_t* __restrict__ guide, int guide_bits,
                                 int n, uint64_t* __restrict__ out) {
   const uint64_t total = qcdf[n - 1];
   const uint64_t kk = key_dev ? *key_dev : key;
+  const int gsh = guide_shift(total, guide_bits);
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < count; p += (int64_t)gridDim.x * blockDim.x) {
-    int st = 0;
+    uint64_t hi = 0;
     if (total) {
-      uint64_t rng = history_state(kk, (uint64_t)(s0 + p));
-      st = birth_state(qcdf, guide, guide_bits, n, total, mul_u53(xorshift64(rng), total));
+      const uint64_t t = birth_draw(kk, s0 + p, total);
+      hi = guide ? (t >> gsh) : (uint64_t)birth_state(qcdf, guide, guide_bits, n, total, t);
     }
-    out[p] = ((uint64_t)(uint32_t)st << 32) | (uint64_t)p;
+    out[p] = (hi << 32) | (uint64_t)p;
+  }
+}
+
+// pass 2 (with a guide): the bucket in the high word becomes the state
+__global__ void k_resolve_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* __restrict__ key_dev,
+                                 const uint64_t* __restrict__ qcdf, const uint32_t* __restrict__ guide,
+                                 int guide_bits, int n, uint64_t* __restrict__ io) {
+  const uint64_t total = qcdf[n - 1];
+  const uint64_t kk = key_dev ? *key_dev : key;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t kv = io[i];
+    const uint64_t p = kv & 0xFFFFFFFFull;
+    int st = 0;
+    if (total) st = birth_state(qcdf, guide, guide_bits, n, total, birth_draw(kk, s0 + (int64_t)p, total));
+    io[i] = ((uint64_t)(uint32_t)st << 32) | p;
   }
 }
 
@@ -2767,12 +2791,21 @@ hipError_t launch_random_births(int64_t s0, int64_t count, uint64_t key, const u
   return hipGetLastError();
 }
 
-hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int state_bits, void* temp,
+hipError_t launch_resolve_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* key_dev,
+                                 const uint64_t* qcdf, const uint32_t* guide, int guide_bits, int n, uint64_t* io,
+                                 hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count >= ((int64_t)1 << 32) || n <= 0 || !guide) return hipErrorInvalidValue;
+  k_resolve_births<<<grid_for(count, 256), 256, 0, s>>>(s0, count, key, key_dev, qcdf, guide, guide_bits, n, io);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int key_bits, void* temp,
                               size_t* temp_bytes, hipStream_t s) {
-  if (count < 0 || count >= ((int64_t)1 << 32) || state_bits < 1 || state_bits > 31) return hipErrorInvalidValue;
-  // stable LSD passes over the state bits only: within a state the entries
-  // keep their id order
-  return rocprim::radix_sort_keys(temp, *temp_bytes, in, out, (uint32_t)count, 32u, 32u + (unsigned)state_bits, s);
+  if (count < 0 || count >= ((int64_t)1 << 32) || key_bits < 1 || key_bits > 31) return hipErrorInvalidValue;
+  // stable LSD passes over the key bits only: within a key the entries keep
+  // their id order
+  return rocprim::radix_sort_keys(temp, *temp_bytes, in, out, (uint32_t)count, 32u, 32u + (unsigned)key_bits, s);
 }
 
 constexpr int kSellRound = 16;  // entries per lane per round of loads

@@ -156,16 +156,23 @@ struct WalkConfig {
 // Zeroes the queue counters, then launches the walk.
 hipError_t launch_walk(const WalkArgs& a, const WalkConfig& c, hipStream_t s);
 
-// Births of a random source drawn ahead of its walk: out[p] = (state << 32) | p
-// for history id s0 + p, p in [0, count) (count < 2^32), the state the
-// history's first draw selects from qcdf (with the guide table when given),
-// its stream keyed by *key_dev when set, else key.
+// Births of a random source drawn ahead of its walk, for history ids s0 + p,
+// p in [0, count) (count < 2^32), streams keyed by *key_dev when set, else
+// key: out[p] = (hi << 32) | p with hi the guide bucket of the history's
+// first draw when a guide table is given (launch_resolve_births then turns
+// it into the state), else the state that draw selects from qcdf.
 hipError_t launch_random_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* key_dev,
                                 const uint64_t* qcdf, const uint32_t* guide, int guide_bits, int n, uint64_t* out,
                                 hipStream_t s);
-// Stable radix sort of those entries by state (bits 32 .. 32 + state_bits):
-// with temp == nullptr only *temp_bytes is set (the storage needed).
-hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int state_bits, void* temp,
+// The bucket in the high word of each entry replaced by the history's birth
+// state (in place; guide required), best run on entries sorted by bucket.
+hipError_t launch_resolve_births(int64_t s0, int64_t count, uint64_t key, const uint64_t* key_dev,
+                                 const uint64_t* qcdf, const uint32_t* guide, int guide_bits, int n, uint64_t* io,
+                                 hipStream_t s);
+// Stable radix sort of those entries by their high word's low key_bits bits
+// (bits 32 .. 32 + key_bits): with temp == nullptr only *temp_bytes is set
+// (the storage needed).
+hipError_t launch_sort_births(const uint64_t* in, uint64_t* out, int64_t count, int key_bits, void* temp,
                               size_t* temp_bytes, hipStream_t s);
 
 // Importance features: sums[0] = sum |a_jj|, sums[1] = sum |v_j| (part holds

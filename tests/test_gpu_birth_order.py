@@ -71,6 +71,35 @@ def test_ordered_births_paths_bitexact(octx, sampler, prec):
     assert (gp[:, 1] >= 0).mean() > 0.9
 
 
+@pytest.mark.parametrize("n", [7, 40])
+def test_ordered_births_skewed_source_bitexact(octx, n):
+    """A source whose weight sits on a few states across 16 decades, most
+    states unweighted: many guide buckets hold one state or none and one
+    holds many, and the births resolved after the bucket sort must still be
+    the states the walk's own first draw selects."""
+    ctx = octx
+    prob = capi.build_problem(capi.make_spec(capi.DIFFUSION3D_7PT, n, seed=5, p0=4))
+    L, nh = 40, 20000
+    _setup(ctx, prob, False, {"History Length": L, "Weight Cutoff": 1e-3, "Number of Histories": nh,
+                              "Random Seed": 5})
+    dom, dinv = _oracle(prob, False)
+    rng = np.random.default_rng(n)
+    rhs = np.zeros(prob.n)
+    hot = rng.choice(prob.n, size=min(prob.n, 37), replace=False)
+    rhs[hot] = np.logspace(-8, 8, hot.size) * rng.choice([-1.0, 1.0], size=hot.size)
+    rhs[hot[:5]] = 1e-300  # weights far below the rest round to no births at all
+    ctx.set_params({"Birth Order": "State"})
+    gp, gw = ctx.record_paths(rhs, 3, 0, nh, L)
+    mp = po.mc_params(nh, L, 1e-3, 64, po.EST_COLLISION, po.SRC_RANDOM, seed=5, sampler=po.SAMPLER_CDF)
+    op, ow = po.record_paths(dom, rhs, mp, 0, nh, L, solve_index=3)
+    assert np.array_equal(gp, op) and np.array_equal(gw, ow)
+    born = np.unique(gp[:, 0])
+    assert set(born.tolist()) <= set(hot.tolist())
+    xg = ctx.mc_solve_raw(rhs, 3)
+    xo, _ = po.mc_solve(dom, rhs, mp, solve_index=3)
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("tile", [0, 1024])
 @pytest.mark.parametrize("est", ["Collision", "Expected Value"])
 def test_ordered_births_tally_matches_oracle(octx, est, tile):
