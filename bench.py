@@ -299,6 +299,30 @@ def cpu_baseline(prob, w, plain_iterations=None, nh_sample=1 << 24):
                              f"(the plain form's count), MC transport only"}
 
 
+def residual_probe(w, prob, calls=10):
+    """r = PL (b - A x), point-Jacobi PL, on the workload's operator in the
+    bench's state order: `calls` calls per residual format (host b and x, so
+    the boundary's copies and gathers run too; a kernel trace separates the
+    residual kernels).  Returns {format: host ms per call}."""
+    ctx = capi.Context(0)
+    if w.get("bricks"):
+        ctx.set_state_order(capi.nested_brick_order(w["nx"], w["nx"], w["nx"], w["bricks"]))
+    ctx.set_problem(prob, "f32" if w["f32"] else "f64")
+    ctx.set_point_jacobi()
+    x = np.random.default_rng(1).standard_normal(prob.n)
+    out = {}
+    for fmt in ("Sliced ELL", "CSR"):
+        ctx.set_params({"Residual Format": fmt})
+        ctx.prec_residual(prob.b, x)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            _, nrm = ctx.prec_residual(prob.b, x)
+        out[fmt] = 1e3 * (time.perf_counter() - t0) / calls
+        log(f"residual probe: {fmt}: {calls} calls, {out[fmt]:.2f} ms per call with host vectors, |r| = {nrm:.6e}")
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +341,9 @@ def main():
                     "brick sizes) or the generator's x-fastest numbering")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
+    ap.add_argument("--residual-probe", action="store_true",
+                    help="only run the preconditioned residual 10 times in each residual format on the "
+                    "workload's operator in the bench's state order (residual kernel profiles)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -338,6 +365,9 @@ def main():
     log(f"generating {args.workload}")
     prob = make_problem(w)
     log(f"n={prob.n} nnz={prob.nnz}")
+    if args.residual_probe:
+        residual_probe(w, prob)
+        return
     ctx = capi.Context(0)
     # The context numbers the unknowns brick by brick (P A P^T; b and x stay
     # in the generator's numbering at the API): the walks born close together
