@@ -335,9 +335,11 @@ struct WaveTile {
   }
 
   // One tally v at state st.  The state may sit in any slot of its set, whose
-  // keys are read together; empty ones are claimed by compare-and-swap in slot
-  // order (lanes of one wave with the same state agree on the slot, so a
-  // state never holds two).  A full set sends the tally to the FP64 tally.
+  // keys are read together.  Slots are claimed by compare-and-swap, always the
+  // first one that read empty (so a set's held slots are a prefix of it and a
+  // state holds at most one: lanes of one wave with the same state agree on
+  // the slot); a lane that loses that slot to another state tries the next
+  // one once.  A full set, or a second loss, sends the tally to the FP64 tally.
   template <typename V>
   __device__ void add(double* __restrict__ tally, int st, V v) {
     const uint32_t s0 = direct ? ((uint32_t)st & ((uint32_t)(slots >> kTileWaysLog2) - 1u)) << kTileWaysLog2
@@ -352,19 +354,24 @@ struct WaveTile {
       kq[4 * q + 2] = (int)kk.z;
       kq[4 * q + 3] = (int)kk.w;
     }
-    int slot = -1;
+    // selects, not a first-match chain: st is in at most one slot
+    int slot = -1, empty = kTileWays;
 #pragma unroll
-    for (int j = 0; j < kTileWays; ++j)
-      if (slot < 0 && kq[j] == st) slot = (int)s0 + j;
-#pragma unroll
-    for (int j = 0; j < kTileWays; ++j) {
-      if (slot < 0 && kq[j] < 0) {
-        const int k = atomicCAS(&keys[s0 + j], -1, st);
-        if (k < 0 || k == st) slot = (int)s0 + j;
+    for (int j = kTileWays - 1; j >= 0; --j) {
+      slot = kq[j] == st ? j : slot;
+      empty = kq[j] < 0 ? j : empty;
+    }
+    if (slot < 0 && empty < kTileWays) {
+      int k = atomicCAS(&keys[s0 + empty], -1, st);
+      if (k < 0 || k == st) {
+        slot = empty;
+      } else if (empty + 1 < kTileWays) {
+        k = atomicCAS(&keys[s0 + empty + 1], -1, st);
+        if (k < 0 || k == st) slot = empty + 1;
       }
     }
     if (slot >= 0)
-      tile_add<TILE>(vals, comp, slot, (TAcc)v);
+      tile_add<TILE>(vals, comp, (int)s0 + slot, (TAcc)v);
     else
       tally_add(&tally[st], (double)v);
   }
@@ -467,8 +474,7 @@ __device__ __forceinline__ void grab_stratified(const WalkArgs& a, int n, int gs
 // (the first window state whose last id passes it, by binary lifting over the
 // lanes) and its weight.  Wave-uniform: every lane shuffles, the lanes not
 // being born ignore the result.
-__device__ __forceinline__ void window_birth(const StratWindow& g, int64_t qnext, int rank, int& bpos, double& bw0) {
-  const uint32_t keyb = (uint32_t)(qnext - g.qbase) + (uint32_t)rank;
+__device__ __forceinline__ void window_pos(const StratWindow& g, uint32_t keyb, int& bpos, double& bw0) {
   bpos = 0;
 #pragma unroll
   for (int step = 32; step >= 1; step >>= 1) {
@@ -477,6 +483,20 @@ __device__ __forceinline__ void window_birth(const StratWindow& g, int64_t qnext
   }
   bw0 = __shfl(g.win_w0, bpos);
 }
+__device__ __forceinline__ void window_birth(const StratWindow& g, int64_t qnext, int rank, int& bpos, double& bw0) {
+  window_pos(g, (uint32_t)(qnext - g.qbase) + (uint32_t)rank, bpos, bw0);
+}
+
+// What a birth at one position of a grab needs before its first step: the
+// history's stream state and, in a stratified window, its state's offset
+// from the window's first state and its birth weight.  A wave computes them
+// for 64 consecutive positions at once (lane L: position 64 blk + L), with
+// every lane busy, instead of per birth round with only the newborn lanes.
+struct BirthSlot {
+  uint64_t r = 0;
+  int b = 0;
+  double w = 0.0;
+};
 
 // Stratified birth outside a window: the first k in [g.slo, g.shi] with
 // ncdf[k] > hid, and its weight.
@@ -546,6 +566,19 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
   StratWindow grab;             // stratified: birth states of the current batch
   const int gsh = SRC == SRC_STRATIFIED && EST != EST_FORWARD ? guide_shift(total, a.guide_bits) : 0;
+  // The births of the current grab, 64 positions at a time: c0 holds block
+  // cblk of positions relative to gbase, c1 block cblk + 1 once c1ok
+  // (wave-uniform cursor; not with births drawn ahead, whose ids are
+  // permuted).
+  const bool cached = !(SRC == SRC_RANDOM && EST != EST_FORWARD && a.order);
+  BirthSlot c0, c1;
+  int64_t gbase = 0, cblk = -1;
+  bool c1ok = false;
+  auto fill = [&](BirthSlot& c, int64_t blk) {  // wave-uniform
+    const int64_t p = blk * 64 + lane;
+    c.r = history_state(key, (uint64_t)(gbase + p));
+    if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_pos(grab, (uint32_t)p, c.b, c.w);
+  };
   bool exhausted = false;       // wave-uniform
   bool alive = false;
   int64_t hid = 0;
@@ -581,14 +614,42 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           // that window of the count table, or none
           if (SRC == SRC_STRATIFIED && EST != EST_FORWARD)
             grab_stratified(a, n, gsh, totald, qnext, qend, lane, grab);
+          gbase = qnext;
+          cblk = -1;
+          c1ok = false;
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
         const int take = (int64_t)cnt < avail ? cnt : (int)avail;
         const int rank = __popcll(need & __lanemask_lt());
-        int bpos = 0;
-        double bw0 = 0.0;
-        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
+        // this round's positions [q, q + take) lie in blocks cblk and cblk + 1
+        const int64_t q = qnext - gbase;
+        BirthSlot bs;
+        if (cached) {
+          if ((q >> 6) != cblk) {  // the cursor entered the next block
+            if (c1ok)
+              c0 = c1;
+            else
+              fill(c0, q >> 6);
+            cblk = q >> 6;
+            c1ok = false;
+          }
+          if (q + take > (cblk + 1) * 64 && !c1ok) {
+            fill(c1, cblk + 1);
+            c1ok = true;
+          }
+          // position q + rank sits in lane (q + rank) mod 64, of block cblk
+          // when that lane is at or past q mod 64, else of block cblk + 1
+          const BirthSlot& m = lane >= (int)(q & 63) ? c0 : c1;
+          const int src = (int)((q + rank) & 63);
+          bs.r = __shfl(m.r, src);
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) {
+            bs.b = __shfl(m.b, src);
+            bs.w = __shfl(m.w, src);
+          }
+        }
+        int bpos = bs.b;
+        double bw0 = bs.w;
         if (!alive && rank < take) {
           hid = qnext + rank;
           if (SRC == SRC_RANDOM && EST != EST_FORWARD && a.order) {
@@ -598,7 +659,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
             hid = a.order_base + (int64_t)(kv & 0xFFFFFFFFull);
             state = (int)(kv >> 32);
           }
-          rng = history_state(key, (uint64_t)hid);
+          rng = cached ? bs.r : history_state(key, (uint64_t)hid);
           double w0;
           if (EST == EST_FORWARD) {
             state = (int)(hid / a.per_state);
