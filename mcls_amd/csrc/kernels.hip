@@ -47,6 +47,14 @@ __device__ __forceinline__ uint64_t xorshift64(uint64_t& x) {
   return x;
 }
 
+// The value x holds in the wave's first active lane, as a wave-uniform value
+// (kept in scalar registers, so the branches that depend on it are scalar).
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t history_state(uint64_t key, uint64_t hid) {
   uint64_t s = splitmix64(key ^ splitmix64(hid));
   return s ? s : kGolden;
@@ -602,7 +610,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
           const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
           unsigned long long b = 0;
           if (lane == leader) b = atomicAdd(ctr, (unsigned long long)a.batch);
-          b = __shfl(b, leader);
+          b = first_lane_u64(b);  // the leader is the first active lane
           qnext = sbeg + (int64_t)b;
           qend = qnext + a.batch < send ? qnext + a.batch : send;
           if (qnext >= send) {
@@ -885,7 +893,7 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
           const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
           unsigned long long b = 0;
           if (lane == leader) b = atomicAdd(ctr, (unsigned long long)a.batch);
-          b = __shfl(b, leader);
+          b = first_lane_u64(b);  // the leader is the first active lane
           qnext = sbeg + (int64_t)b;
           qend = qnext + a.batch < send ? qnext + a.batch : send;
           if (qnext >= send) {
