@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       bool absorbed = false;  // strided: the record says the row is empty
       if constexpr (STRIDED) {
         const uint32_t slot = __umulhi((uint32_t)(r >> 32), d.stride);
-        const uint4 sr = d.srec[(size_t)state * d.stride + slot];
+        const uint4 sr = d.srec[(uint64_t)(uint32_t)state * d.stride + slot];  // state >= 0
         tally_step();
         const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
         absorbed = sr.y == kStridedAbsorbing;
@@ -953,7 +953,7 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
       if (alive[j]) {
         r[j] = xorshift64(rng[j]);
         const uint32_t slot = __umulhi((uint32_t)(r[j] >> 32), stride);
-        sr[j] = d.srec[(size_t)state[j] * stride + slot];
+        sr[j] = d.srec[(uint64_t)(uint32_t)state[j] * stride + slot];  // state >= 0
       }
     }
 #pragma unroll
