@@ -175,6 +175,13 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        issues both of their record loads before waiting on either
  *        ("Alias Strided" FP32 walks, collision or forward estimator; other
  *        walks and path recording use one per lane)
+ *   i64 "Strided Record Bytes" (16) 8: "Alias Strided" walks (one history
+ *        per lane) read the same tables in 8-byte records {column, alias
+ *        slot and sign; threshold}, the row's W from the FP32 row array and,
+ *        when the draw takes the alias, the alias slot's own record for its
+ *        column: the same paths in half the table footprint (rows of at most
+ *        32 entries, fewer than 2^26 - 1 unknowns, else MCLS_ERR_UNSUPPORTED
+ *        at the next walk); the 16-byte tables are kept beside them
  *   str "Tally Tile Precision" ("FP64" | "Walk" | "Compensated"): the
  *        tile's partial sums in FP64 (12 bytes of LDS per entry), in the
  *        walk's precision (FP32 walks: 8 bytes, more resident waves per

@@ -97,6 +97,7 @@ struct Params {
   int tile_kind = mcls::kTileF64;  // tile partial sums: FP64, the walk's precision, or FP32 compensated
   bool residual_csr = false;        // "Residual Format" = "CSR": vector-CSR residual kernel
   int64_t walks_per_lane = 1;  // 2: two histories per lane (strided alias walks)
+  int64_t strided_bytes = 16;  // 8: strided alias walks read the 8-byte records
   int birth_order = 0;           // "Birth Order" = "State": random births walked in state order
   int64_t compute_criteria = 0;  // print the convergence criteria after each domain build
   int forward = 0;               // "Monte Carlo Type" = "Forward"
@@ -165,8 +166,10 @@ struct mcls_gpu_context {
   DBuf<mcls::Rec32> d_tgt64;
   bool alias_valid = false;
   DBuf<uint4> d_srec;        // strided alias records, n * d_stride
+  DBuf<uint2> d_srec8;       // their 8-byte form (when asked for)
   int d_stride = 0;
   bool strided_valid = false;
+  bool strided8 = false;     // d_srec8 holds the current tables
   DBuf<int> maxlen;          // longest domain row (device scalar)
   bool tgt32_valid = false, tgt64_valid = false;
   // learned proposal: importance g and its transition tables
@@ -705,11 +708,17 @@ int ensure_strided(mcls_gpu_context* c) {
     return fail(MCLS_ERR_UNSUPPORTED, "Transition Sampler 'Alias Strided': a domain row has " + std::to_string(longest) +
                                           " entries (at most 64)");
   const int stride = longest > 0 ? longest : 1;
+  const bool want8 = c->prm.strided_bytes == 8;
+  if (want8 && (stride > 32 || c->n >= (1 << 26) - 1))
+    return fail(MCLS_ERR_UNSUPPORTED, "Strided Record Bytes 8 needs rows of at most 32 entries and fewer than 2^26 - 1 "
+                                      "unknowns");
   MCLS_TRY(c->d_srec.ensure((size_t)c->n * (size_t)stride, "strided alias records"));
+  if (want8) MCLS_TRY(c->d_srec8.ensure((size_t)c->n * (size_t)stride, "strided alias records (8-byte)"));
   HIP_TRY(mcls::launch_pack_strided(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, stride,
-                                    c->d_srec.p, c->stream));
+                                    c->d_srec.p, want8 ? c->d_srec8.p : nullptr, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->d_stride = stride;
+  c->strided8 = want8;
   c->strided_valid = true;
   return MCLS_OK;
 }
@@ -894,6 +903,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, 
   a.d.hdr64 = c->d_hdr64.p;
   a.d.rec32 = c->d_rec32.p;
   a.d.srec = c->d_srec.p;
+  a.d.srec8 = c->strided8 && c->prm.strided_bytes == 8 ? c->d_srec8.p : nullptr;
   a.d.stride = (uint32_t)c->d_stride;
   a.d.slot64 = c->d_slot64.p;
   a.d.tgt32 = c->d_tgt32.p;
@@ -1800,6 +1810,10 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
   } else if (key_is(key, "Walks Per Lane")) {
     if (v != 1 && v != 2) return fail(MCLS_ERR_INVALID_ARG, "Walks Per Lane must be 1 or 2");
     p.walks_per_lane = v;
+  } else if (key_is(key, "Strided Record Bytes")) {
+    if (v != 16 && v != 8) return fail(MCLS_ERR_INVALID_ARG, "Strided Record Bytes must be 16 or 8");
+    if (v == 8 && !c->strided8) c->strided_valid = false;  // built with the tables on next use
+    p.strided_bytes = v;
   } else if (key_is(key, "Walk Batch")) {
     if (v < 64 || v > 4096 || v % 64 != 0)
       return fail(MCLS_ERR_INVALID_ARG, "Walk Batch must be a multiple of 64 in [64, 4096]");

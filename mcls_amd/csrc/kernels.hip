@@ -478,10 +478,10 @@ __device__ __forceinline__ void grab_stratified(const WalkArgs& a, int n, int gs
   }
 }
 
-// Window birth of history id qnext + rank: its state's offset from g.slo
-// (the first window state whose last id passes it, by binary lifting over the
-// lanes) and its weight.  Wave-uniform: every lane shuffles, the lanes not
-// being born ignore the result.
+// Window birth at position keyb of the grab (history id g.qbase + keyb): its
+// state's offset from g.slo (the first window state whose last id passes it,
+// by binary lifting over the lanes) and its weight.  Wave-uniform: every lane
+// shuffles, each with its own position.
 __device__ __forceinline__ void window_pos(const StratWindow& g, uint32_t keyb, int& bpos, double& bw0) {
   bpos = 0;
 #pragma unroll
@@ -490,9 +490,6 @@ __device__ __forceinline__ void window_pos(const StratWindow& g, uint32_t keyb, 
     if (cv <= keyb) bpos += step;
   }
   bw0 = __shfl(g.win_w0, bpos);
-}
-__device__ __forceinline__ void window_birth(const StratWindow& g, int64_t qnext, int rank, int& bpos, double& bw0) {
-  window_pos(g, (uint32_t)(qnext - g.qbase) + (uint32_t)rank, bpos, bw0);
 }
 
 // What a birth at one position of a grab needs before its first step: the
@@ -747,12 +744,26 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
       bool absorbed = false;  // strided: the record says the row is empty
       if constexpr (STRIDED) {
         const uint32_t slot = __umulhi((uint32_t)(r >> 32), d.stride);
-        const uint4 sr = d.srec[(uint64_t)(uint32_t)state * d.stride + slot];  // state >= 0
-        tally_step();
+        const uint64_t rbase = (uint64_t)(uint32_t)state * d.stride;  // state >= 0
         const float u2 = (float)((uint32_t)r >> 8) * 0x1.0p-24f;
-        absorbed = sr.y == kStridedAbsorbing;
-        cp = u2 < __uint_as_float(sr.x) ? sr.y : sr.z;
-        if (!absorbed) w = w * (Real)__uint_as_float(sr.w);
+        if (d.srec8) {
+          // 8-byte records: the row's W from W32 beside the record, and the
+          // alias slot's own record when the draw takes the alias
+          const uint2 s8 = d.srec8[rbase + slot];
+          const float Wi = d.W32[state];
+          tally_step();
+          absorbed = s8.x == kStrided8Absorbing;
+          uint32_t cx = s8.x;
+          if (!absorbed && !(u2 < __uint_as_float(s8.y))) cx = d.srec8[rbase + ((s8.x >> 26) & 31u)].x;
+          cp = (cx & 0x80000000u) | (cx & 0x03FFFFFFu);
+          if (!absorbed) w = w * (Real)Wi;
+        } else {
+          const uint4 sr = d.srec[rbase + slot];
+          tally_step();
+          absorbed = sr.y == kStridedAbsorbing;
+          cp = u2 < __uint_as_float(sr.x) ? sr.y : sr.z;
+          if (!absorbed) w = w * (Real)__uint_as_float(sr.w);
+        }
       } else if constexpr (FUSED) {
         // alias draw: slot from the high 32 bits, keep-or-alias from the low
         // 32; the record also holds both candidates' row headers
@@ -872,6 +883,16 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
   int64_t qnext = 0, qend = 0;
   StratWindow grab;
   const int gsh = SRC == SRC_STRATIFIED && EST != EST_FORWARD ? guide_shift(total, a.guide_bits) : 0;
+  // the births of the current grab, 64 positions at a time (as k_walk)
+  const bool cached = !(SRC == SRC_RANDOM && EST != EST_FORWARD && a.order);
+  BirthSlot c0, c1;
+  int64_t gbase = 0, cblk = -1;
+  bool c1ok = false;
+  auto fill = [&](BirthSlot& c, int64_t blk) {  // wave-uniform
+    const int64_t p = blk * 64 + lane;
+    c.r = history_state(key, (uint64_t)(gbase + p));
+    if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_pos(grab, (uint32_t)p, c.b, c.w);
+  };
   bool exhausted = false;
   bool alive[J] = {false, false};
   int state[J] = {0, 0};
@@ -902,14 +923,39 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
           }
           if (SRC == SRC_STRATIFIED && EST != EST_FORWARD)
             grab_stratified(a, n, gsh, totald, qnext, qend, lane, grab);
+          gbase = qnext;
+          cblk = -1;
+          c1ok = false;
         }
         const int64_t avail = qend - qnext;
         const int cnt = __popcll(need);
         const int take = (int64_t)cnt < avail ? cnt : (int)avail;
         const int rank = __popcll(need & __lanemask_lt());
-        int bpos = 0;
-        double bw0 = 0.0;
-        if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) window_birth(grab, qnext, rank, bpos, bw0);
+        const int64_t q = qnext - gbase;
+        BirthSlot bs;
+        if (cached) {
+          if ((q >> 6) != cblk) {
+            if (c1ok)
+              c0 = c1;
+            else
+              fill(c0, q >> 6);
+            cblk = q >> 6;
+            c1ok = false;
+          }
+          if (q + take > (cblk + 1) * 64 && !c1ok) {
+            fill(c1, cblk + 1);
+            c1ok = true;
+          }
+          const BirthSlot& m = lane >= (int)(q & 63) ? c0 : c1;
+          const int src = (int)((q + rank) & 63);
+          bs.r = __shfl(m.r, src);
+          if (SRC == SRC_STRATIFIED && EST != EST_FORWARD && grab.window) {
+            bs.b = __shfl(m.b, src);
+            bs.w = __shfl(m.w, src);
+          }
+        }
+        const int bpos = bs.b;
+        const double bw0 = bs.w;
         if (!alive[j] && rank < take) {
           int64_t hid = qnext + rank;
           if (SRC == SRC_RANDOM && EST != EST_FORWARD && a.order) {
@@ -917,7 +963,7 @@ __global__ __launch_bounds__(256) void k_walk2(WalkArgs a) {
             hid = a.order_base + (int64_t)(kv & 0xFFFFFFFFull);
             state[j] = (int)(kv >> 32);
           }
-          rng[j] = history_state(key, (uint64_t)hid);
+          rng[j] = cached ? bs.r : history_state(key, (uint64_t)hid);
           double w0;
           if (EST == EST_FORWARD) {
             state[j] = (int)(hid / a.per_state);
@@ -1633,14 +1679,18 @@ __global__ void k_max_row_len(int n, const int* __restrict__ d_rp, int* out) {
 // the row's first entry.  Absorbing rows get kStridedAbsorbing records.
 __global__ void k_pack_strided(int n, const int* __restrict__ d_rp, const uint32_t* __restrict__ col,
                                const double* __restrict__ h64, const double* __restrict__ W64,
-                               const float* __restrict__ W32, int stride, uint4* __restrict__ srec) {
+                               const float* __restrict__ W32, int stride, uint4* __restrict__ srec,
+                               uint2* __restrict__ srec8) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int beg = d_rp[i];
     const int z = d_rp[i + 1] - beg;
     uint4* out = srec + (size_t)i * (size_t)stride;
+    uint2* out8 = srec8 ? srec8 + (size_t)i * (size_t)stride : nullptr;
     if (z == 0) {
-      for (int k = 0; k < stride; ++k)
+      for (int k = 0; k < stride; ++k) {
         out[k] = make_uint4(0u, kStridedAbsorbing, kStridedAbsorbing, 0u);
+        if (out8) out8[k] = make_uint2(kStrided8Absorbing, 0u);
+      }
       continue;
     }
     const double W = W64[i];
@@ -1677,6 +1727,11 @@ __global__ void k_pack_strided(int n, const int* __restrict__ d_rp, const uint32
       const uint32_t other = col[beg + al[k]];
       const uint32_t self = k < z ? col[beg + k] : other;
       out[k] = make_uint4(__float_as_uint((float)thr[k]), self, other, wbits);
+      // 8-byte form: the entry's own column and sign, and its alias slot,
+      // whose record holds the alias column and sign
+      if (out8)
+        out8[k] = make_uint2((self & 0x80000000u) | ((uint32_t)al[k] << 26) | (self & 0x03FFFFFFu),
+                             __float_as_uint((float)thr[k]));
     }
   }
 }
@@ -2628,9 +2683,10 @@ hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
-                               const float* W32, int stride, uint4* srec, hipStream_t s) {
+                               const float* W32, int stride, uint4* srec, uint2* srec8, hipStream_t s) {
   if (stride < 1 || stride > 64) return hipErrorInvalidValue;
-  k_pack_strided<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, stride, srec);
+  if (srec8 && (stride > 32 || n >= (1 << 26) - 1)) return hipErrorInvalidValue;
+  k_pack_strided<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, stride, srec, srec8);
   return hipGetLastError();
 }
 hipError_t launch_gather_rows(int64_t n, int w, size_t elem, const int* ord, const void* src, void* dst,

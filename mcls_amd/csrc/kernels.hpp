@@ -68,6 +68,10 @@ struct DomainView {
   // strided alias tables: n * stride records {thr, col, alias col, W_i}
   const uint4* srec;
   uint32_t stride;        // the longest row's length (>= 1)
+  // the same tables in 8-byte records (nullable; n < 2^26 - 1, stride <= 32):
+  // {col (26 bits) | alias slot << 26 (5 bits) | sign << 31, thr}, the row's
+  // W read from W32 and the alias slot's own record read for its column
+  const uint2* srec8;
 };
 
 // EST_FORWARD: the forward method on the forward domain (rows of H): history
@@ -87,6 +91,7 @@ enum Sampler : int {
 };
 // marks the records of an absorbing row (no entries): the walk ends there
 constexpr uint32_t kStridedAbsorbing = 0xFFFFFFFFu;
+constexpr uint32_t kStrided8Absorbing = 0x7FFFFFFFu;  // 8-byte records' first word
 
 // Work queue of the walk kernel: one counter per XCD slot, 128 bytes apart.
 constexpr int kQueueSlots = 8;
@@ -214,8 +219,10 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
 hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s);
 // Strided alias tables: Vose over each row padded with zero-probability
 // slots to `stride` entries; records {thr (FP32), col, alias col, W32_i}.
+// srec8 (nullable, stride <= 32, columns below 2^26 - 1): the 8-byte form.
 hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64,
-                               const double* W64, const float* W32, int stride, uint4* srec, hipStream_t s);
+                               const double* W64, const float* W32, int stride, uint4* srec, uint2* srec8,
+                               hipStream_t s);
 
 // Target records (either output may be null).  FP64 records also carry the
 // entry's own CDF value, so one load serves the search and the transition.

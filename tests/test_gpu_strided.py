@@ -3,7 +3,9 @@ walks) on the GPU against the oracle: sampled paths and final weights
 bit-exact (adjoint and forward, random and stratified births, operators with
 absorbing rows and rows shorter than the stride), MC tallies to the FP32
 summation order, MCSA to a direct solve, the LDS tile and the state order on
-top, and the error paths."""
+top, and the error paths - for the 16-byte records and their 8-byte form
+("Strided Record Bytes" = 8: the same draws, the alias column read from the
+alias slot's own record and W from the row array)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -45,10 +47,11 @@ def _problem(kind):
     return p, po.Csr(p.rp, p.col, p.values()), "f64"
 
 
+@pytest.mark.parametrize("rec", [16, 8])
 @pytest.mark.parametrize("forward", [False, True])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
 @pytest.mark.parametrize("kind", ["s27", "poisson", "absorbing"])
-def test_paths_bitexact(kind, source, forward):
+def test_paths_bitexact(kind, source, forward, rec):
     prob, A, dt = _problem(kind)
     L, nh = 300, 6000
     dinv = po.jacobi_dinv(A)
@@ -59,7 +62,8 @@ def test_paths_bitexact(kind, source, forward):
         ctx.set_problem(prob, dt)
         ctx.set_point_jacobi()
         ctx.set_params(dict(BASE, **{"Source Sampling Type": source, "Number of Histories": nh, "Random Seed": 41,
-                                     "Monte Carlo Type": "Forward" if forward else "Adjoint"}))
+                                     "Monte Carlo Type": "Forward" if forward else "Adjoint",
+                                     "Strided Record Bytes": rec}))
         count = 2000
         gp, gw = ctx.record_paths(rhs, 7, 50, count, L)
     est = po.EST_FORWARD if forward else po.EST_COLLISION
@@ -71,10 +75,10 @@ def test_paths_bitexact(kind, source, forward):
     assert (gp[:, 1] >= 0).mean() > 0.5
 
 
-@pytest.mark.parametrize("walks", [1, 2])
+@pytest.mark.parametrize("walks,rec", [(1, 16), (2, 16), (1, 8)])
 @pytest.mark.parametrize("source", ["Random", "Stratified"])
 @pytest.mark.parametrize("tile", [0, 256])
-def test_tally_matches_oracle(source, tile, walks):
+def test_tally_matches_oracle(source, tile, walks, rec):
     prob, A, dt = _problem("s27")
     nh, L = 40000, 100
     dinv = po.jacobi_dinv(A)
@@ -85,7 +89,8 @@ def test_tally_matches_oracle(source, tile, walks):
         ctx.set_point_jacobi()
         ctx.set_params(dict(BASE, **{"Source Sampling Type": source, "Number of Histories": nh, "History Length": L,
                                      "Weight Cutoff": 1e-2, "Random Seed": 5, "Tally Tile": tile,
-                                     "Tally Tile Precision": "Walk", "Walks Per Lane": walks}))
+                                     "Tally Tile Precision": "Walk", "Walks Per Lane": walks,
+                                     "Strided Record Bytes": rec}))
         xg = ctx.mc_solve_raw(rhs, 3)
         st = ctx.stats()
     mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION,
@@ -121,9 +126,9 @@ def test_forward_two_walks_per_lane_matches_oracle():
         assert np.linalg.norm(x - xo) <= 2e-5 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("walks", [1, 2])
+@pytest.mark.parametrize("walks,rec", [(1, 16), (2, 16), (1, 8)])
 @pytest.mark.parametrize("ordered", [False, True])
-def test_mcsa_converges(ordered, walks):
+def test_mcsa_converges(ordered, walks, rec):
     prob, A, dt = _problem("s27")
     with capi.Context(0) as ctx:
         if ordered:
@@ -134,7 +139,7 @@ def test_mcsa_converges(ordered, walks):
                                      "Weight Cutoff": 1e-1, "History Length": 1000, "Random Seed": 9,
                                      "Convergence Tolerance": 1e-8, "Maximum Iterations": 60,
                                      "Tally Tile": 512, "Tally Tile Precision": "Walk",
-                                     "Walks Per Lane": walks}))
+                                     "Walks Per Lane": walks, "Strided Record Bytes": rec}))
         x = ctx.mcsa_solve(prob.b)
         st = ctx.stats()
     assert st.converged == 1 and st.achieved_tol <= 1e-8
@@ -158,6 +163,23 @@ def test_errors():
         assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
         ctx.set_params({"Estimator Type": "Collision"})
         assert np.all(np.isfinite(ctx.mc_solve(prob.b)))
+        with pytest.raises(capi.GpuError) as e:
+            ctx.set_params({"Strided Record Bytes": 12})
+        assert e.value.code == capi.MCLS_ERR_INVALID_ARG
+    # 8-byte records hold an alias slot of 5 bits: rows of more than 32 entries refuse them
+    n = 40
+    S = sp.csr_matrix(-0.01 * np.ones((n, n)) + np.diag(np.full(n, 2.0)))
+    p = capi.Problem(None, S.indptr.astype(np.int64), S.indices.astype(np.int32), S.data.astype(np.float64), None,
+                     np.ones(n))
+    with capi.Context(0) as ctx:
+        ctx.set_problem(p, "f64")
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Number of Histories": 1000, "Strided Record Bytes": 8}))
+        with pytest.raises(capi.GpuError) as e:
+            ctx.mc_solve(p.b)
+        assert e.value.code == capi.MCLS_ERR_UNSUPPORTED
+        ctx.set_params({"Strided Record Bytes": 16})
+        assert np.all(np.isfinite(ctx.mc_solve(p.b)))
         with pytest.raises(capi.GpuError):
             ctx.set_params({"Walks Per Lane": 3})
     # a domain row of more than 64 entries
