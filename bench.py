@@ -341,6 +341,9 @@ def main():
                     "brick sizes) or the generator's x-fastest numbering")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (HBM counter calibration)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="a solver parameter over the form's own (profiling variants; the line's config "
+                    "records it and no committed traffic record matches it)")
     ap.add_argument("--residual-probe", action="store_true",
                     help="only run the preconditioned residual 10 times in each residual format on the "
                     "workload's operator in the bench's state order (residual kernel profiles)")
@@ -384,6 +387,17 @@ def main():
     base = {"Convergence Tolerance": 1e-8, "Maximum Iterations": 60, "Number of Histories": w["histories"],
             "Random Seed": 2024}
     ctx.set_params(base)
+    overrides = {}
+    for kv in args.set:
+        k, _, v = kv.partition("=")
+        try:
+            overrides[k] = int(v)
+        except ValueError:
+            try:
+                overrides[k] = float(v)
+            except ValueError:
+                overrides[k] = v
+    FORMS[args.form].update(overrides)
     ctx.set_params(FORMS[args.form])
     setup = {}
     t0 = time.perf_counter()
@@ -451,7 +465,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {w['label']}", "form": args.form, "n": prob.n,
                    "nnz": prob.nnz, "histories_per_iteration": w["histories"],
-                   "state_order": order_label, "parallelism": "1 GPU"},
+                   "state_order": order_label, "parallelism": "1 GPU",
+                   **({"overrides": overrides} if overrides else {})},
         "time_to_tol_s": total / args.steps,
         "time_to_tol_host_io_s": host_io_s,
         "mcsa_iterations": iters,
@@ -460,8 +475,8 @@ def main():
         "walk_steps_per_s": steps / (walk_ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, args.form, steps_per_launch,
-                                                 ctx_tile, order_label),
+                     "traffic": None if overrides else measured_traffic(args.workload, args.form,
+                                                                        steps_per_launch, ctx_tile, order_label),
                      # not counted in this run (counters need their own profiler
                      # passes): the committed rocprofv3 FETCH_SIZE + WRITE_SIZE record
                      # of this form, calibrated on a copy kernel, when it matches
