@@ -78,9 +78,10 @@ FORMS = {
     # the forward method with the chosen form's walk (no per-step tally atomics)
     "forward": {"Walk Precision": "FP32", "Source Sampling Type": "Stratified",
                 "Estimator Type": "Collision", "Weight Cutoff": 1e-1, "History Length": 1000,
-                # the carried-header records: measured faster than the strided
-                # ones for this walk (0.227 s against 0.252 s on the 256^3 workload)
-                "Transition Sampler": "Alias", "Monte Carlo Type": "Forward"},
+                # the strided records: 0.151 s against 0.202 s for the
+                # carried-header ones on the 256^3 workload (19.5 against 26.7
+                # ms per walk launch)
+                "Transition Sampler": "Alias Strided", "Monte Carlo Type": "Forward"},
 }
 
 
