@@ -565,7 +565,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs a) {
   // state (FP32 alias records, CDF / learned target records); FP64 alias
   // walks load it per step.
   constexpr bool FUSED = SAMP == SAMPLER_ALIAS && sizeof(Real) == 4;
-  // strided alias records hold the current row's W: no header at all
+  // strided alias records hold the current row's W (the 8-byte form reads
+  // it from W32 beside the record): no header at all
   constexpr bool STRIDED = SAMP == SAMPLER_ALIAS_STRIDED;
   constexpr bool CARRY = (SAMP != SAMPLER_ALIAS || FUSED) && !STRIDED;
   int64_t qnext = 0, qend = 0;  // wave-uniform batch cursor
