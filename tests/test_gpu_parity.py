@@ -176,7 +176,7 @@ def test_random_births_skewed_source_bitexact(ctx, n):
     assert len(np.unique(gp[:, 0])) >= 3
 
 
-@pytest.mark.parametrize("batch", [64, 512])
+@pytest.mark.parametrize("batch", [64, 192, 512, 4096])
 @pytest.mark.parametrize("n", [40, 3000])
 def test_stratified_births_skewed_source_bitexact(ctx, n, batch):
     """Stratified births from a source whose magnitudes span twelve decades,

@@ -101,6 +101,34 @@ def test_tally_matches_oracle(source, tile, walks, rec):
     assert np.linalg.norm(xg - xo) <= 2e-5 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("walks", [1, 2])
+@pytest.mark.parametrize("batch", [64, 192, 4096])
+def test_tally_over_grab_sizes_matches_oracle(batch, walks):
+    """The births of a grab are computed 64 positions at a time and kept for
+    the current and the next block: grabs of one block, of three and of 64,
+    with one and two histories per lane, all tally what the oracle does."""
+    prob, A, dt = _problem("s27")
+    nh, L = 30000, 100
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    rhs[::7] *= -3.0
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Source Sampling Type": "Stratified", "Number of Histories": nh,
+                                     "History Length": L, "Weight Cutoff": 1e-2, "Random Seed": 12,
+                                     "Tally Tile": 512, "Tally Tile Precision": "FP64", "Walk Batch": batch,
+                                     "Walks Per Lane": walks}))
+        xg = ctx.mc_solve_raw(rhs, 4)
+        st = ctx.stats()
+    mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION, po.SRC_STRATIFIED, seed=12,
+                      sampler=po.SAMPLER_ALIAS_STRIDED)
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=4)
+    assert st.steps == steps
+    assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo)
+
+
 def test_forward_two_walks_per_lane_matches_oracle():
     """Two histories per lane on the forward method: the same per-state sums
     as the oracle (and as one history per lane) to the FP32 summation order."""
