@@ -181,7 +181,8 @@ int mcls_gpu_clear_right_prec(mcls_gpu_context* ctx);
  *        when the draw takes the alias, the alias slot's own record for its
  *        column: the same paths in half the table footprint (rows of at most
  *        32 entries, fewer than 2^26 - 1 unknowns, else MCLS_ERR_UNSUPPORTED
- *        at the next walk); the 16-byte tables are kept beside them
+ *        at the next walk); the 16-byte tables are then built only for
+ *        "Walks Per Lane" 2, which has no 8-byte form
  *   str "Tally Tile Precision" ("FP64" | "Walk" | "Compensated"): the
  *        tile's partial sums in FP64 (12 bytes of LDS per entry), in the
  *        walk's precision (FP32 walks: 8 bytes, more resident waves per

@@ -170,6 +170,7 @@ struct mcls_gpu_context {
   int d_stride = 0;
   bool strided_valid = false;
   bool strided8 = false;     // d_srec8 holds the current tables
+  bool strided16 = false;    // d_srec holds them (8-byte walks with one history per lane skip it)
   DBuf<int> maxlen;          // longest domain row (device scalar)
   bool tgt32_valid = false, tgt64_valid = false;
   // learned proposal: importance g and its transition tables
@@ -712,13 +713,21 @@ int ensure_strided(mcls_gpu_context* c) {
   if (want8 && (stride > 32 || c->n >= (1 << 26) - 1))
     return fail(MCLS_ERR_UNSUPPORTED, "Strided Record Bytes 8 needs rows of at most 32 entries and fewer than 2^26 - 1 "
                                       "unknowns");
-  MCLS_TRY(c->d_srec.ensure((size_t)c->n * (size_t)stride, "strided alias records"));
+  // the 16-byte records only where a walk reads them: the default, and two
+  // histories per lane (k_walk2 has no 8-byte form)
+  const bool want16 = !want8 || c->prm.walks_per_lane == 2;
+  if (want16)
+    MCLS_TRY(c->d_srec.ensure((size_t)c->n * (size_t)stride, "strided alias records"));
+  else
+    c->d_srec.release();
   if (want8) MCLS_TRY(c->d_srec8.ensure((size_t)c->n * (size_t)stride, "strided alias records (8-byte)"));
+  else c->d_srec8.release();
   HIP_TRY(mcls::launch_pack_strided(c->n, c->d_rp.p, c->d_col.p, c->d_h64.p, c->d_W64.p, c->d_W32.p, stride,
-                                    c->d_srec.p, want8 ? c->d_srec8.p : nullptr, c->stream));
+                                    want16 ? c->d_srec.p : nullptr, want8 ? c->d_srec8.p : nullptr, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->d_stride = stride;
   c->strided8 = want8;
+  c->strided16 = want16;
   c->strided_valid = true;
   return MCLS_OK;
 }
@@ -902,7 +911,7 @@ WalkArgs walk_args(mcls_gpu_context* c, const SolveBufs& sb, const double* rhs, 
   a.d.hdr32 = c->d_hdr32.p;
   a.d.hdr64 = c->d_hdr64.p;
   a.d.rec32 = c->d_rec32.p;
-  a.d.srec = c->d_srec.p;
+  a.d.srec = c->strided16 ? c->d_srec.p : nullptr;
   a.d.srec8 = c->strided8 && c->prm.strided_bytes == 8 ? c->d_srec8.p : nullptr;
   a.d.stride = (uint32_t)c->d_stride;
   a.d.slot64 = c->d_slot64.p;
@@ -1809,10 +1818,12 @@ int mcls_gpu_set_param_i64(mcls_gpu_context* c, const char* key, int64_t v) {
     if (c->has_matrix) MCLS_TRY(ensure_vectors(c));
   } else if (key_is(key, "Walks Per Lane")) {
     if (v != 1 && v != 2) return fail(MCLS_ERR_INVALID_ARG, "Walks Per Lane must be 1 or 2");
+    if (v == 2 && !c->strided16) c->strided_valid = false;  // its 16-byte records, built on next use
     p.walks_per_lane = v;
   } else if (key_is(key, "Strided Record Bytes")) {
     if (v != 16 && v != 8) return fail(MCLS_ERR_INVALID_ARG, "Strided Record Bytes must be 16 or 8");
-    if (v == 8 && !c->strided8) c->strided_valid = false;  // built with the tables on next use
+    if ((v == 8 && !c->strided8) || (v == 16 && !c->strided16))
+      c->strided_valid = false;  // built with the tables on next use
     p.strided_bytes = v;
   } else if (key_is(key, "Walk Batch")) {
     if (v < 64 || v > 4096 || v % 64 != 0)

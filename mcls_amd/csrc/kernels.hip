@@ -1685,11 +1685,11 @@ __global__ void k_pack_strided(int n, const int* __restrict__ d_rp, const uint32
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int beg = d_rp[i];
     const int z = d_rp[i + 1] - beg;
-    uint4* out = srec + (size_t)i * (size_t)stride;
+    uint4* out = srec ? srec + (size_t)i * (size_t)stride : nullptr;
     uint2* out8 = srec8 ? srec8 + (size_t)i * (size_t)stride : nullptr;
     if (z == 0) {
       for (int k = 0; k < stride; ++k) {
-        out[k] = make_uint4(0u, kStridedAbsorbing, kStridedAbsorbing, 0u);
+        if (out) out[k] = make_uint4(0u, kStridedAbsorbing, kStridedAbsorbing, 0u);
         if (out8) out8[k] = make_uint2(kStrided8Absorbing, 0u);
       }
       continue;
@@ -1727,7 +1727,7 @@ __global__ void k_pack_strided(int n, const int* __restrict__ d_rp, const uint32
     for (int k = 0; k < stride; ++k) {
       const uint32_t other = col[beg + al[k]];
       const uint32_t self = k < z ? col[beg + k] : other;
-      out[k] = make_uint4(__float_as_uint((float)thr[k]), self, other, wbits);
+      if (out) out[k] = make_uint4(__float_as_uint((float)thr[k]), self, other, wbits);
       // 8-byte form: the entry's own column and sign, and its alias slot,
       // whose record holds the alias column and sign
       if (out8)
@@ -2685,7 +2685,7 @@ hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s) {
 }
 hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64, const double* W64,
                                const float* W32, int stride, uint4* srec, uint2* srec8, hipStream_t s) {
-  if (stride < 1 || stride > 64) return hipErrorInvalidValue;
+  if (stride < 1 || stride > 64 || (!srec && !srec8)) return hipErrorInvalidValue;
   if (srec8 && (stride > 32 || n >= (1 << 26) - 1)) return hipErrorInvalidValue;
   k_pack_strided<<<grid_for(n, 256), 256, 0, s>>>(n, d_rp, col, h64, W64, W32, stride, srec, srec8);
   return hipGetLastError();

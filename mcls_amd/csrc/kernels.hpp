@@ -219,7 +219,8 @@ hipError_t launch_pack_rows(int n, const int* d_rp, const uint32_t* col, const d
 hipError_t launch_max_row_len(int n, const int* d_rp, int* out, hipStream_t s);
 // Strided alias tables: Vose over each row padded with zero-probability
 // slots to `stride` entries; records {thr (FP32), col, alias col, W32_i}.
-// srec8 (nullable, stride <= 32, columns below 2^26 - 1): the 8-byte form.
+// srec8 (nullable, stride <= 32, columns below 2^26 - 1): the 8-byte form;
+// srec nullable when srec8 is given.
 hipError_t launch_pack_strided(int n, const int* d_rp, const uint32_t* col, const double* h64,
                                const double* W64, const float* W32, int stride, uint4* srec, uint2* srec8,
                                hipStream_t s);

@@ -129,6 +129,32 @@ def test_tally_over_grab_sizes_matches_oracle(batch, walks):
     assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo)
 
 
+def test_record_forms_switch_and_rebuild():
+    """8-byte records alone (no 16-byte tables), then two histories per lane
+    (which builds the 16-byte ones), then back to 16-byte records with one
+    history per lane: every switch walks the same histories, whose tallies
+    match the oracle's."""
+    prob, A, dt = _problem("s27")
+    nh, L = 20000, 100
+    dinv = po.jacobi_dinv(A)
+    dom = po.build_domain(A, dinv)
+    rhs = prob.b * dinv
+    mp = po.mc_params(nh, L, 1e-2, 32, po.EST_COLLISION, po.SRC_STRATIFIED, seed=8,
+                      sampler=po.SAMPLER_ALIAS_STRIDED)
+    xo, steps = po.mc_solve(dom, rhs, mp, solve_index=6)
+    with capi.Context(0) as ctx:
+        ctx.set_problem(prob, dt)
+        ctx.set_point_jacobi()
+        ctx.set_params(dict(BASE, **{"Source Sampling Type": "Stratified", "Number of Histories": nh,
+                                     "History Length": L, "Weight Cutoff": 1e-2, "Random Seed": 8,
+                                     "Tally Tile": 256, "Tally Tile Precision": "FP64"}))
+        for rec, walks in ((8, 1), (8, 2), (16, 1), (8, 1)):
+            ctx.set_params({"Strided Record Bytes": rec, "Walks Per Lane": walks})
+            xg = ctx.mc_solve_raw(rhs, 6)
+            assert ctx.stats().steps == steps, (rec, walks)
+            assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo), (rec, walks)
+
+
 def test_forward_two_walks_per_lane_matches_oracle():
     """Two histories per lane on the forward method: the same per-state sums
     as the oracle (and as one history per lane) to the FP32 summation order."""
